@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the DiffPumas hot path on one MI355X.
+
+Legs
+  adjoint : deterministic CSDA flux + reverse-mode d flux/d rho over a
+            128x128x64 voxel field of 10 m cells, densities U[2000, 3000]
+            kg/m3 (seeded), 16384 rays (8 detectors x 2048 directions,
+            elevation 15-75 deg), 256 log-spaced energy nodes 1-1e4 GeV
+            (charge-summed, i.e. mu+ and mu-).  One step = forward + per-segment
+            Jacobian + dense vector-Jacobian product.  Unit: rays*voxels/s
+            (sum of segments over rays per step).
+  mc      : flat two-layer backward MC, 100 m StandardRock at 2650 kg/m3,
+            elevation 45 deg, E 1-100 GeV log-uniform, CSDA mode with the
+            density gradient surrogate; unit: events/s.
+
+Inputs are resident in HBM before the timed region.  Timing: W warm-up
+steps, then K steps between device synchronisations; kernel time of the
+dominant kernel from HIP events on the stream it runs on.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "diffpumas.jl_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def adjoint_workload(dp, physics, ctx, torch):
+    from workloads import cube_rays, energy_nodes, uniform_densities
+    geo = dp.CubeGeometry(640.0, 128, 128, 64, physics.get_material_index("StandardRock"),
+                          physics.get_material_index("Air"), width=1280.0)
+    sxy, dirs, cos_t = cube_rays(geo, 8, 2048, seed=2024)
+    paths = dp.trace_cube_paths(geo, sxy, dirs, cos_t, ctx=ctx)
+    rho = uniform_densities(geo.n_cells, seed=7)
+    ne, nw, nq = energy_nodes(256, 1.0, 1e4)
+    return geo, paths, rho, ne, nw, nq
+
+
+def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
+    import ctypes as C
+    geo, paths, rho, ne, nw, nq = adjoint_workload(dp, physics, ctx, torch)
+    dev = torch.device("cuda:0")
+    T = lambda a, dt: torch.as_tensor(a, dtype=dt).to(dev)
+    rp = T(paths.row_ptr, torch.int32)
+    cell = T(paths.cell, torch.int32)
+    length = T(paths.length, torch.float64)
+    cos_t = T(paths.cos_theta, torch.float64)
+    z_exit = T(paths.z_exit, torch.float64)
+    d_rho = T(rho, torch.float64)
+    d_ne, d_nw, d_nq = T(ne, torch.float64), T(nw, torch.float64), T(nq, torch.float64)
+    flux = torch.zeros(paths.n_rays, dtype=torch.float64, device=dev)
+    jac = torch.zeros(max(paths.n_segments, 1), dtype=torch.float64, device=dev)
+    jrem = torch.zeros(paths.n_rays, dtype=torch.float64, device=dev)
+    fbar = torch.ones(paths.n_rays, dtype=torch.float64, device=dev)
+    grad = torch.zeros(geo.n_cells, dtype=torch.float64, device=dev)
+    cp = L.Paths(paths.n_rays, paths.n_segments, paths.max_segments,
+                 C.cast(rp.data_ptr(), L.i32_p), C.cast(cell.data_ptr(), L.i32_p),
+                 C.cast(length.data_ptr(), L.dbl_p), C.cast(cos_t.data_ptr(), L.dbl_p),
+                 C.cast(z_exit.data_ptr(), L.dbl_p))
+    nodes = L.CsdaNodes(geo.rock_material, geo.air_material, geo.z_max, 3e4, 1.0, len(ne),
+                        C.cast(d_ne.data_ptr(), L.dbl_p), C.cast(d_nw.data_ptr(), L.dbl_p),
+                        C.cast(d_nq.data_ptr(), L.dbl_p))
+    h = physics.device_handle(ctx)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+    lib = ctx.lib
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        L.check(lib.dp_csda_cube_dev(ctx.h, h, C.byref(cp), C.byref(nodes), d_rho.data_ptr(),
+                                     geo.n_cells, flux.data_ptr(), jac.data_ptr(),
+                                     jrem.data_ptr()))
+        if ev is not None:
+            ev[1].record(stream)
+        L.check(lib.dp_csda_cube_vjp_dev(ctx.h, C.byref(cp), jac.data_ptr(), jrem.data_ptr(),
+                                         fbar.data_ptr(), geo.n_cells, grad.data_ptr()))
+
+    log(f"adjoint: rays={paths.n_rays} segments={paths.n_segments} max_seg={paths.max_segments}")
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / steps
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    ctx.set_stream(None)
+    units = paths.n_segments
+    # algorithmic bytes of the forward+Jacobian kernel: 28 B per ray*voxel
+    # (cell 4, length 8, density gather 8, Jacobian write 8) + 8 B per ray (flux)
+    alg_bytes = 28.0 * units + 8.0 * paths.n_rays
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    flux_h = flux.cpu().numpy()
+    return {
+        "value": units / (ms_step * 1e-3), "ms_per_step": ms_step, "kernel_ms": kern_ms,
+        "rays": paths.n_rays, "segments": units, "nodes": len(ne),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "flux_sum": float(flux_h.sum()), "grad_norm": float(grad.norm().item()),
+        "_problem": (geo, paths, rho, ne, nw, nq),
+    }
+
+
+def run_mc(dp, physics, ctx, torch, n_events, steps, warmup):
+    kw = dict(rock_thickness=100.0, rock_density=2650.0, elevation=45.0, energy_min=1.0,
+              energy_max=100.0, n_samples=n_events, compute_gradient=True, seed=1,
+              straggling=False, scattering=False, ctx=ctx)
+    for _ in range(warmup):
+        dp.run_backward_mc(physics, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms = 0.0
+    for _ in range(steps):
+        r = dp.run_backward_mc(physics, **kw)
+        kms += ctx.last_kernel_ms()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    return {"value": n_events / (ms * 1e-3), "ms_per_step": ms, "kernel_ms": kms / steps,
+            "events": n_events, "flux": r["flux"], "sigma": r["sigma"],
+            "gradient": r["gradient"]}
+
+
+def cpu_adjoint_baseline(physics, problem, budget_s=12.0):
+    """The oracle's reverse-mode CPU restatement on a bounded sample of rays,
+    all host cores (OpenMP)."""
+    import numpy as np
+    import _oracle as O
+    geo, paths, rho, ne, nw, nq = problem
+    threads = os.cpu_count() or 1
+    n = 16
+    while True:
+        idx = np.arange(0, paths.n_rays, max(1, paths.n_rays // n))[:n]
+        rp = [0]
+        cells, lens = [], []
+        for r in idx:
+            a, b = paths.row_ptr[r], paths.row_ptr[r + 1]
+            cells.append(paths.cell[a:b]); lens.append(paths.length[a:b])
+            rp.append(rp[-1] + (b - a))
+        cells = np.concatenate(cells); lens = np.concatenate(lens)
+        t0 = time.perf_counter()
+        O.cube_csda_reverse(physics, geo.rock_material, geo.air_material, np.array(rp), cells,
+                            lens, paths.cos_theta[idx], paths.z_exit[idx], geo.z_max, rho, ne, nw,
+                            nq, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= paths.n_rays:
+            return {"value": len(cells) / dt, "unit": "rays*voxels/s", "cores": threads,
+                    "kind": "port",
+                    "sample": f"{len(idx)} of {paths.n_rays} rays, all 256 nodes, "
+                              f"{len(cells)} ray*voxels, {dt:.2f} s"}
+        n *= 2
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mc-events", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(rays and events shard trivially; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import diffpumas_hip as dp
+    from diffpumas_hip import _lib as L
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    ctx = dp.Context(0)
+    physics = dp.create_physics_csda(("StandardRock", "Air"))
+    adj = run_adjoint(dp, L, physics, ctx, torch, a.steps, a.warmup)
+    log(f"adjoint: {adj['value']:.4g} rays*voxels/s, {adj['ms_per_step']:.3f} ms/step, "
+        f"kernel {adj['kernel_ms']:.3f} ms")
+    mc = run_mc(dp, physics, ctx, torch, a.mc_events, a.steps, a.warmup)
+    log(f"mc(csda): {mc['value']:.4g} events/s, kernel {mc['kernel_ms']:.3f} ms")
+    cpu = None if a.no_cpu else cpu_adjoint_baseline(physics, adj.pop("_problem"))
+    adj.pop("_problem", None)
+    out = {
+        "metric": "adjoint rays*voxels/s (deterministic CSDA flux + d flux/d rho), 1 GPU",
+        "value": adj["value"], "unit": "rays*voxels/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": adj["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "csda forward+adjoint, 128x128x64 voxels of 10 m, rho U[2000,3000], "
+                               "16384 rays (8 det x 2048 dirs), 256 nodes 1-1e4 GeV",
+                   "rays": adj["rays"], "segments": adj["segments"], "nodes": adj["nodes"]},
+        "roofline": adj["roofline"],
+        "cpu_baseline": cpu,
+        "kernel_ms": adj["kernel_ms"],
+        "mc_csda_events_per_s": mc["value"],
+        "mc_csda": {k: mc[k] for k in ("events", "flux", "sigma", "gradient", "kernel_ms",
+                                       "ms_per_step")},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

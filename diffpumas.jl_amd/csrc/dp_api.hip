@@ -1,0 +1,152 @@
+// Context lifecycle, error reporting, table upload.
+#include "dp_internal.hpp"
+#include <cstring>
+#include <mutex>
+
+namespace dph {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+int ensure(DevBuf& b, size_t bytes) {
+  if (bytes <= b.bytes && b.ptr) return DP_OK;
+  if (b.ptr) {
+    hipError_t e = hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return fail(DP_E_HIP, std::string("hipFree: ") + hipGetErrorString(e));
+  }
+  size_t want = align_up(bytes < 256 ? 256 : bytes, 256);
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    return fail(DP_E_HIP, std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+  }
+  b.bytes = want;
+  return DP_OK;
+}
+void release(DevBuf& b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+}  // namespace dph
+
+extern "C" {
+
+const char* dp_last_error(void) { return dph::g_last_error.c_str(); }
+
+int dp_create(dp_ctx* out, int device) {
+  if (!out) return dph::fail(DP_E_ARGUMENT, "dp_create: null output");
+  *out = nullptr;
+  int n = 0;
+  DP_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return dph::fail(DP_E_ARGUMENT, "dp_create: device " + std::to_string(device) + " not present");
+  DP_HIP(hipSetDevice(device));
+  dp_ctx c = new dp_ctx_s();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return dph::fail(DP_E_HIP, "hipGetDeviceProperties"); }
+  c->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return dph::fail(DP_E_HIP, "dp_create: stream/event creation failed");
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return DP_OK;
+}
+
+int dp_destroy(dp_ctx c) {
+  if (!c) return DP_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  dph::release(c->scratch);
+  dph::release(c->scratch2);
+  dph::release(c->counter);
+  dph::release(c->staging);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return DP_OK;
+}
+
+int dp_set_stream(dp_ctx c, void* stream) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_set_stream: null context");
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return DP_OK;
+}
+
+int dp_synchronize(dp_ctx c) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_synchronize: null context");
+  DP_HIP(hipStreamSynchronize(c->stream));
+  return DP_OK;
+}
+
+int dp_device_info(dp_ctx c, int32_t* n_cu, int32_t* wave) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_device_info: null context");
+  if (n_cu) *n_cu = c->n_cu;
+  if (wave) *wave = 64;
+  return DP_OK;
+}
+
+double dp_last_kernel_ms(dp_ctx c) { return c ? (double)c->last_ms : 0.0; }
+
+int dp_upload_tables(dp_ctx c, const dp_table_desc* d, dp_tables* out) {
+  if (!c || !d || !out) return dph::fail(DP_E_ARGUMENT, "dp_upload_tables: null argument");
+  *out = nullptr;
+  if (d->n_materials <= 0 || d->n_knots < 3 || !d->data || !d->scalars)
+    return dph::fail(DP_E_ARGUMENT, "dp_upload_tables: need >=1 material, >=3 knots, data and scalars");
+  if (!(d->mass > 0) || !(d->ctau > 0))
+    return dph::fail(DP_E_ARGUMENT, "dp_upload_tables: mass and ctau must be positive");
+  const int M = d->n_materials, K = d->n_knots;
+  // Energy grids must be strictly increasing and log_energies consistent.
+  for (int m = 0; m < M; ++m) {
+    const double* e = d->data + ((size_t)m * DP_T_NARRAYS + DP_T_ENERGY) * K;
+    for (int i = 1; i < K; ++i)
+      if (!(e[i] > e[i - 1]))
+        return dph::fail(DP_E_ARGUMENT, "dp_upload_tables: energy grid of material " +
+                                            std::to_string(m) + " is not strictly increasing");
+  }
+  DP_HIP(hipSetDevice(c->device));
+  dp_tables t = new dp_tables_s();
+  t->n_materials = M;
+  t->n_knots = K;
+  t->pow2 = dph::pow2_for(K);
+  t->mass = d->mass;
+  t->ctau = d->ctau;
+  t->host_scalars.assign(d->scalars, d->scalars + (size_t)M * 4);
+  for (int m = 0; m < M; ++m)
+    t->last_energy.push_back(d->data[((size_t)m * DP_T_NARRAYS + DP_T_ENERGY) * K + K - 1]);
+  size_t nb = (size_t)M * DP_T_NARRAYS * K * sizeof(double);
+  if (hipMalloc(&t->data, nb) != hipSuccess || hipMalloc(&t->scalars, (size_t)M * 4 * sizeof(double)) != hipSuccess) {
+    if (t->data) (void)hipFree(t->data);
+    delete t;
+    return dph::fail(DP_E_HIP, "dp_upload_tables: hipMalloc failed");
+  }
+  if (hipMemcpy(t->data, d->data, nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(t->scalars, d->scalars, (size_t)M * 4 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(t->data);
+    (void)hipFree(t->scalars);
+    delete t;
+    return dph::fail(DP_E_HIP, "dp_upload_tables: copy failed");
+  }
+  *out = t;
+  return DP_OK;
+}
+
+int dp_free_tables(dp_ctx c, dp_tables t) {
+  if (!t) return DP_OK;
+  if (c) (void)hipSetDevice(c->device);
+  if (t->data) (void)hipFree(t->data);
+  if (t->scalars) (void)hipFree(t->scalars);
+  delete t;
+  return DP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,487 @@
+// Per-voxel-density CSDA flux and its exact reverse-mode derivative.
+//
+// Work decomposition: one persistent block per ray at a time, taken from a
+// work queue; the block's lanes are the energy nodes of that ray (all nodes
+// walk the same segment list, so segment data and densities are wave-uniform
+// loads).  Rock and air tables (energy, log energy, range, stopping power)
+// are staged in LDS.
+//
+// Forward sweep per lane: the CSDA segment map of compute_flux_from_segments
+// (examples/diff_flux_cube.jl:386-413).  For segment v with E_in -> E_out
+//   c_v = dE_out/dE_in,  r_v = dE_out/drho_v,
+//   b_v = dlog w_v/dE_in, e_v = dlog w_v/dE_out      (w_v = S(E_out)/S(E_in) * decay)
+// are stored in the block's scratch slice.  The tail (remaining rock at
+// rho[0], air, flux_gccly) is differentiated in registers.  Reverse sweep:
+//   a* = T + e_v;  d flux/d rho_v = flux * a* * r_v;  T <- a* c_v + b_v
+// with T = d log(flux)/dE at the segment's exit.  These are the derivatives
+// of the executed discrete program (bracket slopes; zero where a lookup is
+// clamped), i.e. what Zygote returns on the reference.  Per-segment sums over
+// nodes use a fixed wave/LDS tree, so results are deterministic.
+#include "dp_internal.hpp"
+#include "dp_device.hpp"
+#include <hipcub/hipcub.hpp>
+
+using namespace dpk;
+
+namespace {
+
+struct CsdaArgs {
+  const double* tab;
+  int K, pow2;
+  int rock, air;
+  double mass, ctau;
+  int n_rays, max_seg;
+  const int32_t* row_ptr;
+  const int32_t* cell;
+  const double* len;
+  const double* cos_theta;
+  const double* z_exit;
+  double z_max, palt, scale;
+  int n_nodes;
+  const double* ne;
+  const double* nw;
+  const double* nq;
+  const double* dens;
+  double* flux;
+  double* jac;
+  double* jac_rem;
+  double* scratch;
+  int* queue;
+  int* err;
+};
+
+struct StepOut {
+  double E, lE;      // exit energy and its log
+  double ratio;      // S(E_out)/S(E_in)
+  double dec;        // decay factor of the segment
+  double c, b, e;    // partials (see header)
+  double k;          // dE_out/dX (X = grammage)
+  bool ok;
+};
+
+// One CSDA segment: range inversion at grammage X_in(E) + gram, then the
+// stopping-power ratio and the decay factor at the mean energy.
+__device__ __forceinline__ StepOut csda_segment(const double* sE, const double* sLE,
+                                                const double* sR, const double* sS, int K,
+                                                int pow2, double E, double lE, double gram,
+                                                double path, double mass, double ctau) {
+  StepOut o;
+  Lookup2 in = interp_e2(sE, sLE, sR, sS, K, pow2, E, lE);  // y0 = range, y1 = stopping
+  double Xf = in.y0 + gram;
+  Lookup inv = inverse_range(sE, sLE, sR, K, pow2, Xf);
+  o.E = inv.y;
+  o.ok = !(o.E >= 1e12) && isfinite(o.E);
+  o.lE = log(o.E);
+  Lookup so = interp_e(sE, sLE, sS, K, pow2, o.E, o.lE);
+  double dlog;
+  o.dec = decay_weight_dlog(path, (E + o.E) / 2.0, mass, ctau, &dlog);
+  o.ratio = so.y / in.y1;
+  o.k = inv.dy;
+  o.c = inv.dy * in.dy0;
+  o.b = -in.dy1 / in.y1 + 0.5 * dlog;
+  o.e = so.dy / so.y + 0.5 * dlog;
+  return o;
+}
+
+__global__ __launch_bounds__(256) void csda_cube_kernel(CsdaArgs a) {
+  extern __shared__ double lds[];
+  const int K = a.K;
+  const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = bd >> 6;
+  double* rE = lds;
+  double* rLE = lds + K;
+  double* rR = lds + 2 * K;
+  double* rS = lds + 3 * K;
+  double* aE = lds + 4 * K;
+  double* aLE = lds + 5 * K;
+  double* aR = lds + 6 * K;
+  double* aS = lds + 7 * K;
+  double* wsum = lds + 8 * K;               // [nw][max_seg]
+  double* wrem = wsum + (size_t)nw * a.max_seg;  // [nw]
+  double* wflux = wrem + nw;                // [nw]
+  __shared__ int s_ray;
+
+  {
+    const double* tr = a.tab + (size_t)a.rock * 19 * K;
+    const double* ta = a.tab + (size_t)a.air * 19 * K;
+    for (int i = tid; i < K; i += bd) {
+      rE[i] = tr[0 * K + i]; rLE[i] = tr[1 * K + i]; rR[i] = tr[3 * K + i]; rS[i] = tr[2 * K + i];
+      aE[i] = ta[0 * K + i]; aLE[i] = ta[1 * K + i]; aR[i] = ta[3 * K + i]; aS[i] = ta[2 * K + i];
+    }
+  }
+  double* sc = a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd;
+  const double rho0 = a.dens[0];
+
+  for (;;) {
+    if (tid == 0) s_ray = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int r = s_ray;
+    if (r >= a.n_rays) break;
+    const int b0 = a.row_ptr[r];
+    const int ns = a.row_ptr[r + 1] - b0;
+    if (ns < 0 || ns > a.max_seg) {
+      if (tid == 0) {
+        atomicOr(a.err, 1);
+        a.flux[r] = __builtin_nan("");
+        if (a.jac_rem) a.jac_rem[r] = __builtin_nan("");
+      }
+      __syncthreads();
+      continue;
+    }
+    for (int s = lane; s < ns; s += 64) wsum[wave * a.max_seg + s] = 0.0;
+    if (lane == 0) { wrem[wave] = 0.0; wflux[wave] = 0.0; }
+
+    const double cosr = a.cos_theta[r];
+    const double zex = a.z_exit[r];
+    const double cs = fmax(fabs(cosr), 0.01);
+    const bool has_rem = zex < a.z_max - EPS64;
+    const double rem = (a.z_max - zex) / cs;
+    const double Xair = air_grammage(a.z_max, a.palt, cosr);
+    const double path_air = (a.palt - a.z_max) / cs;
+
+    for (int n0 = 0; n0 < a.n_nodes; n0 += bd) {
+      const int n = n0 + tid;
+      const bool active = n < a.n_nodes;
+      double E = active ? a.ne[n] : a.ne[0];
+      double lE = log(E);
+      double W = 1.0;
+      bool ok = true;
+      // ------------------------------------------------ forward sweep
+      for (int s = 0; s < ns; ++s) {
+        const int cell = a.cell[b0 + s];
+        const double d = a.len[b0 + s];
+        const double rho = a.dens[cell];
+        StepOut o = csda_segment(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, a.ctau);
+        ok = ok && o.ok;
+        W = W * o.ratio;
+        W = W * o.dec;
+        double* p = sc + (size_t)s * 4 * bd + tid;
+        p[0] = o.c;
+        p[bd] = o.b;
+        p[2 * bd] = o.e;
+        p[3 * bd] = o.k * d;
+        E = o.E;
+        lE = o.lE;
+      }
+      StepOut orem;
+      orem.c = orem.b = orem.e = orem.k = 0.0;
+      if (has_rem) {
+        orem = csda_segment(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass, a.ctau);
+        ok = ok && orem.ok;
+        W = W * orem.ratio;
+        W = W * orem.dec;
+        E = orem.E;
+        lE = orem.lE;
+      }
+      StepOut oair = csda_segment(aE, aLE, aR, aS, K, a.pow2, E, lE, Xair, path_air, a.mass, a.ctau);
+      ok = ok && oair.ok;
+      W = W * oair.ratio;
+      W = W * oair.dec;
+      Dd phi = flux_gccly_d(cosr, oair.E, active ? a.nq[n] : 0.0);
+      const double fl = (active && ok) ? 2.0 * a.nw[n] * (W * phi.v) : 0.0;
+      // d log(tail)/dE at the exit of the cell list
+      double T = 0.0;
+      if (fl != 0.0) {
+        double as = phi.d / phi.v + oair.e;
+        T = as * oair.c + oair.b;
+      }
+      double fs = wave_sum(fl);
+      if (lane == 0) wflux[wave] += fs;
+      if (has_rem) {
+        double as = T + orem.e;
+        double g = fl != 0.0 ? fl * as * (orem.k * rem) : 0.0;
+        T = as * orem.c + orem.b;
+        double gs = wave_sum(g);
+        if (lane == 0) wrem[wave] += gs;
+      }
+      if (a.jac) {
+        // ---------------------------------------------- reverse sweep
+        for (int s = ns - 1; s >= 0; --s) {
+          const double* p = sc + (size_t)s * 4 * bd + tid;
+          double c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
+          double as = T + e;
+          double g = fl != 0.0 ? fl * as * rr : 0.0;
+          T = as * c + bb;
+          double gs = wave_sum(g);
+          if (lane == 0) wsum[wave * a.max_seg + s] += gs;
+        }
+      }
+    }
+    __syncthreads();
+    if (a.jac) {
+      for (int s = tid; s < ns; s += bd) {
+        double v = 0.0;
+        for (int w = 0; w < nw; ++w) v += wsum[w * a.max_seg + s];
+        a.jac[b0 + s] = a.scale * v;
+      }
+    }
+    if (tid == 0) {
+      double f = 0.0, g = 0.0;
+      for (int w = 0; w < nw; ++w) { f += wflux[w]; g += wrem[w]; }
+      a.flux[r] = a.scale * f;
+      if (a.jac_rem) a.jac_rem[r] = a.scale * g;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void reset_counters(int* q, int n) {
+  int i = threadIdx.x;
+  if (i < n) q[i] = 0;
+}
+
+// ------------------------------------------------------------------ VJP
+__global__ void vjp_keys(int n_rays, int n_seg, const int32_t* __restrict__ row_ptr,
+                         const int32_t* __restrict__ cell, int32_t* __restrict__ keys,
+                         int32_t* __restrict__ vals, int32_t* __restrict__ ray_of) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rays) return;
+  for (int s = row_ptr[r]; s < row_ptr[r + 1]; ++s) {
+    keys[s] = cell[s];
+    vals[s] = s;
+    ray_of[s] = r;
+  }
+  keys[n_seg + r] = 0;
+  vals[n_seg + r] = n_seg + r;
+  ray_of[n_seg + r] = r;
+}
+
+__device__ __forceinline__ int lower_bound_i(const int32_t* a, int n, int x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void vjp_gather(int n_cells, int n_seg, int n_ent, const int32_t* __restrict__ skeys,
+                           const int32_t* __restrict__ svals, const int32_t* __restrict__ ray_of,
+                           const double* __restrict__ jac, const double* __restrict__ jac_rem,
+                           const double* __restrict__ fbar, double* __restrict__ grad) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  int p = lower_bound_i(skeys, n_ent, c);
+  double acc = 0.0;
+  for (; p < n_ent && skeys[p] == c; ++p) {
+    int v = svals[p];
+    double j = v < n_seg ? jac[v] : (jac_rem ? jac_rem[v - n_seg] : 0.0);
+    acc += fbar[ray_of[v]] * j;
+  }
+  grad[c] = acc;
+}
+
+int check_paths(const dp_paths* p, const dp_csda_nodes* nd, int32_t n_cells, dp_tables t) {
+  if (!p || !nd || !t) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: null argument");
+  if (p->n_rays < 0 || p->n_segments < 0 || p->max_segments < 0)
+    return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: negative sizes");
+  if (nd->n_nodes <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: need at least one energy node");
+  if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: need at least one cell");
+  if (nd->rock_material < 0 || nd->rock_material >= t->n_materials || nd->air_material < 0 ||
+      nd->air_material >= t->n_materials)
+    return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: material index out of range");
+  if (!(nd->primary_altitude > nd->z_max))
+    return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: primary altitude must lie above z_max");
+  return DP_OK;
+}
+
+size_t lds_bytes(int K, int nw, int max_seg) {
+  return ((size_t)8 * K + (size_t)nw * max_seg + 2 * nw) * sizeof(double);
+}
+
+int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
+                const double* dens, double* flux, double* jac, double* jac_rem, int* qerr) {
+  if (p->n_rays == 0) return DP_OK;
+  const int bd = nd->n_nodes >= 256 ? 256 : ((nd->n_nodes + 63) / 64) * 64;
+  const int nw = bd / 64;
+  const int max_seg = p->max_segments > 0 ? p->max_segments : 1;
+  size_t lds = lds_bytes(t->n_knots, nw, max_seg);
+  if (lds > 160 * 1024)
+    return dph::fail(DP_E_CAPACITY, "dp_csda_cube: max_segments " + std::to_string(max_seg) +
+                                        " exceeds the LDS budget");
+  int occ = 0;
+  DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel, bd, lds));
+  if (occ < 1) occ = 1;
+  int grid = c->n_cu * occ;
+  if (grid > p->n_rays) grid = p->n_rays;
+  size_t scratch = (size_t)grid * max_seg * 4 * bd * sizeof(double);
+  int rc = dph::ensure(c->scratch, scratch);
+  if (rc) return rc;
+  CsdaArgs a;
+  a.tab = t->data; a.K = t->n_knots; a.pow2 = t->pow2;
+  a.rock = nd->rock_material; a.air = nd->air_material;
+  a.mass = t->mass; a.ctau = t->ctau;
+  a.n_rays = p->n_rays; a.max_seg = max_seg;
+  a.row_ptr = p->row_ptr; a.cell = p->cell; a.len = p->length;
+  a.cos_theta = p->cos_theta; a.z_exit = p->z_exit;
+  a.z_max = nd->z_max; a.palt = nd->primary_altitude; a.scale = nd->scale;
+  a.n_nodes = nd->n_nodes; a.ne = nd->node_energy; a.nw = nd->node_weight; a.nq = nd->node_charge;
+  a.dens = dens; a.flux = flux; a.jac = jac; a.jac_rem = jac_rem;
+  a.scratch = (double*)c->scratch.ptr;
+  a.queue = qerr; a.err = qerr + 1;
+  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, qerr, 2);
+  (void)hipEventRecord(c->ev0, c->stream);
+  hipLaunchKernelGGL(csda_cube_kernel, dim3(grid), dim3(bd), lds, c->stream, a);
+  (void)hipEventRecord(c->ev1, c->stream);
+  DP_HIP(hipGetLastError());
+  return DP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dp_csda_cube_dev(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
+                     const double* dens, int32_t n_cells, double* flux, double* jac,
+                     double* jac_rem) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_dev: null context");
+  int rc = check_paths(p, nd, n_cells, t);
+  if (rc) return rc;
+  DP_HIP(hipSetDevice(c->device));
+  rc = dph::ensure(c->counter, 256);
+  if (rc) return rc;
+  return launch_csda(c, t, p, nd, dens, flux, jac, jac_rem, (int*)c->counter.ptr);
+}
+
+int dp_csda_cube(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
+                 const double* dens, int32_t n_cells, double* flux, double* jac,
+                 double* jac_rem) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: null context");
+  int rc = check_paths(p, nd, n_cells, t);
+  if (rc) return rc;
+  const int nr = p->n_rays, ns = p->n_segments, nn = nd->n_nodes;
+  if (nr == 0) return DP_OK;
+  if (p->row_ptr[0] != 0 || p->row_ptr[nr] != ns)
+    return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: row_ptr must start at 0 and end at n_segments");
+  int mx = 0;
+  for (int r = 0; r < nr; ++r) {
+    int k = p->row_ptr[r + 1] - p->row_ptr[r];
+    if (k < 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: row_ptr must be non-decreasing");
+    mx = k > mx ? k : mx;
+  }
+  for (int s = 0; s < ns; ++s)
+    if (p->cell[s] < 0 || p->cell[s] >= n_cells)
+      return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: cell index out of range at segment " + std::to_string(s));
+  for (int i = 0; i < nn; ++i)
+    if (!(nd->node_energy[i] > 0))
+      return dph::fail(DP_E_ARGUMENT, "dp_csda_cube: node energies must be positive");
+  DP_HIP(hipSetDevice(c->device));
+  size_t o_rp = 0;
+  size_t o_cell = dph::align_up(o_rp + 4 * (size_t)(nr + 1), 256);
+  size_t o_len = dph::align_up(o_cell + 4 * (size_t)ns, 256);
+  size_t o_cos = dph::align_up(o_len + 8 * (size_t)ns, 256);
+  size_t o_zex = dph::align_up(o_cos + 8 * (size_t)nr, 256);
+  size_t o_ne = dph::align_up(o_zex + 8 * (size_t)nr, 256);
+  size_t o_nw = dph::align_up(o_ne + 8 * (size_t)nn, 256);
+  size_t o_nq = dph::align_up(o_nw + 8 * (size_t)nn, 256);
+  size_t o_den = dph::align_up(o_nq + 8 * (size_t)nn, 256);
+  size_t o_flux = dph::align_up(o_den + 8 * (size_t)n_cells, 256);
+  size_t o_jac = dph::align_up(o_flux + 8 * (size_t)nr, 256);
+  size_t o_jr = dph::align_up(o_jac + 8 * (size_t)ns, 256);
+  size_t o_q = dph::align_up(o_jr + 8 * (size_t)nr, 256);
+  size_t total = o_q + 256;
+  rc = dph::ensure(c->staging, total);
+  if (rc) return rc;
+  char* b = (char*)c->staging.ptr;
+  hipStream_t st = c->stream;
+  DP_HIP(hipMemcpyAsync(b + o_rp, p->row_ptr, 4 * (size_t)(nr + 1), hipMemcpyHostToDevice, st));
+  if (ns) DP_HIP(hipMemcpyAsync(b + o_cell, p->cell, 4 * (size_t)ns, hipMemcpyHostToDevice, st));
+  if (ns) DP_HIP(hipMemcpyAsync(b + o_len, p->length, 8 * (size_t)ns, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_cos, p->cos_theta, 8 * (size_t)nr, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_zex, p->z_exit, 8 * (size_t)nr, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_ne, nd->node_energy, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_nw, nd->node_weight, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_nq, nd->node_charge, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_den, dens, 8 * (size_t)n_cells, hipMemcpyHostToDevice, st));
+  dp_paths dp = *p;
+  dp.max_segments = mx;
+  dp.row_ptr = (int32_t*)(b + o_rp); dp.cell = (int32_t*)(b + o_cell); dp.length = (double*)(b + o_len);
+  dp.cos_theta = (double*)(b + o_cos); dp.z_exit = (double*)(b + o_zex);
+  dp_csda_nodes dn = *nd;
+  dn.node_energy = (double*)(b + o_ne); dn.node_weight = (double*)(b + o_nw); dn.node_charge = (double*)(b + o_nq);
+  int* qerr = (int*)(b + o_q);
+  rc = launch_csda(c, t, &dp, &dn, (double*)(b + o_den), (double*)(b + o_flux),
+                   jac ? (double*)(b + o_jac) : nullptr, jac_rem ? (double*)(b + o_jr) : nullptr, qerr);
+  if (rc) return rc;
+  int herr[2] = {0, 0};
+  DP_HIP(hipMemcpyAsync(flux, b + o_flux, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));
+  if (jac && ns) DP_HIP(hipMemcpyAsync(jac, b + o_jac, 8 * (size_t)ns, hipMemcpyDeviceToHost, st));
+  if (jac_rem) DP_HIP(hipMemcpyAsync(jac_rem, b + o_jr, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipMemcpyAsync(herr, qerr, 8, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipStreamSynchronize(st));
+  (void)hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1);
+  if (herr[1]) return dph::fail(DP_E_CAPACITY, "dp_csda_cube: a ray exceeded max_segments");
+  return DP_OK;
+}
+
+int dp_csda_cube_vjp_dev(dp_ctx c, const dp_paths* p, const double* jac, const double* jac_rem,
+                         const double* fbar, int32_t n_cells, double* grad) {
+  if (!c || !p || !jac || !fbar || !grad) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: null argument");
+  if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: need at least one cell");
+  DP_HIP(hipSetDevice(c->device));
+  const int nr = p->n_rays, ns = p->n_segments, ne = ns + nr;
+  if (nr == 0) {
+    DP_HIP(hipMemsetAsync(grad, 0, 8 * (size_t)n_cells, c->stream));
+    return DP_OK;
+  }
+  int bits = 1;
+  while ((1ll << bits) < (long long)n_cells) ++bits;
+  size_t tmp = 0;
+  DP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (int32_t*)nullptr, (int32_t*)nullptr,
+                                            (int32_t*)nullptr, (int32_t*)nullptr, ne, 0, bits,
+                                            c->stream));
+  size_t o_k = 0, o_v = dph::align_up(4 * (size_t)ne, 256);
+  size_t o_sk = dph::align_up(o_v + 4 * (size_t)ne, 256);
+  size_t o_sv = dph::align_up(o_sk + 4 * (size_t)ne, 256);
+  size_t o_ro = dph::align_up(o_sv + 4 * (size_t)ne, 256);
+  size_t o_t = dph::align_up(o_ro + 4 * (size_t)ne, 256);
+  int rc = dph::ensure(c->scratch2, o_t + tmp + 256);
+  if (rc) return rc;
+  char* b = (char*)c->scratch2.ptr;
+  int32_t *k = (int32_t*)(b + o_k), *v = (int32_t*)(b + o_v), *sk = (int32_t*)(b + o_sk),
+          *sv = (int32_t*)(b + o_sv), *ro = (int32_t*)(b + o_ro);
+  hipLaunchKernelGGL(vjp_keys, dim3((nr + 255) / 256), dim3(256), 0, c->stream, nr, ns,
+                     p->row_ptr, p->cell, k, v, ro);
+  DP_HIP(hipGetLastError());
+  DP_HIP(hipcub::DeviceRadixSort::SortPairs(b + o_t, tmp, k, sk, v, sv, ne, 0, bits, c->stream));
+  hipLaunchKernelGGL(vjp_gather, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
+                     ns, ne, sk, sv, ro, jac, jac_rem, fbar, grad);
+  DP_HIP(hipGetLastError());
+  return DP_OK;
+}
+
+int dp_csda_cube_vjp(dp_ctx c, const dp_paths* p, const double* jac, const double* jac_rem,
+                     const double* fbar, int32_t n_cells, double* grad) {
+  if (!c || !p || !jac || !fbar || !grad) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: null argument");
+  if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: need at least one cell");
+  const int nr = p->n_rays, ns = p->n_segments;
+  for (int s = 0; s < ns; ++s)
+    if (p->cell[s] < 0 || p->cell[s] >= n_cells)
+      return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: cell index out of range");
+  DP_HIP(hipSetDevice(c->device));
+  size_t o_rp = 0, o_cell = dph::align_up(4 * (size_t)(nr + 1), 256);
+  size_t o_j = dph::align_up(o_cell + 4 * (size_t)ns, 256);
+  size_t o_jr = dph::align_up(o_j + 8 * (size_t)ns, 256);
+  size_t o_fb = dph::align_up(o_jr + 8 * (size_t)nr, 256);
+  size_t o_g = dph::align_up(o_fb + 8 * (size_t)nr, 256);
+  int rc = dph::ensure(c->staging, o_g + 8 * (size_t)n_cells + 256);
+  if (rc) return rc;
+  char* b = (char*)c->staging.ptr;
+  hipStream_t st = c->stream;
+  DP_HIP(hipMemcpyAsync(b + o_rp, p->row_ptr, 4 * (size_t)(nr + 1), hipMemcpyHostToDevice, st));
+  if (ns) DP_HIP(hipMemcpyAsync(b + o_cell, p->cell, 4 * (size_t)ns, hipMemcpyHostToDevice, st));
+  if (ns) DP_HIP(hipMemcpyAsync(b + o_j, jac, 8 * (size_t)ns, hipMemcpyHostToDevice, st));
+  if (jac_rem && nr) DP_HIP(hipMemcpyAsync(b + o_jr, jac_rem, 8 * (size_t)nr, hipMemcpyHostToDevice, st));
+  if (nr) DP_HIP(hipMemcpyAsync(b + o_fb, fbar, 8 * (size_t)nr, hipMemcpyHostToDevice, st));
+  dp_paths dp = *p;
+  dp.row_ptr = (int32_t*)(b + o_rp);
+  dp.cell = (int32_t*)(b + o_cell);
+  rc = dp_csda_cube_vjp_dev(c, &dp, (double*)(b + o_j), jac_rem ? (double*)(b + o_jr) : nullptr,
+                            (double*)(b + o_fb), n_cells, (double*)(b + o_g));
+  if (rc) return rc;
+  DP_HIP(hipMemcpyAsync(grad, b + o_g, 8 * (size_t)n_cells, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipStreamSynchronize(st));
+  return DP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/diffpumas_hip.h"
+
+namespace dph {
+
+// Per-thread error message of the last failing call.
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define DP_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return dph::fail(DP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Growable device scratch buffer owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+int ensure(DevBuf& b, size_t bytes);
+void release(DevBuf& b);
+
+}  // namespace dph
+
+struct dp_tables_s {
+  int32_t n_materials = 0, n_knots = 0, pow2 = 0;
+  double mass = 0, ctau = 0;
+  double* data = nullptr;      // device: M * NARR * K
+  double* scalars = nullptr;   // device: M * 4
+  std::vector<double> host_scalars;  // M * 4
+  std::vector<double> last_energy;   // per material, top of the table
+};
+
+struct dp_ctx_s {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int n_cu = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.f;
+  dph::DevBuf scratch;       // kernel scratch (partials, queues)
+  dph::DevBuf scratch2;      // second scratch (sort buffers)
+  dph::DevBuf counter;       // small: work-queue counters
+  dph::DevBuf staging;       // host-API staging of inputs/outputs
+};
+
+namespace dph {
+// Largest power of two <= n - 2 (bracket search start).
+inline int pow2_for(int n) {
+  int p = 1;
+  while (p * 2 <= n - 2) p *= 2;
+  return n - 2 >= 1 ? p : 0;
+}
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+}  // namespace dph

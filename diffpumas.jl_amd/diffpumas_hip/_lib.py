@@ -1,0 +1,220 @@
+"""ctypes binding of libdiffpumas_hip.so (the C ABI in include/diffpumas_hip.h).
+
+The library must be built (diffpumas.jl_amd/build_lib.py); there is no
+fallback path: every call goes to the HIP kernels, and a missing library or
+GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libdiffpumas_hip.so")
+
+DP_OK = 0
+DP_E_ARGUMENT = -1
+DP_E_DOMAIN = -2
+DP_E_EXHAUSTED = -3
+DP_E_HIP = -4
+DP_E_CAPACITY = -5
+
+T_NARRAYS = 19
+(T_ENERGY, T_LOG_ENERGY, T_CSDA_STOP, T_CSDA_RANGE, T_CSDA_TIME, T_MIXED_STOP, T_MIXED_RANGE,
+ T_STRAGGLING, T_IONIZATION, T_BREMS, T_PAIR, T_PHOTO, T_CROSS_SECTION, T_TRANSPORT_PATH,
+ T_ELASTIC_PATH, T_ELASTIC_CUTOFF, T_MAGNETIC_ROTATION, T_SCREENING, T_SPIN_FACTOR) = range(19)
+
+
+class DomainError(ValueError):
+    """Julia DomainError: an energy outside a prepared table."""
+
+
+class ArgumentError(ValueError):
+    """Julia ArgumentError: invalid mode, density, charge or size."""
+
+
+class TransportError(RuntimeError):
+    """Julia ErrorException from the transport (event/step caps)."""
+
+
+class HipError(RuntimeError):
+    """HIP runtime failure."""
+
+
+class CapacityError(RuntimeError):
+    """An output or scratch capacity was exceeded."""
+
+
+_ERRORS = {DP_E_ARGUMENT: ArgumentError, DP_E_DOMAIN: DomainError,
+           DP_E_EXHAUSTED: TransportError, DP_E_HIP: HipError, DP_E_CAPACITY: CapacityError}
+
+dbl_p = C.POINTER(C.c_double)
+i32_p = C.POINTER(C.c_int32)
+
+
+class TableDesc(C.Structure):
+    _fields_ = [("n_materials", C.c_int32), ("n_knots", C.c_int32), ("data", dbl_p),
+                ("scalars", dbl_p), ("mass", C.c_double), ("ctau", C.c_double)]
+
+
+class CubeGrid(C.Structure):
+    _fields_ = [("x_min", C.c_double), ("x_max", C.c_double), ("y_min", C.c_double),
+                ("y_max", C.c_double), ("z_min", C.c_double), ("z_max", C.c_double),
+                ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+
+class Paths(C.Structure):
+    _fields_ = [("n_rays", C.c_int32), ("n_segments", C.c_int32), ("max_segments", C.c_int32),
+                ("row_ptr", i32_p), ("cell", i32_p), ("length", dbl_p),
+                ("cos_theta", dbl_p), ("z_exit", dbl_p)]
+
+
+class CsdaNodes(C.Structure):
+    _fields_ = [("rock_material", C.c_int32), ("air_material", C.c_int32),
+                ("z_max", C.c_double), ("primary_altitude", C.c_double), ("scale", C.c_double),
+                ("n_nodes", C.c_int32), ("node_energy", dbl_p), ("node_weight", dbl_p),
+                ("node_charge", dbl_p)]
+
+
+class FlatGeometry(C.Structure):
+    _fields_ = [("rock_thickness", C.c_double), ("rock_density", C.c_double),
+                ("rock_material", C.c_int32), ("air_material", C.c_int32),
+                ("porous_material", C.c_int32), ("porous_density", C.c_double),
+                ("porous_thickness", C.c_double), ("primary_altitude", C.c_double)]
+
+
+class McConfig(C.Structure):
+    _fields_ = [("elevation", C.c_double), ("cos_theta", C.c_double), ("sin_theta", C.c_double),
+                ("energy_min", C.c_double), ("energy_max", C.c_double),
+                ("energy_threshold", C.c_double), ("mode_switch", C.c_double),
+                ("n_events", C.c_int64), ("seed", C.c_uint64), ("straggling", C.c_int32),
+                ("scattering", C.c_int32), ("compute_gradient", C.c_int32)]
+
+
+class McResult(C.Structure):
+    _fields_ = [("flux", C.c_double), ("sigma", C.c_double), ("gradient_flux", C.c_double),
+                ("gradient", C.c_double), ("source_energy_max", C.c_double),
+                ("n_events", C.c_int64)]
+
+
+# Exported symbols and their argument types.
+_SIGS = {
+    "dp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "dp_destroy": (C.c_int, [C.c_void_p]),
+    "dp_last_error": (C.c_char_p, []),
+    "dp_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dp_synchronize": (C.c_int, [C.c_void_p]),
+    "dp_device_info": (C.c_int, [C.c_void_p, i32_p, i32_p]),
+    "dp_upload_tables": (C.c_int, [C.c_void_p, C.POINTER(TableDesc), C.POINTER(C.c_void_p)]),
+    "dp_free_tables": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dp_trace_cube": (C.c_int, [C.c_void_p, C.POINTER(CubeGrid), C.c_int32, dbl_p, dbl_p, i32_p,
+                                i32_p, i32_p, dbl_p, dbl_p]),
+    "dp_trace_cube_dev": (C.c_int, [C.c_void_p, C.POINTER(CubeGrid), C.c_int32, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
+    "dp_trace_edges": (C.c_int, [C.c_void_p, dbl_p, C.c_int32, dbl_p, C.c_int32, dbl_p,
+                                 C.c_int32, C.c_int32, dbl_p, dbl_p, dbl_p, i32_p, i32_p, i32_p,
+                                 dbl_p, dbl_p]),
+    "dp_csda_cube": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Paths), C.POINTER(CsdaNodes),
+                               dbl_p, C.c_int32, dbl_p, dbl_p, dbl_p]),
+    "dp_csda_cube_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Paths),
+                                   C.POINTER(CsdaNodes), C.c_void_p, C.c_int32, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
+    "dp_csda_cube_vjp": (C.c_int, [C.c_void_p, C.POINTER(Paths), dbl_p, dbl_p, dbl_p,
+                                   C.c_int32, dbl_p]),
+    "dp_csda_cube_vjp_dev": (C.c_int, [C.c_void_p, C.POINTER(Paths), C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int32, C.c_void_p]),
+    "dp_mc_flat_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),
+                                      C.POINTER(McConfig), C.POINTER(McResult)]),
+    "dp_last_kernel_ms": (C.c_double, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not found: build it with diffpumas.jl_amd/build_lib.py")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def check(rc):
+    if rc != DP_OK:
+        msg = load().dp_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, RuntimeError)(msg)
+
+
+def ptr(a, ctype=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """One HIP device context (stream, scratch, timing events)."""
+
+    def __init__(self, device=0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.dp_create(C.byref(h), int(device)))
+        self.h = h
+        self.lib = lib
+
+    def close(self):
+        if self.h:
+            self.lib.dp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle):
+        check(self.lib.dp_set_stream(self.h, C.c_void_p(stream_handle)))
+
+    def synchronize(self):
+        check(self.lib.dp_synchronize(self.h))
+
+    def last_kernel_ms(self):
+        return float(self.lib.dp_last_kernel_ms(self.h))
+
+    def device_info(self):
+        n = C.c_int32()
+        w = C.c_int32()
+        check(self.lib.dp_device_info(self.h, C.byref(n), C.byref(w)))
+        return n.value, w.value
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx

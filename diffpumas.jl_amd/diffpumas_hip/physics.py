@@ -1,0 +1,274 @@
+"""Physics tables: PUMAS dE/dx loader, MDF recipes and the MaterialTable layout.
+
+Mirrors the reference's table interface (src/Physics.jl:64-116, :534-599;
+src/DEDXLoader.jl:51-191).  A PhysicsTables object holds the 19 arrays of
+every material on one knot grid as a numpy array [material][array][knot]
+(the layout dp_upload_tables copies to HBM) and uploads itself lazily to
+the device of a Context.
+"""
+import math
+import os
+import re
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from . import _lib
+from ._lib import (T_CSDA_RANGE, T_CSDA_STOP, T_CSDA_TIME, T_ENERGY, T_LOG_ENERGY,
+                   T_MAGNETIC_ROTATION, T_NARRAYS, T_SPIN_FACTOR, T_IONIZATION, T_BREMS, T_PAIR,
+                   T_PHOTO)
+
+MUON_MASS = 0.10565839        # src/Constants.jl:47
+MUON_C_TAU = 658.654          # src/Constants.jl:26
+LARMOR_FACTOR = 0.299792458   # src/Constants.jl:29
+
+
+class DEDXData:
+    """Parsed PUMAS dE/dx table (src/DEDXLoader.jl:24-43), GeV and kg/m2 units."""
+
+    def __init__(self, name, mass, zoa, cols):
+        self.material_name = name
+        self.particle_mass = mass
+        self.ZoA = zoa
+        (self.kinetic_energy, self.momentum, self.ionization, self.bremsstrahlung,
+         self.pair_production, self.photonuclear, self.radiative_total, self.stopping_power,
+         self.csda_range, self.density_effect, self.beta) = [np.array(c, dtype=np.float64)
+                                                              for c in cols]
+
+
+def _parse_header(lines):
+    """parse_header, src/DEDXLoader.jl:51-92."""
+    name, mass, zoa, start = "", 0.0, 0.0, None
+    for i, line in enumerate(lines):
+        s = line.strip()
+        if "Incident particle is a Muon with M =" in s:
+            m = re.search(r"M = ([\d.]+) MeV", s)
+            if m:
+                mass = float(m.group(1)) * 1e-3
+        if "Index =" in s and ":" in s:
+            m = re.search(r"Index = \d+: (\w+)", s)
+            if m:
+                name = m.group(1)
+        if "<Z/A>" in s:
+            m = re.search(r"<Z/A> = ([\d.]+)", s)
+            if m:
+                zoa = float(m.group(1))
+        if "[MeV]" in s and "[MeV/c]" in s:
+            start = i + 1
+            break
+    return name, mass, zoa, start
+
+
+def load_dedx_file(path):
+    """load_dedx_file, src/DEDXLoader.jl:100-191.
+
+    The data section starts after the units line; when a file has no units
+    line but a column header (as in test/test_dedx_first_record.jl), the data
+    starts at the first row that parses as 11 numbers, so the first physical
+    record is kept.
+    """
+    with open(path) as f:
+        lines = f.read().splitlines()
+    name, mass, zoa, start = _parse_header(lines)
+    if start is None:
+        start = 0
+    cols = [[] for _ in range(11)]
+    conv_dedx, conv_range = 1e-4, 10.0
+    for line in lines[start:]:
+        s = line.strip()
+        if not s or s.startswith("#"):
+            continue
+        parts = s.split()
+        if len(parts) < 11:
+            continue
+        try:
+            v = [float(p) for p in parts[:11]]
+        except ValueError:
+            continue
+        cols[0].append(v[0] * 1e-3)
+        cols[1].append(v[1] * 1e-3)
+        for j in range(2, 8):
+            cols[j].append(v[j] * conv_dedx)
+        cols[8].append(v[8] * conv_range)
+        cols[9].append(v[9])
+        cols[10].append(v[10])
+    if not cols[0]:
+        raise ValueError(f"Could not find data section in DEDX file {path}")
+    return DEDXData(name, mass, zoa, cols)
+
+
+# --------------------------------------------------------------- MDF recipes
+class Element:
+    def __init__(self, name, Z, A, I):
+        self.name, self.Z, self.A, self.I = name, float(Z), float(A), float(I)
+
+
+class BaseMaterial:
+    """Material recipe: elements with mass fractions (src/Materials.jl:29-133)."""
+
+    def __init__(self, name, density, elements, fractions, I=None):
+        self.name = name
+        self.density = float(density)
+        self.elements = list(elements)
+        tot = sum(fractions)
+        self.fractions = [f / tot for f in fractions]
+        self.ZoA = sum(f * e.Z / e.A for e, f in zip(self.elements, self.fractions))
+        if I is None:
+            # mean excitation energy from the fraction-weighted log I of the
+            # elements (Bragg additivity on Z/A-weighted log I)
+            num = sum(f * e.Z / e.A * math.log(e.I) for e, f in zip(self.elements, self.fractions))
+            I = math.exp(num / self.ZoA)
+        self.I = float(I)
+
+
+def load_mdf(path):
+    """Elements, materials and composites of a PUMAS MDF file (src/Loader.jl:115-287)."""
+    root = ET.parse(path).getroot()
+    elements = {}
+    for e in root.findall("element"):
+        elements[e.get("name")] = Element(e.get("name"), e.get("Z"), e.get("A"),
+                                          float(e.get("I")) * 1e-9)
+    materials = {}
+    for m in root.findall("material"):
+        comps = m.findall("component")
+        els = [elements[c.get("name")] for c in comps]
+        fr = [float(c.get("fraction")) for c in comps]
+        I = float(m.get("I")) * 1e-9 if m.get("I") is not None else None
+        materials[m.get("name")] = BaseMaterial(m.get("name"), float(m.get("density")) * 1e3,
+                                                els, fr, I)
+    composites = {}
+    for m in root.findall("composite"):
+        composites[m.get("name")] = [(c.get("name"), float(c.get("fraction")))
+                                     for c in m.findall("component")]
+    return elements, materials, composites
+
+
+# ------------------------------------------------------------ table building
+def dedx_file_name(material):
+    """File naming of find_dedx_file (src/DEDXLoader.jl:199-245)."""
+    snake = re.sub(r"(?<!^)(?=[A-Z])", "_", material).lower()
+    return f"{snake}.txt"
+
+
+def csda_columns_from_dedx(dedx, mass):
+    """The columns create_material_table_from_dedx copies from the file and
+    the integrated proper time (src/Physics.jl:146-181), plus the
+    kinematic columns (:256-263).  Returns {array index: values}."""
+    n = len(dedx.kinetic_energy)
+    e = dedx.kinetic_energy.copy()
+    s = dedx.stopping_power.copy()
+    t = np.zeros(n)
+    for i in range(1, n):
+        K, Kp = e[i], e[i - 1]
+        g = 1.0 + K / mass
+        gp = 1.0 + Kp / mass
+        b = math.sqrt(1.0 - 1.0 / (g * g))
+        bp = math.sqrt(1.0 - 1.0 / (gp * gp))
+        gm = (g + gp) / 2
+        bm = (b + bp) / 2
+        avg = (s[i] + s[i - 1]) / 2
+        dK = K - Kp
+        t[i] = t[i - 1] + dK / (avg * bm * gm)
+    mag = np.zeros(n)
+    spin = np.zeros(n)
+    for i in range(n):
+        K = e[i]
+        g = 1.0 + K / mass
+        b = math.sqrt(1.0 - 1.0 / (g * g))
+        p = math.sqrt(K * (K + 2 * mass))
+        en = K + mass
+        spin[i] = K * (en + mass) / (en * en)
+        mag[i] = LARMOR_FACTOR / (b * p)
+    return {T_ENERGY: e, T_LOG_ENERGY: np.log(e), T_CSDA_STOP: s,
+            T_CSDA_RANGE: dedx.csda_range.copy(), T_CSDA_TIME: t,
+            T_IONIZATION: dedx.ionization.copy(), T_BREMS: dedx.bremsstrahlung.copy(),
+            T_PAIR: dedx.pair_production.copy(), T_PHOTO: dedx.photonuclear.copy(),
+            T_MAGNETIC_ROTATION: mag, T_SPIN_FACTOR: spin}
+
+
+class PhysicsTables:
+    """PhysicsTables{Float64} (src/Physics.jl:109-116) in the device layout."""
+
+    def __init__(self, names, data, scalars, mass=MUON_MASS, ctau=MUON_C_TAU, recipes=None,
+                 complete=False):
+        self.names = list(names)
+        self.materials = {n: i for i, n in enumerate(self.names)}
+        self.data = np.ascontiguousarray(data, dtype=np.float64)
+        self.scalars = np.ascontiguousarray(scalars, dtype=np.float64)
+        self.mass = float(mass)
+        self.ctau = float(ctau)
+        self.recipes = recipes or {}
+        self.complete = complete  # False: only the CSDA columns are filled
+        self._dev = {}
+
+    @property
+    def n_materials(self):
+        return self.data.shape[0]
+
+    @property
+    def n_knots(self):
+        return self.data.shape[2]
+
+    def table(self, m, a):
+        return self.data[m, a]
+
+    def get_material_index(self, name):
+        """get_material_index, src/Physics.jl:606-608 (0-based, -1 if absent)."""
+        return self.materials.get(name, -1)
+
+    def supported_max(self):
+        return float(self.data[:, T_ENERGY, -1].min())
+
+    def device_handle(self, ctx):
+        """Upload once per context; returns the dp_tables handle."""
+        key = id(ctx)
+        h = self._dev.get(key)
+        if h is None:
+            import ctypes as C
+            desc = _lib.TableDesc(self.n_materials, self.n_knots, _lib.ptr(self.data),
+                                  _lib.ptr(self.scalars), self.mass, self.ctau)
+            out = C.c_void_p()
+            _lib.check(ctx.lib.dp_upload_tables(ctx.h, C.byref(desc), C.byref(out)))
+            h = out
+            self._dev[key] = (ctx, h)
+            return h
+        return h[1]
+
+    def release(self):
+        for ctx, h in self._dev.values():
+            if ctx.h:
+                ctx.lib.dp_free_tables(ctx.h, h)
+        self._dev.clear()
+
+
+def default_data_dir():
+    here = os.path.dirname(os.path.abspath(__file__))
+    return os.path.join(here, "..", "..", "tests", "golden")
+
+
+def create_physics_csda(materials=("StandardRock", "Air"), data_dir=None, mdf=None):
+    """create_physics (src/Physics.jl:534-599) for tabulated materials, CSDA
+    columns only (energies, log energies, stopping power, range, proper time,
+    per-process stopping powers, magnetic rotation, spin factor)."""
+    data_dir = data_dir or default_data_dir()
+    mdf = mdf or os.path.join(data_dir, "materials.xml")
+    _, recipes, _ = load_mdf(mdf)
+    names = list(materials)
+    cols, scal = [], []
+    K = None
+    for name in names:
+        path = os.path.join(data_dir, "pumas-materials", "dedx", "muon", dedx_file_name(name))
+        d = load_dedx_file(path)
+        c = csda_columns_from_dedx(d, MUON_MASS)
+        n = len(d.kinetic_energy)
+        if K is None:
+            K = n
+        elif n != K:
+            raise ValueError("all materials must share one energy grid")
+        arr = np.zeros((T_NARRAYS, n))
+        for k, v in c.items():
+            arr[k] = v
+        cols.append(arr)
+        rec = recipes.get(name)
+        scal.append([rec.density if rec else 0.0, rec.I if rec else 0.0, d.ZoA, 0.0])
+    return PhysicsTables(names, np.stack(cols), np.array(scal), recipes=recipes, complete=False)

@@ -1,0 +1,204 @@
+/*
+ * libdiffpumas_hip — MI355X (gfx950) kernels for DiffPumas' backward muon
+ * transport and tomography forward model, behind a plain C ABI.
+ *
+ * The reference (grinisrit/diffpumas.jl) is pure Julia with no FFI seam; each
+ * entry point below replaces the Julia function named in its comment and is
+ * what a Julia `ccall` binding binds (see INTEGRATION.md for the stubs).
+ *
+ * Conventions
+ *   - Every function returns int: 0 = DP_OK, negative = error class.  The
+ *     message of the last failure on the calling thread is dp_last_error().
+ *   - Material, cell and ray indices are 0-based (the Julia side adds 1).
+ *   - Host-pointer entry points copy inputs to device memory, run, copy
+ *     results back and synchronise before returning.  The *_dev variants take
+ *     device pointers and enqueue on the context's stream without
+ *     synchronising.
+ *   - A dp_ctx is not thread-safe; use one context per host thread.
+ *   - Results do not depend on launch shape, and repeated calls with the same
+ *     inputs return bitwise identical outputs.
+ */
+#ifndef DIFFPUMAS_HIP_H
+#define DIFFPUMAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define DP_OK 0
+#define DP_E_ARGUMENT (-1)  /* ArgumentError (bad mode, density, charge, size) */
+#define DP_E_DOMAIN (-2)    /* DomainError (energy outside a prepared table)   */
+#define DP_E_EXHAUSTED (-3) /* ErrorException (event/step caps, zero progress) */
+#define DP_E_HIP (-4)       /* HIP runtime failure                             */
+#define DP_E_CAPACITY (-5)  /* output or scratch capacity exceeded             */
+
+/* ------------------------------------------------------ material tables
+ * MaterialTable (src/Physics.jl:64-102): 19 arrays on one knot grid, in the
+ * order below, stored [material][array][knot].                             */
+enum dp_table_array {
+  DP_T_ENERGY = 0, DP_T_LOG_ENERGY, DP_T_CSDA_STOP, DP_T_CSDA_RANGE, DP_T_CSDA_TIME,
+  DP_T_MIXED_STOP, DP_T_MIXED_RANGE, DP_T_STRAGGLING, DP_T_IONIZATION, DP_T_BREMS,
+  DP_T_PAIR, DP_T_PHOTO, DP_T_CROSS_SECTION, DP_T_TRANSPORT_PATH, DP_T_ELASTIC_PATH,
+  DP_T_ELASTIC_CUTOFF, DP_T_MAGNETIC_ROTATION, DP_T_SCREENING, DP_T_SPIN_FACTOR,
+  DP_T_NARRAYS
+};
+
+typedef struct dp_ctx_s* dp_ctx;
+typedef struct dp_tables_s* dp_tables;
+
+typedef struct {
+  int32_t n_materials;
+  int32_t n_knots;          /* common knot grid (create_physics uses one grid) */
+  const double* data;       /* n_materials * DP_T_NARRAYS * n_knots            */
+  const double* scalars;    /* n_materials * 4: density kg/m3, I GeV, Z/A, 0   */
+  double mass;              /* particle mass, GeV                               */
+  double ctau;              /* decay length, m                                  */
+} dp_table_desc;
+
+/* ------------------------------------------------------------- lifecycle */
+int dp_create(dp_ctx* out, int device);
+int dp_destroy(dp_ctx ctx);
+const char* dp_last_error(void);
+/* Use `stream` (a hipStream_t) for all work of this context; NULL selects
+ * the context's own stream. */
+int dp_set_stream(dp_ctx ctx, void* stream);
+int dp_synchronize(dp_ctx ctx);
+/* Number of compute units and wavefronts the autotuned launches assume. */
+int dp_device_info(dp_ctx ctx, int32_t* n_cu, int32_t* wave_size);
+
+/* Replaces: the PhysicsTables object handed to every transport call
+ * (create_physics, src/Physics.jl:534-599).  Copies the arrays to HBM. */
+int dp_upload_tables(dp_ctx ctx, const dp_table_desc* desc, dp_tables* out);
+int dp_free_tables(dp_ctx ctx, dp_tables tables);
+
+/* ---------------------------------------------------------- traversal
+ * Uniform cube grid, CubeGeometry (examples/diff_flux_cube.jl:85-129). */
+typedef struct {
+  double x_min, x_max, y_min, y_max, z_min, z_max;
+  int32_t nx, ny, nz;
+  double dx, dy, dz;
+} dp_cube_grid;
+
+/* Replaces: trace_cube_path (examples/diff_flux_cube.jl:304-364), one ray
+ * per entry.  start_xy[2n], dir[3n] (unit, upward-going).  Two-phase: with
+ * cells == NULL only counts[n] (segments per ray) and z_exit[n] are written;
+ * otherwise row_ptr[n+1] (from the counts) locates each ray's segments.    */
+int dp_trace_cube(dp_ctx ctx, const dp_cube_grid* grid, int32_t n_rays,
+                  const double* start_xy, const double* dir, int32_t* counts,
+                  const int32_t* row_ptr, int32_t* cells, double* lengths, double* z_exit);
+int dp_trace_cube_dev(dp_ctx ctx, const dp_cube_grid* grid, int32_t n_rays,
+                      const double* start_xy, const double* dir, int32_t* counts,
+                      const int32_t* row_ptr, int32_t* cells, double* lengths, double* z_exit);
+
+/* Replaces: trace_path(::VoxelVolume) (examples/lvd_tomography.jl:738-779)
+ * on a rectilinear grid given by its edge vectors.  origin[3n] is the
+ * detector, surface_distance[n] the distance to the topography exit.      */
+int dp_trace_edges(dp_ctx ctx, const double* x_edges, int32_t nxe, const double* y_edges,
+                   int32_t nye, const double* z_edges, int32_t nze, int32_t n_rays,
+                   const double* origin, const double* dir, const double* surface_distance,
+                   int32_t* counts, const int32_t* row_ptr, int32_t* cells, double* lengths,
+                   double* traveled);
+
+/* ------------------------------------------- per-voxel-density CSDA
+ * Ray paths in CSR form (rows = rays).                                     */
+typedef struct {
+  int32_t n_rays;
+  int32_t n_segments;
+  int32_t max_segments;      /* max over rays of row_ptr[r+1]-row_ptr[r] */
+  const int32_t* row_ptr;    /* n_rays + 1 */
+  const int32_t* cell;       /* n_segments, 0-based */
+  const double* length;      /* n_segments, m */
+  const double* cos_theta;   /* n_rays */
+  const double* z_exit;      /* n_rays */
+} dp_paths;
+
+typedef struct {
+  int32_t rock_material;
+  int32_t air_material;
+  double z_max;              /* top of the voxel block (air interface), m */
+  double primary_altitude;   /* m (src/Geometry.jl:36 default 3e4)        */
+  double scale;              /* multiplies every ray flux (e.g. 1/N)      */
+  int32_t n_nodes;           /* energy nodes shared by all rays           */
+  const double* node_energy; /* GeV */
+  const double* node_weight; /* integration weight (sum uses 2*w)        */
+  const double* node_charge; /* -1, 0 (charge-summed) or +1               */
+} dp_csda_nodes;
+
+/* Replaces: compute_flux_from_segments (examples/diff_flux_cube.jl:374-468)
+ * summed as in compute_flux_gradient_cube (:531-541), per ray:
+ *   flux[r] = scale * sum_n 2 w_n flux(ray r, E_n, q_n)
+ * and, if jac != NULL, its reverse-mode derivative:
+ *   jac[s]     = d flux[r] / d rho[cell[s]] through segment s of ray r
+ *   jac_rem[r] = d flux[r] / d rho[0] through the remaining-rock segment.
+ * density[n_cells] is the per-voxel density, kg/m3.                       */
+int dp_csda_cube(dp_ctx ctx, dp_tables tables, const dp_paths* paths, const dp_csda_nodes* nodes,
+                 const double* density, int32_t n_cells, double* flux, double* jac,
+                 double* jac_rem);
+int dp_csda_cube_dev(dp_ctx ctx, dp_tables tables, const dp_paths* paths,
+                     const dp_csda_nodes* nodes, const double* density, int32_t n_cells,
+                     double* flux, double* jac, double* jac_rem);
+
+/* Replaces: the pullback of Zygote.gradient in compute_flux_gradient_cube
+ * (examples/diff_flux_cube.jl:547): grad[c] = sum_r flux_bar[r] d flux[r]/d rho[c]
+ * from the per-segment Jacobian, summed per cell in segment order (a stable
+ * cell-sorted transpose of the paths is built on each call), so the result
+ * is deterministic and independent of launch shape.                        */
+int dp_csda_cube_vjp(dp_ctx ctx, const dp_paths* paths, const double* jac, const double* jac_rem,
+                     const double* flux_bar, int32_t n_cells, double* grad);
+int dp_csda_cube_vjp_dev(dp_ctx ctx, const dp_paths* paths, const double* jac,
+                         const double* jac_rem, const double* flux_bar, int32_t n_cells,
+                         double* grad);
+
+/* -------------------------------------------- flat backward Monte Carlo
+ * TwoLayerGeometry (src/Geometry.jl:58-78), materials 0-based, porous < 0
+ * disables the porous layer.                                               */
+typedef struct {
+  double rock_thickness;
+  double rock_density;
+  int32_t rock_material;
+  int32_t air_material;
+  int32_t porous_material;
+  double porous_density;
+  double porous_thickness;
+  double primary_altitude;
+} dp_flat_geometry;
+
+typedef struct {
+  double elevation;          /* degrees; cos/sin of the zenith from host   */
+  double cos_theta;          /* cos((90 - elevation) * pi / 180)            */
+  double sin_theta;
+  double energy_min, energy_max;
+  double energy_threshold;   /* source energy maximum (<= table max)        */
+  double mode_switch;        /* energy_threshold_low, GeV (default 100)     */
+  int64_t n_events;
+  uint64_t seed;             /* Philox key; event e uses counter (e, block) */
+  int32_t straggling;
+  int32_t scattering;
+  int32_t compute_gradient;
+} dp_mc_config;
+
+typedef struct {
+  double flux, sigma;
+  double gradient_flux, gradient;  /* NaN when compute_gradient == 0 */
+  double source_energy_max;
+  int64_t n_events;
+} dp_mc_result;
+
+/* Replaces: _compute_flux_estimate (src/Geometry.jl:818-896) and, with
+ * compute_gradient, _sampled_csda_flux_gradient (:933-945) — the body of
+ * run_backward_mc (:1286-1338).  CSDA mode (straggling == scattering == 0). */
+int dp_mc_flat_backward(dp_ctx ctx, dp_tables tables, const dp_flat_geometry* geo,
+                        const dp_mc_config* cfg, dp_mc_result* out);
+
+/* Last kernel duration in ms, timed with HIP events on the context's stream
+ * around the dominant kernel of the previous call (0 if none). */
+double dp_last_kernel_ms(dp_ctx ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIFFPUMAS_HIP_H */

@@ -1,0 +1,316 @@
+// CPU oracle: extern "C" entry points loaded by tests/ (ctypes) and by the
+// cpu_baseline leg of bench.py.  TEST INFRASTRUCTURE ONLY.
+#include "oracle_core.hpp"
+#include "oracle_geometry.hpp"
+#include "oracle_philox.hpp"
+#include <cstring>
+#include <map>
+#include <omp.h>
+
+using namespace orc;
+
+static Tables make_tables(int M, int K, const double* data, const double* scal, double mass,
+                          double ctau) {
+  Tables T;
+  T.M = M; T.K = K; T.data = data; T.scalars = scal; T.mass = mass; T.ctau = ctau;
+  return T;
+}
+
+extern "C" {
+
+void orc_philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                      uint32_t k1, uint32_t* out) {
+  uint32_t c[4] = {c0, c1, c2, c3};
+  Philox4x32::block10(c, k0, k1);
+  std::memcpy(out, c, sizeof(c));
+}
+
+void orc_philox_uniforms(uint64_t seed, uint64_t event, int n, double* out) {
+  Philox4x32 r(seed, event);
+  for (int i = 0; i < n; ++i) out[i] = r.uniform();
+}
+
+double orc_flux_gccly(double c, double k, double q) { return flux_gccly(c, k, q); }
+double orc_flux_gaisser(double c, double k, double q) { return flux_gaisser(c, k, q); }
+double orc_dflux_gccly_dk(double c, double k, double q) {
+  return flux_gccly(c, Dual(k, 1.0), q).d;
+}
+
+// Property lookup on material m, array a, with its derivative d/dE.
+void orc_property(int M, int K, const double* data, const double* scal, int m, int a, int n,
+                  const double* e, double* out, double* dout) {
+  Tables T = make_tables(M, K, data, scal, MUON_MASS, MUON_C_TAU);
+  for (int i = 0; i < n; ++i) {
+    Dual y = property(T, m, a, Dual(e[i], 1.0));
+    out[i] = y.v;
+    if (dout) dout[i] = y.d;
+  }
+}
+void orc_kinetic(int M, int K, const double* data, const double* scal, int m, int mixed, int n,
+                 const double* x, double* out, double* dout) {
+  Tables T = make_tables(M, K, data, scal, MUON_MASS, MUON_C_TAU);
+  for (int i = 0; i < n; ++i) {
+    Dual y = prop_kinetic(T, mixed != 0, m, Dual(x[i], 1.0));
+    out[i] = y.v;
+    if (dout) dout[i] = y.d;
+  }
+}
+
+// ---------------------------------------------------------------- tracing
+// Pass 1 (cells == nullptr): writes counts[r].  Pass 2: fills the CSR arrays
+// at row_ptr offsets.  z_exit may be null.
+void orc_trace_cube(const double* grid /*12: xmin xmax ymin ymax zmin zmax nx ny nz dx dy dz*/,
+                    int n_rays, const double* start_xy, const double* dirs, int32_t* counts,
+                    const int32_t* row_ptr, int32_t* cells, double* lens, double* z_exit) {
+  CubeGrid g{grid[0], grid[1], grid[2], grid[3], grid[4], grid[5],
+             (int)grid[6], (int)grid[7], (int)grid[8], grid[9], grid[10], grid[11]};
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int r = 0; r < n_rays; ++r) {
+    std::vector<int> c;
+    std::vector<double> l;
+    double ze = trace_cube(g, start_xy[2 * r], start_xy[2 * r + 1], dirs[3 * r], dirs[3 * r + 1],
+                           dirs[3 * r + 2], c, l);
+    if (counts) counts[r] = (int32_t)c.size();
+    if (z_exit) z_exit[r] = ze;
+    if (cells) {
+      int32_t o = row_ptr[r];
+      for (size_t s = 0; s < c.size(); ++s) { cells[o + s] = c[s]; lens[o + s] = l[s]; }
+    }
+  }
+}
+
+void orc_trace_edges(const double* xe, int nxe, const double* ye, int nye, const double* ze,
+                     int nze, int n_rays, const double* origins, const double* dirs,
+                     const double* surface_distance, int32_t* counts, const int32_t* row_ptr,
+                     int32_t* cells, double* lens, double* traveled) {
+  EdgeGrid g{xe, nxe, ye, nye, ze, nze};
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int r = 0; r < n_rays; ++r) {
+    std::vector<int> c;
+    std::vector<double> l;
+    double tr = trace_edges(g, origins[3 * r], origins[3 * r + 1], origins[3 * r + 2],
+                            dirs[3 * r], dirs[3 * r + 1], dirs[3 * r + 2],
+                            surface_distance[r], c, l);
+    if (counts) counts[r] = (int32_t)c.size();
+    if (traveled) traveled[r] = tr;
+    if (cells) {
+      int32_t o = row_ptr[r];
+      for (size_t s = 0; s < c.size(); ++s) { cells[o + s] = c[s]; lens[o + s] = l[s]; }
+    }
+  }
+}
+
+// ------------------------------------------------- per-voxel-density CSDA
+// flux[r] = scale * sum_n 2 w_n flux(ray r, E_n, q_n)   (the summand of
+// compute_flux_gradient_cube, examples/diff_flux_cube.jl:531-541).
+// If grad_cells != null, the derivative of flux[r] with respect to each
+// distinct cell on ray r (and cell 0 through the remaining-rock segment) is
+// computed by forward-mode seeding, one cell at a time, and written as
+// (grad_cells, grad_vals) triples at grad_ptr[r] (capacity = nseg + 1).
+void orc_cube_csda(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                   int rock, int air, int n_rays, const int32_t* row_ptr, const int32_t* cells,
+                   const double* lens, const double* cos_theta, const double* z_exit, double z_max,
+                   double palt, int n_cells, const double* dens, int n_nodes, const double* ne,
+                   const double* nw, const double* nq, double scale, double* flux,
+                   const int32_t* grad_ptr, int32_t* grad_cells, double* grad_vals,
+                   int32_t* grad_count, int threads) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    std::vector<Dual> ddens;
+#pragma omp for schedule(dynamic, 4)
+    for (int r = 0; r < n_rays; ++r) {
+      const int b = row_ptr[r], ns = row_ptr[r + 1] - b;
+      const int* c = cells + b;
+      const double* l = lens + b;
+      double acc = 0.0;
+      for (int n = 0; n < n_nodes; ++n)
+        acc += 2.0 * nw[n] * cube_csda_flux(T, rock, air, c, l, ns, cos_theta[r], z_exit[r], z_max,
+                                           dens, ne[n], nq[n], palt);
+      flux[r] = scale * acc;
+      if (!grad_cells) continue;
+      // distinct cells of the ray, plus cell 0 for the remaining rock
+      std::map<int, int> uniq;
+      for (int s = 0; s < ns; ++s) uniq.emplace(c[s], 0);
+      if (z_exit[r] < z_max - EPS64) uniq.emplace(0, 0);
+      if ((int)ddens.size() != n_cells) {
+        ddens.assign(n_cells, Dual());
+        for (int i = 0; i < n_cells; ++i) ddens[i] = Dual(dens[i], 0.0);
+      }
+      int o = grad_ptr[r], cnt = 0;
+      for (auto& kv : uniq) {
+        int cell = kv.first;
+        ddens[cell].d = 1.0;
+        double g = 0.0;
+        for (int n = 0; n < n_nodes; ++n)
+          g += 2.0 * nw[n] * cube_csda_flux(T, rock, air, c, l, ns, cos_theta[r], z_exit[r],
+                                            z_max, ddens.data(), ne[n], nq[n], palt).d;
+        ddens[cell].d = 0.0;
+        grad_cells[o + cnt] = cell;
+        grad_vals[o + cnt] = scale * g;
+        ++cnt;
+      }
+      grad_count[r] = cnt;
+    }
+  }
+}
+
+// Reverse-mode form of the same derivative (used as the CPU baseline of the
+// adjoint; O(segments) per ray·node).  Each CSDA segment is a map
+// (E_in, rho) -> (E_out, log w) differentiated by two dual evaluations; the
+// reverse sweep accumulates d log(flux)/dE backwards along the ray.
+// jac[s] = d flux[r]/d rho through segment s; jac_rem[r] through the
+// remaining rock (density of cell 0).
+}  // extern "C"
+namespace {
+struct SegJac { double c, r, bE, bR; };  // dE/dE_in, dE/drho, dlogw/dE_in, dlogw/drho
+
+template <class R>
+inline void seg_map(const Tables& T, int m, R E, R rho, double dist, R& Eo, R& logw, bool& ok) {
+  R Xi = prop_range(T, false, m, E);
+  R ea = prop_kinetic(T, false, m, Xi + dist * rho);
+  ok = !(val(ea) >= 1e12) && std::isfinite(val(ea));
+  R si = prop_stop(T, false, m, E);
+  R sf = prop_stop(T, false, m, ea);
+  logw = log(sf / si) + log(decay_weight(T, dist, (E + ea) / 2.0));
+  Eo = ea;
+}
+inline SegJac seg_jac(const Tables& T, int m, double E, double rho, double dist, double& Eo,
+                      double& w, bool& ok) {
+  Dual eo, lw;
+  seg_map(T, m, Dual(E, 1.0), Dual(rho, 0.0), dist, eo, lw, ok);
+  SegJac j;
+  j.c = eo.d; j.bE = lw.d;
+  seg_map(T, m, Dual(E, 0.0), Dual(rho, 1.0), dist, eo, lw, ok);
+  j.r = eo.d; j.bR = lw.d;
+  Eo = eo.v;
+  w = std::exp(lw.v);
+  return j;
+}
+}  // namespace
+extern "C" {
+
+void orc_cube_csda_reverse(int M, int K, const double* data, const double* scal, double mass,
+                           double ctau, int rock, int air, int n_rays, const int32_t* row_ptr,
+                           const int32_t* cells, const double* lens, const double* cos_theta,
+                           const double* z_exit, double z_max, double palt, const double* dens,
+                           int n_nodes, const double* ne, const double* nw, const double* nq,
+                           double scale, double* flux, double* jac, double* jac_rem, int threads) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    std::vector<SegJac> tape;
+#pragma omp for schedule(dynamic, 1)
+    for (int r = 0; r < n_rays; ++r) {
+      const int b = row_ptr[r], ns = row_ptr[r + 1] - b;
+      const double cs = std::max(std::fabs(cos_theta[r]), 0.01);
+      const bool has_rem = z_exit[r] < z_max - EPS64;
+      const double rem = (z_max - z_exit[r]) / cs;
+      const double Xair = air_grammage(z_max, palt, cos_theta[r]);
+      const double path_air = (palt - z_max) / cs;
+      double facc = 0.0, racc = 0.0;
+      for (int s = 0; s < ns; ++s) jac[b + s] = 0.0;
+      tape.resize(ns);
+      for (int n = 0; n < n_nodes; ++n) {
+        double E = ne[n], W = 1.0, w;
+        bool ok = true, okk;
+        for (int s = 0; s < ns; ++s) {
+          double Eo;
+          tape[s] = seg_jac(T, rock, E, dens[cells[b + s]], lens[b + s], Eo, w, okk);
+          ok = ok && okk; W *= w; E = Eo;
+        }
+        SegJac jr{0, 0, 0, 0};
+        if (has_rem) {
+          double Eo;
+          jr = seg_jac(T, rock, E, dens[0], rem, Eo, w, okk);
+          ok = ok && okk; W *= w; E = Eo;
+        }
+        // air + flux as one function of the energy entering the air
+        Dual eo, lw;
+        bool oka;
+        Dual Xi = prop_range(T, false, air, Dual(E, 1.0));
+        eo = prop_kinetic(T, false, air, Xi + Xair);
+        oka = !(eo.v >= 1e12) && std::isfinite(eo.v);
+        lw = log(prop_stop(T, false, air, eo) / prop_stop(T, false, air, Dual(E, 1.0))) +
+             log(decay_weight(T, path_air, (Dual(E, 1.0) + eo) / 2.0)) +
+             log(flux_gccly(cos_theta[r], eo, nq[n]));
+        ok = ok && oka;
+        const double f = ok ? 2.0 * nw[n] * W * std::exp(lw.v) : 0.0;
+        facc += f;
+        if (f == 0.0) continue;
+        double Tad = lw.d;  // d log(tail)/dE at the exit of the last cell segment
+        if (has_rem) { racc += f * (jr.bR + Tad * jr.r); Tad = jr.bE + Tad * jr.c; }
+        for (int s = ns - 1; s >= 0; --s) {
+          const SegJac& j = tape[s];
+          jac[b + s] += f * (j.bR + Tad * j.r);
+          Tad = j.bE + Tad * j.c;
+        }
+      }
+      flux[r] = scale * facc;
+      for (int s = 0; s < ns; ++s) jac[b + s] *= scale;
+      if (jac_rem) jac_rem[r] = scale * racc;
+    }
+  }
+}
+
+// ------------------------------------------- flat two-layer CSDA (one event)
+// Returns the flux of compute_flux_single (deterministic CSDA walk) and its
+// density derivative (forward mode through the adaptive stepping).
+void orc_flat_csda(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                   const double* geo /*8*/, const int32_t* mats /*3: rock air porous*/,
+                   double energy, double cos_theta, double charge, double threshold,
+                   double mode_switch, int scat_limit, double* out /*flux, dflux, E, z, w*/) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Layered g{geo[0], geo[1], mats[0], mats[1], mats[2], geo[2], geo[3], geo[4]};
+  Dual f = flat_csda_flux(T, g, Dual(g.rock_density, 1.0), energy, cos_theta, charge, threshold,
+                          mode_switch, scat_limit != 0);
+  CsdaState<double> s = flat_csda_walk(T, g, g.rock_density, energy, cos_theta, threshold,
+                                       mode_switch, scat_limit != 0);
+  out[0] = f.v; out[1] = f.d; out[2] = s.energy; out[3] = s.z; out[4] = s.weight;
+}
+
+// _compute_flux_estimate with straggling = scattering = false
+// (src/Geometry.jl:818-896) plus _sampled_csda_flux_gradient (:933-945).
+// Event e draws its energy and charge uniforms from Philox(seed, e).
+// out = flux, sigma, gradient_flux, gradient.
+void orc_mc_flat_csda(int M, int K, const double* data, const double* scal, double mass,
+                      double ctau, const double* geo, const int32_t* mats, double emin,
+                      double emax, double cos_theta, double threshold, double mode_switch,
+                      int scat_limit, long long n, unsigned long long seed, int threads,
+                      int with_gradient, double* out) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Layered g{geo[0], geo[1], mats[0], mats[1], mats[2], geo[2], geo[3], geo[4]};
+  // gradient surrogate geometry: TwoLayerGeometry(thickness, rho, rock, air)
+  Layered gg{geo[0], geo[1], mats[0], mats[1], -1, 0.0, 0.0, PRIMARY_ALTITUDE};
+  if (threads > 0) omp_set_num_threads(threads);
+  double sw = 0, sw2 = 0, sgf = 0, sg = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : sw, sw2, sgf, sg)
+  for (long long e = 0; e < n; ++e) {
+    Philox4x32 rng(seed, (uint64_t)e);
+    double k, w;
+    sample_loguniform(emin, emax, rng.uniform(), k, w);
+    double q = rng.uniform() > 0.5 ? 1.0 : -1.0;
+    w *= 2.0;
+    double f = flat_csda_flux(T, g, g.rock_density, k, cos_theta, q, threshold, mode_switch,
+                              scat_limit != 0);
+    double wi = w * f;
+    sw += wi;
+    sw2 += wi * wi;
+    if (with_gradient) {
+      Dual fd = flat_csda_flux(T, gg, Dual(g.rock_density, 1.0), k, cos_theta, q, threshold,
+                               mode_switch, false);
+      sgf += w * fd.v;
+      sg += w * fd.d;
+    }
+  }
+  double flux = sw / (double)n;
+  double var = sw2 / (double)n - flux * flux;
+  out[0] = flux;
+  out[1] = g.rock_thickness > 0 ? std::sqrt(std::max(var, 0.0) / (double)n) : 0.0;
+  out[2] = sgf / (double)n;
+  out[3] = sg / (double)n;
+}
+
+}  // extern "C"

@@ -1,0 +1,324 @@
+// CPU oracle: voxel traversal and deterministic CSDA transport.
+// TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+#pragma once
+#include "oracle_core.hpp"
+#include <algorithm>
+
+namespace orc {
+
+// ================================================================ traversal
+// Uniform cube grid of examples/diff_flux_cube.jl:85-129 (fields as stored
+// by the CubeGeometry constructor; cell sizes are passed, not recomputed).
+struct CubeGrid {
+  double x_min, x_max, y_min, y_max, z_min, z_max;
+  int nx, ny, nz;
+  double dx, dy, dz;
+};
+
+// get_cell_indices, examples/diff_flux_cube.jl:132-144 (1-based i, j, k)
+inline bool cube_cell(const CubeGrid& g, double x, double y, double z, int& i, int& j, int& k) {
+  if (x < g.x_min || x > g.x_max || y < g.y_min || y > g.y_max || z < g.z_min || z > g.z_max)
+    return false;
+  i = std::min(std::max((int)std::floor((x - g.x_min) / g.dx) + 1, 1), g.nx);
+  j = std::min(std::max((int)std::floor((y - g.y_min) / g.dy) + 1, 1), g.ny);
+  k = std::min(std::max((int)std::floor((z - g.z_min) / g.dz) + 1, 1), g.nz);
+  return true;
+}
+
+// distance_to_boundary, examples/diff_flux_cube.jl:194-280.  Returns the
+// distance; *exits is set when the next cell lies outside the cube.
+inline double cube_boundary(const CubeGrid& g, double x, double y, double z,
+                           double ux, double uy, double uz, bool* exits, bool* inside) {
+  int i, j, k;
+  if (!cube_cell(g, x, y, z, i, j, k)) { *inside = false; *exits = true; return 0.0; }
+  *inside = true;
+  double x_lo = g.x_min + (i - 1) * g.dx, x_hi = g.x_min + i * g.dx;
+  double y_lo = g.y_min + (j - 1) * g.dy, y_hi = g.y_min + j * g.dy;
+  double z_lo = g.z_min + (k - 1) * g.dz, z_hi = g.z_min + k * g.dz;
+  double t_min = INFINITY;
+  bool exiting = false;
+  const double eps = 1e-10;
+  if (std::fabs(ux) > eps) {
+    if (ux > 0) { double t = (x_hi - x) / ux; if (t > eps && t < t_min) { t_min = t; exiting = (i + 1 > g.nx); } }
+    else        { double t = (x_lo - x) / ux; if (t > eps && t < t_min) { t_min = t; exiting = (i - 1 < 1); } }
+  }
+  if (std::fabs(uy) > eps) {
+    if (uy > 0) { double t = (y_hi - y) / uy; if (t > eps && t < t_min) { t_min = t; exiting = (j + 1 > g.ny); } }
+    else        { double t = (y_lo - y) / uy; if (t > eps && t < t_min) { t_min = t; exiting = (j - 1 < 1); } }
+  }
+  if (std::fabs(uz) > eps) {
+    if (uz > 0) { double t = (z_hi - z) / uz; if (t > eps && t < t_min) { t_min = t; exiting = (k + 1 > g.nz); } }
+    else        { double t = (z_lo - z) / uz; if (t > eps && t < t_min) { t_min = t; exiting = (k - 1 < 1); } }
+  }
+  *exits = exiting || t_min == INFINITY;
+  return t_min;
+}
+
+// trace_cube_path, examples/diff_flux_cube.jl:304-364.  The direction is
+// supplied by the caller (computed once on the host from elevation/azimuth).
+// Cells are returned 0-based.  Returns the z where the path left the cube.
+inline double trace_cube(const CubeGrid& g, double x0, double y0, double ux, double uy, double uz,
+                         std::vector<int>& cells, std::vector<double>& lens) {
+  double px = x0, py = y0, pz = 0.0, z_exit = 0.0;
+  for (int step = 0; step < 1000; ++step) {
+    int i, j, k;
+    if (!cube_cell(g, px, py, pz, i, j, k)) break;
+    int cell = (i - 1) + (j - 1) * g.nx + (k - 1) * g.nx * g.ny;
+    bool exits, inside;
+    double dist = cube_boundary(g, px, py, pz, ux, uy, uz, &exits, &inside);
+    if (dist <= 1e-10 || !std::isfinite(dist)) {
+      px += ux * 0.01; py += uy * 0.01; pz += uz * 0.01;
+      continue;
+    }
+    cells.push_back(cell);
+    lens.push_back(dist);
+    px += ux * dist; py += uy * dist; pz += uz * dist;
+    z_exit = pz;
+    if (exits) break;
+  }
+  return z_exit;
+}
+
+// Rectilinear voxel volume of examples/lvd_tomography.jl:181-187.
+// searchsortedlast on the edge vector (1-based result, 0 when x < edges[1]).
+inline int searchsortedlast(const double* e, int n, double x) {
+  int lo = 0, hi = n;  // count of edges <= x
+  while (lo < hi) { int mid = (lo + hi) >> 1; if (e[mid] <= x) lo = mid + 1; else hi = mid; }
+  return lo;
+}
+struct EdgeGrid {
+  const double* xe; int nxe;
+  const double* ye; int nye;
+  const double* ze; int nze;
+};
+// voxel_cell_indices, examples/lvd_tomography.jl:583-594
+inline bool edge_cell(const EdgeGrid& g, double x, double y, double z, int& i, int& j, int& k) {
+  if (x < g.xe[0] || x > g.xe[g.nxe - 1] || y < g.ye[0] || y > g.ye[g.nye - 1] ||
+      z < g.ze[0] || z > g.ze[g.nze - 1])
+    return false;
+  i = std::min(std::max(searchsortedlast(g.xe, g.nxe, x), 1), g.nxe - 1);
+  j = std::min(std::max(searchsortedlast(g.ye, g.nye, y), 1), g.nye - 1);
+  k = std::min(std::max(searchsortedlast(g.ze, g.nze, z), 1), g.nze - 1);
+  return true;
+}
+// distance_to_voxel_boundary, examples/lvd_tomography.jl:596-627
+inline double edge_boundary(const EdgeGrid& g, double x, double y, double z,
+                            double ux, double uy, double uz) {
+  int i, j, k;
+  if (!edge_cell(g, x, y, z, i, j, k)) return 0.0;
+  double x_lo = g.xe[i - 1], x_hi = g.xe[i], y_lo = g.ye[j - 1], y_hi = g.ye[j];
+  double z_lo = g.ze[k - 1], z_hi = g.ze[k];
+  double t_min = INFINITY;
+  const double e = 1e-10;
+  if (std::fabs(ux) > e) { double t = ux > 0 ? (x_hi - x) / ux : (x_lo - x) / ux; if (t > e) t_min = std::min(t_min, t); }
+  if (std::fabs(uy) > e) { double t = uy > 0 ? (y_hi - y) / uy : (y_lo - y) / uy; if (t > e) t_min = std::min(t_min, t); }
+  if (std::fabs(uz) > e) { double t = uz > 0 ? (z_hi - z) / uz : (z_lo - z) / uz; if (t > e) t_min = std::min(t_min, t); }
+  return std::isfinite(t_min) ? t_min : 0.0;
+}
+// trace_path(::VoxelVolume), examples/lvd_tomography.jl:738-779.  The ray
+// origin is the detector (the reference puts it at the origin); the surface
+// distance comes from the caller.  Returns the distance travelled in volume.
+inline double trace_edges(const EdgeGrid& g, double ox, double oy, double oz,
+                          double ux, double uy, double uz, double surface_distance,
+                          std::vector<int>& cells, std::vector<double>& lens) {
+  const double POINT_EPS = 1e-4;
+  double x = ox + ux * POINT_EPS, y = oy + uy * POINT_EPS, z = oz + uz * POINT_EPS;
+  double traveled = POINT_EPS;
+  const int nx = g.nxe - 1, ny = g.nye - 1;
+  while (traveled < surface_distance - POINT_EPS) {
+    int i, j, k;
+    if (!edge_cell(g, x, y, z, i, j, k)) break;
+    int cell = (i - 1) + (j - 1) * nx + (k - 1) * nx * ny;
+    double dist = edge_boundary(g, x, y, z, ux, uy, uz);
+    if (!std::isfinite(dist) || dist <= 0.0) break;
+    double remaining = surface_distance - traveled;
+    double step = std::min(dist, remaining);
+    if (step > 1e-8) { cells.push_back(cell); lens.push_back(step); traveled += step; }
+    if (dist >= remaining - 1e-6) break;
+    x += ux * (step + POINT_EPS);
+    y += uy * (step + POINT_EPS);
+    z += uz * (step + POINT_EPS);
+  }
+  return traveled;
+}
+
+// ===================================================== per-voxel-density CSDA
+// compute_flux_from_segments, examples/diff_flux_cube.jl:374-468.  R is the
+// density type (double, or Dual to get one directional derivative).
+template <class R>
+inline R cube_csda_flux(const Tables& T, int rock, int air, const int* cells, const double* lens,
+                        int nseg, double cos_theta, double z_exit, double z_max,
+                        const R* dens, double energy0, double charge, double palt) {
+  R energy = R(energy0);
+  R weight = R(1.0);
+  for (int s = 0; s < nseg; ++s) {
+    R rho = dens[cells[s]];
+    double dist = lens[s];
+    R gram = dist * rho;
+    R Xi = prop_range(T, false, rock, energy);
+    R Xf = Xi + gram;
+    R ea = prop_kinetic(T, false, rock, Xf);
+    if (val(ea) >= 1e12 || !std::isfinite(val(ea))) return R(0.0);
+    R si = prop_stop(T, false, rock, energy);
+    R sf = prop_stop(T, false, rock, ea);
+    weight = weight * (sf / si);
+    R Eavg = (energy + ea) / 2.0;
+    weight = weight * decay_weight(T, dist, Eavg);
+    energy = ea;
+  }
+  double cs = std::max(std::fabs(cos_theta), 0.01);
+  if (z_exit < z_max - EPS64) {
+    double rem = (z_max - z_exit) / cs;
+    R rho1 = dens[0];
+    R Xi = prop_range(T, false, rock, energy);
+    R Xf = Xi + rem * rho1;
+    R ea = prop_kinetic(T, false, rock, Xf);
+    if (val(ea) >= 1e12 || !std::isfinite(val(ea))) return R(0.0);
+    R si = prop_stop(T, false, rock, energy);
+    R sf = prop_stop(T, false, rock, ea);
+    weight = weight * (sf / si);
+    weight = weight * decay_weight(T, rem, (energy + ea) / 2.0);
+    energy = ea;
+  }
+  double Xair = air_grammage(z_max, palt, cos_theta);
+  R Xi = prop_range(T, false, air, energy);
+  R ef = prop_kinetic(T, false, air, Xi + Xair);
+  if (val(ef) >= 1e12 || !std::isfinite(val(ef))) return R(0.0);
+  R si = prop_stop(T, false, air, energy);
+  R sf = prop_stop(T, false, air, ef);
+  weight = weight * (sf / si);
+  double path_air = (palt - z_max) / cs;
+  weight = weight * decay_weight(T, path_air, (energy + ef) / 2.0);
+  return weight * flux_gccly(cos_theta, ef, charge);
+}
+
+// ========================================== flat two-layer deterministic CSDA
+// TwoLayerGeometry, src/Geometry.jl:58-78 (materials 0-based; porous < 0
+// disables the porous layer).
+struct Layered {
+  double rock_thickness, rock_density;
+  int rock, air, porous;
+  double porous_density, porous_thickness, primary_altitude;
+};
+
+template <class R> struct CsdaState {
+  R energy, weight, time, distance, grammage;
+  R z;   // only the vertical position matters for the flat geometry
+};
+
+// transport_backward_step, src/Geometry.jl:424-457 (rng = nothing branch).
+template <class R, class D>
+inline void csda_step(const Tables& T, CsdaState<R>& s, int m, D density, R ds, double uz_motion) {
+  R ki = s.energy;
+  R gs = ds * density;
+  R Xi = prop_range(T, false, m, ki);
+  R kf = prop_kinetic(T, false, m, Xi + gs);
+  R gi = 1.0 + ki / T.mass;
+  R gc = 1.0 + kf / T.mass;
+  R ga = (gi + gc) / 2.0;
+  R b2 = 1.0 - 1.0 / (ga * ga);
+  R beta = sqrt(val(b2) >= 1e-12 ? b2 : R(1e-12));
+  R dt = ds / (beta * ga);
+  // position -= ds * direction; direction_z = -uz_motion
+  s.z = s.z - ds * (-uz_motion);
+  R si = prop_stop(T, false, m, ki);
+  R sc = prop_stop(T, false, m, kf);
+  R wf = sc / si;
+  R decay = exp(-dt / T.ctau);
+  s.weight = s.weight * wf * decay;
+  s.energy = kf;
+  s.distance = s.distance + ds;
+  s.grammage = s.grammage + gs;
+  s.time = s.time + dt;
+}
+
+// _backward_geometric_step_limit, src/Geometry.jl:463-476
+template <class R, class D>
+inline R geometric_limit(const Tables& T, R energy, int m, D density, double uz, bool in_air,
+                         bool use_scattering) {
+  R Xi = prop_range(T, false, m, energy);
+  R range_limit = 0.01 * Xi / vmax(density, 1e-6);
+  R lim = range_limit;
+  if (use_scattering) {
+    R lb1 = property(T, m, T_TRANSPORT_PATH, energy);
+    if (val(lb1) > 0 && val(density) > 0) lim = vmin(lim, 0.01 * lb1 / density);
+  }
+  if (in_air) lim = vmin(lim, R(0.05 * 12e3 / std::max(std::fabs(uz), 0.05)));
+  return lim;
+}
+
+// transport_backward_through_geometry with rng = nothing,
+// src/Geometry.jl:534-672: every step is the deterministic CSDA step.  The
+// rock density is the differentiated parameter (R = Dual for the tangent).
+template <class R>
+inline CsdaState<R> flat_csda_walk(const Tables& T, const Layered& g, R rock_density,
+                                   double energy0, double cos_theta, double threshold,
+                                   double mode_switch, bool use_scattering_limit) {
+  CsdaState<R> s{R(energy0), R(1.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+  const double palt = g.primary_altitude;
+  const double uz = cos_theta;  // uz_motion = -direction_z = cos(theta)
+  for (int outer = 0; outer < 64; ++outer) {
+    if (!(val(s.energy) < threshold - EPS64)) break;
+    double z = val(s.z);
+    if (z < 0) break;
+    if (z >= palt - EPS64) break;
+    if (val(s.weight) <= 0 || !std::isfinite(val(s.weight))) break;
+    bool below = val(s.energy) < mode_switch - EPS64;
+    bool use_scat = below ? use_scattering_limit : false;
+    double limit = below ? std::min(mode_switch, threshold) : threshold;
+    for (int inner = 0; inner < 50000; ++inner) {
+      z = val(s.z);
+      if (z < 0 || z >= palt - EPS64) break;
+      if (val(s.energy) >= limit - EPS64) break;
+      if (val(s.weight) <= 0 || !std::isfinite(val(s.weight))) break;
+      bool has_porous = g.porous >= 0 && g.porous_thickness > 0;
+      double porous_z = has_porous ? g.rock_thickness - g.porous_thickness : g.rock_thickness;
+      int m;
+      R density;
+      R stb;
+      bool in_air = !(z < g.rock_thickness);
+      if (!in_air) {
+        if (has_porous && z >= porous_z) { m = g.porous; density = R(g.porous_density); }
+        else { m = g.rock; density = rock_density; }
+        if (uz > EPS64) {
+          double nb = (has_porous && z < porous_z) ? porous_z : g.rock_thickness;
+          stb = (nb - s.z) / uz;
+        } else if (uz < -EPS64) {
+          double pb = (has_porous && z >= porous_z) ? porous_z : 0.0;
+          stb = (pb - s.z) / uz;
+        } else stb = R(1000.0);
+      } else {
+        m = g.air;
+        // locals_air_at_altitude, src/Geometry.jl:95-106, evaluated at the
+        // (differentiable) position
+        density = 1.205 * exp(-s.z / 12e3);
+        if (uz > EPS64) stb = (palt - s.z) / uz;
+        else if (uz < -EPS64) stb = (g.rock_thickness - s.z) / uz;
+        else stb = R(1000.0);
+      }
+      R mg = geometric_limit(T, s.energy, m, density, uz, in_air, use_scat);
+      R step = vmin(stb + 1e-7, mg);
+      // clamp(step, STEP_MIN, 1000)
+      if (val(step) > 1000.0) step = R(1000.0);
+      else if (val(step) < STEP_MIN) step = R(STEP_MIN);
+      csda_step(T, s, m, density, step, uz);
+    }
+  }
+  return s;
+}
+
+// compute_flux_single, src/Geometry.jl:694-757 (deterministic CSDA walk).
+template <class R>
+inline R flat_csda_flux(const Tables& T, const Layered& g, R rho, double energy, double cos_t,
+                        double charge, double threshold, double mode_switch, bool scat_limit) {
+  CsdaState<R> s = flat_csda_walk(T, g, rho, energy, cos_t, threshold, mode_switch, scat_limit);
+  if (val(s.z) >= g.primary_altitude - 1.0 && val(s.energy) < threshold) {
+    double cf = cos_t;  // the CSDA walk never changes the direction
+    cf = std::min(std::max(cf, 0.0), 1.0);
+    if (cf <= 0) return R(0.0);
+    return s.weight * flux_gccly(cf, s.energy, charge);
+  }
+  return R(0.0);
+}
+
+}  // namespace orc

@@ -1,0 +1,218 @@
+"""ctypes access to the CPU oracle (oracle/liboracle_dp.so) for the tests.
+
+The oracle is the checker only; the product path never loads it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle_dp.so")
+
+d_p = C.POINTER(C.c_double)
+i_p = C.POINTER(C.c_int32)
+u_p = C.POINTER(C.c_uint32)
+
+_SIGS = {
+    "orc_philox_block": (None, [C.c_uint32] * 6 + [u_p]),
+    "orc_philox_uniforms": (None, [C.c_uint64, C.c_uint64, C.c_int, d_p]),
+    "orc_flux_gccly": (C.c_double, [C.c_double] * 3),
+    "orc_flux_gaisser": (C.c_double, [C.c_double] * 3),
+    "orc_dflux_gccly_dk": (C.c_double, [C.c_double] * 3),
+    "orc_property": (None, [C.c_int, C.c_int, d_p, d_p, C.c_int, C.c_int, C.c_int, d_p, d_p, d_p]),
+    "orc_kinetic": (None, [C.c_int, C.c_int, d_p, d_p, C.c_int, C.c_int, C.c_int, d_p, d_p, d_p]),
+    "orc_trace_cube": (None, [d_p, C.c_int, d_p, d_p, i_p, i_p, i_p, d_p, d_p]),
+    "orc_trace_edges": (None, [d_p, C.c_int, d_p, C.c_int, d_p, C.c_int, C.c_int, d_p, d_p, d_p,
+                               i_p, i_p, i_p, d_p, d_p]),
+    "orc_cube_csda": (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, C.c_int, C.c_int,
+                             C.c_int, i_p, i_p, d_p, d_p, d_p, C.c_double, C.c_double, C.c_int,
+                             d_p, C.c_int, d_p, d_p, d_p, C.c_double, d_p, i_p, i_p, d_p, i_p,
+                             C.c_int]),
+    "orc_cube_csda_reverse": (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, C.c_int, i_p, i_p, d_p, d_p, d_p, C.c_double,
+                                     C.c_double, d_p, C.c_int, d_p, d_p, d_p, C.c_double, d_p, d_p,
+                                     d_p, C.c_int]),
+    "orc_flat_csda": (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, d_p, i_p,
+                             C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                             d_p]),
+    "orc_mc_flat_csda": (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, d_p, i_p,
+                                C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_int, C.c_longlong, C.c_ulonglong, C.c_int, C.c_int, d_p]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE):
+            raise RuntimeError("oracle not built: make -C oracle")
+        L = C.CDLL(ORACLE)
+        for k, (r, a) in _SIGS.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def P(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def tabargs(ph):
+    return (ph.n_materials, ph.n_knots, P(ph.data), P(ph.scalars))
+
+
+def philox_block(ctr, key):
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox_block(*[C.c_uint32(x) for x in ctr], *[C.c_uint32(x) for x in key], out)
+    return list(out)
+
+
+def philox_uniforms(seed, event, n):
+    out = np.zeros(n)
+    lib().orc_philox_uniforms(seed, event, n, P(out))
+    return out
+
+
+def flux_gccly(c, k, q):
+    return lib().orc_flux_gccly(c, k, q)
+
+
+def flux_gaisser(c, k, q):
+    return lib().orc_flux_gaisser(c, k, q)
+
+
+def property_(ph, m, a, energies):
+    e = f64(energies)
+    out, dout = np.zeros(len(e)), np.zeros(len(e))
+    lib().orc_property(*tabargs(ph), m, a, len(e), P(e), P(out), P(dout))
+    return out, dout
+
+
+def kinetic(ph, m, ranges, mixed=False):
+    x = f64(ranges)
+    out, dout = np.zeros(len(x)), np.zeros(len(x))
+    lib().orc_kinetic(*tabargs(ph), m, int(mixed), len(x), P(x), P(out), P(dout))
+    return out, dout
+
+
+def _csr(counts):
+    rp = np.zeros(len(counts) + 1, dtype=np.int32)
+    np.cumsum(counts, out=rp[1:])
+    return rp
+
+
+def trace_cube(geo, start_xy, dirs):
+    g = geo.as_array()
+    sxy = f64(start_xy).reshape(-1, 2)
+    d = f64(dirs).reshape(-1, 3)
+    n = len(d)
+    counts = np.zeros(n, dtype=np.int32)
+    ze = np.zeros(n)
+    lib().orc_trace_cube(P(g), n, P(sxy), P(d), P(counts, C.c_int32), None, None, None, P(ze))
+    rp = _csr(counts)
+    ns = int(rp[-1])
+    cells = np.zeros(max(ns, 1), dtype=np.int32)
+    lens = np.zeros(max(ns, 1))
+    lib().orc_trace_cube(P(g), n, P(sxy), P(d), None, P(rp, C.c_int32), P(cells, C.c_int32),
+                         P(lens), None)
+    return rp, cells[:ns], lens[:ns], ze
+
+
+def trace_edges(xe, ye, ze, origins, dirs, surface):
+    xe, ye, ze = f64(xe), f64(ye), f64(ze)
+    o = f64(origins).reshape(-1, 3)
+    d = f64(dirs).reshape(-1, 3)
+    sd = f64(surface)
+    n = len(d)
+    counts = np.zeros(n, dtype=np.int32)
+    tr = np.zeros(n)
+    L = lib()
+    L.orc_trace_edges(P(xe), len(xe), P(ye), len(ye), P(ze), len(ze), n, P(o), P(d), P(sd),
+                      P(counts, C.c_int32), None, None, None, P(tr))
+    rp = _csr(counts)
+    ns = int(rp[-1])
+    cells = np.zeros(max(ns, 1), dtype=np.int32)
+    lens = np.zeros(max(ns, 1))
+    L.orc_trace_edges(P(xe), len(xe), P(ye), len(ye), P(ze), len(ze), n, P(o), P(d), P(sd), None,
+                      P(rp, C.c_int32), P(cells, C.c_int32), P(lens), None)
+    return rp, cells[:ns], lens[:ns], tr
+
+
+def cube_csda(ph, geo, rp, cells, lens, cos_t, z_exit, dens, ne, nw, nq, scale=1.0,
+              palt=3e4, gradient=True, threads=0):
+    """Per-ray flux and (optionally) per-(ray, distinct cell) derivatives by
+    forward-mode seeding.  Returns flux[, dict ray -> {cell: d flux/d rho}]."""
+    rp, cells, lens = i32(rp), i32(cells), f64(lens)
+    cos_t, z_exit, dens = f64(cos_t), f64(z_exit), f64(dens)
+    ne, nw, nq = f64(ne), f64(nw), f64(nq)
+    n = len(rp) - 1
+    flux = np.zeros(n)
+    gp = gc = gv = gn = None
+    if gradient:
+        gp = _csr(np.diff(rp) + 1)
+        cap = int(gp[-1])
+        gc = np.zeros(max(cap, 1), dtype=np.int32)
+        gv = np.zeros(max(cap, 1))
+        gn = np.zeros(n, dtype=np.int32)
+    lib().orc_cube_csda(*tabargs(ph), ph.mass, ph.ctau, geo.rock_material, geo.air_material, n,
+                        P(rp, C.c_int32), P(cells if len(cells) else np.zeros(1, np.int32), C.c_int32),
+                        P(lens if len(lens) else np.zeros(1)), P(cos_t), P(z_exit), geo.z_max, palt,
+                        len(dens), P(dens), len(ne), P(ne), P(nw), P(nq), scale, P(flux),
+                        P(gp, C.c_int32), P(gc, C.c_int32), P(gv), P(gn, C.c_int32), threads)
+    if not gradient:
+        return flux
+    grads = []
+    for r in range(n):
+        o = gp[r]
+        grads.append({int(gc[o + k]): float(gv[o + k]) for k in range(gn[r])})
+    return flux, grads
+
+
+def cube_csda_reverse(ph, rock, air, rp, cells, lens, cos_t, z_exit, z_max, dens, ne, nw, nq,
+                      scale=1.0, palt=3e4, threads=0):
+    """Reverse-mode oracle: (flux[n_rays], jac[n_segments], jac_rem[n_rays])."""
+    rp, cells, lens = i32(rp), i32(cells), f64(lens)
+    cos_t, z_exit, dens = f64(cos_t), f64(z_exit), f64(dens)
+    ne, nw, nq = f64(ne), f64(nw), f64(nq)
+    n = len(rp) - 1
+    flux, jrem = np.zeros(n), np.zeros(n)
+    jac = np.zeros(max(int(rp[-1]), 1))
+    lib().orc_cube_csda_reverse(*tabargs(ph), ph.mass, ph.ctau, rock, air, n, P(rp, C.c_int32),
+                                P(cells if len(cells) else np.zeros(1, np.int32), C.c_int32),
+                                P(lens if len(lens) else np.zeros(1)), P(cos_t), P(z_exit), z_max,
+                                palt, P(dens), len(ne), P(ne), P(nw), P(nq), scale, P(flux), P(jac),
+                                P(jrem), threads)
+    return flux, jac[:int(rp[-1])], jrem
+
+
+def flat_csda(ph, geo8, mats3, energy, cos_t, charge, threshold=1e9, mode_switch=100.0,
+              scat_limit=False):
+    out = np.zeros(5)
+    g = f64(geo8)
+    m = i32(mats3)
+    lib().orc_flat_csda(*tabargs(ph), ph.mass, ph.ctau, P(g), P(m, C.c_int32), energy, cos_t,
+                        charge, threshold, mode_switch, int(scat_limit), P(out))
+    return out
+
+
+def mc_flat_csda(ph, geo8, mats3, emin, emax, cos_t, n, seed, threshold=1e9, mode_switch=100.0,
+                 threads=0, gradient=True):
+    out = np.zeros(4)
+    g = f64(geo8)
+    m = i32(mats3)
+    lib().orc_mc_flat_csda(*tabargs(ph), ph.mass, ph.ctau, P(g), P(m, C.c_int32), emin, emax,
+                           cos_t, threshold, mode_switch, 0, n, seed, threads, int(gradient), P(out))
+    return out

@@ -1,0 +1,60 @@
+"""Seeded synthetic workloads shared by tests and bench.py.
+
+Shapes follow the reference's benchmark configurations: a 10 m voxel cube
+with detectors on its lower face and upward rays between 15 and 75 degrees
+elevation, log-spaced energy nodes, lognormal or uniform densities.
+"""
+import math
+
+import numpy as np
+
+
+def cube_rays(geo, n_detectors, n_dirs, seed, el_range=(15.0, 75.0), az_range=(0.0, 360.0)):
+    """Detectors on z = 0 inside the lower face; directions uniform in
+    azimuth and elevation.  Returns (start_xy, dirs, cos_theta)."""
+    rng = np.random.default_rng(seed)
+    span_x = geo.x_max - geo.x_min
+    span_y = geo.y_max - geo.y_min
+    det = np.stack([geo.x_min + span_x * rng.uniform(0.25, 0.75, n_detectors),
+                    geo.y_min + span_y * rng.uniform(0.25, 0.75, n_detectors)], axis=1)
+    el = rng.uniform(*el_range, size=(n_detectors, n_dirs))
+    az = rng.uniform(*az_range, size=(n_detectors, n_dirs))
+    sxy, dirs, cos_t = [], [], []
+    for d in range(n_detectors):
+        for k in range(n_dirs):
+            e = math.radians(el[d, k])
+            a = math.radians(az[d, k])
+            c = math.sin(e)
+            s = math.cos(e)
+            dirs.append((s * math.sin(a), s * math.cos(a), c))
+            cos_t.append(c)
+            sxy.append(det[d])
+    return np.array(sxy), np.array(dirs), np.array(cos_t)
+
+
+def energy_nodes(n, emin, emax):
+    """csda_energy_quadrature (src/Tomography.jl:263-274): log-midpoint
+    nodes, charge 0 (charge-summed), weight 0.5 * E * span."""
+    lo = math.log(emin)
+    span = math.log(emax) - lo
+    e = np.array([math.exp(lo + span * (i - 0.5) / n) for i in range(1, n + 1)])
+    return e, 0.5 * e * span, np.zeros(n)
+
+
+def uniform_densities(n_cells, seed, lo=2000.0, hi=3000.0):
+    return np.random.default_rng(seed).uniform(lo, hi, n_cells)
+
+
+def lognormal_cavity_densities(geo, seed, mean=2650.0, sigma=0.1, cavity=1800.0):
+    rng = np.random.default_rng(seed)
+    n = geo.n_cells
+    rho = mean * np.exp(sigma * rng.standard_normal(n) - 0.5 * sigma * sigma)
+    i = np.arange(n)
+    ix = i % geo.nx
+    iy = (i // geo.nx) % geo.ny
+    iz = i // (geo.nx * geo.ny)
+    cx, cy, cz = 0.5 * geo.nx, 0.45 * geo.ny, 0.6 * geo.nz
+    rx, ry, rz = 0.15 * geo.nx, 0.1 * geo.ny, 0.12 * geo.nz
+    inside = ((ix - cx) / rx) ** 2 + ((iy - cy) / ry) ** 2 + ((iz - cz) / rz) ** 2 <= 1.0
+    rho[inside] = cavity
+    return rho
