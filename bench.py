@@ -73,7 +73,9 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
                         C.cast(d_ne.data_ptr(), L.dbl_p), C.cast(d_nw.data_ptr(), L.dbl_p),
                         C.cast(d_nq.data_ptr(), L.dbl_p))
     h = physics.device_handle(ctx)
-    stream = torch.cuda.current_stream()
+    # a dedicated (non-null) stream shared by the library and the timing events
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
     ctx.set_stream(stream.cuda_stream)
     lib = ctx.lib
 
@@ -89,16 +91,17 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
                                          fbar.data_ptr(), geo.n_cells, grad.data_ptr()))
 
     log(f"adjoint: rays={paths.n_rays} segments={paths.n_segments} max_seg={paths.max_segments}")
-    for _ in range(warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
-    t0 = time.perf_counter()
-    for k in range(steps):
-        step(evs[k])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(evs[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / steps
     kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
     ctx.set_stream(None)
@@ -144,7 +147,7 @@ def cpu_adjoint_baseline(physics, problem, budget_s=12.0):
     import numpy as np
     import _oracle as O
     geo, paths, rho, ne, nw, nq = problem
-    threads = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n = 16
     while True:
         idx = np.arange(0, paths.n_rays, max(1, paths.n_rays // n))[:n]
@@ -175,6 +178,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mc-events", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 1 GiB device copy (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -210,6 +215,13 @@ def main():
         "mc_csda": {k: mc[k] for k in ("events", "flux", "sigma", "gradient", "kernel_ms",
                                        "ms_per_step")},
     }
+    if a.calib_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = 2 * src.numel() * 8
     print(json.dumps(out), flush=True)
     ctx.close()
 

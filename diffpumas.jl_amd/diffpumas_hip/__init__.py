@@ -6,7 +6,7 @@ All compute goes through libdiffpumas_hip.so; there is no CPU fallback.
 """
 from ._lib import (ArgumentError, CapacityError, Context, DomainError, HipError,
                    TransportError, default_context, load)
-from .physics import (PhysicsTables, create_physics_csda, load_dedx_file, load_mdf,
+from .physics import (PhysicsTables, create_physics, create_physics_csda, load_dedx_file, load_mdf,
                       MUON_MASS, MUON_C_TAU)
 from .cube import (CubeGeometry, RayPaths, compute_flux_gradient_cube, csda_cube_flux,
                    csda_cube_vjp, ray_direction, trace_cube_paths, trace_edge_paths)
@@ -14,7 +14,7 @@ from .geometry import PRIMARY_ALTITUDE, compute_flux, run_backward_mc
 
 __all__ = [
     "ArgumentError", "CapacityError", "Context", "DomainError", "HipError", "TransportError",
-    "default_context", "load", "PhysicsTables", "create_physics_csda", "load_dedx_file",
+    "default_context", "load", "PhysicsTables", "create_physics", "create_physics_csda", "load_dedx_file",
     "load_mdf", "MUON_MASS", "MUON_C_TAU", "CubeGeometry", "RayPaths",
     "compute_flux_gradient_cube", "csda_cube_flux", "csda_cube_vjp", "ray_direction",
     "trace_cube_paths", "trace_edge_paths", "PRIMARY_ALTITUDE", "compute_flux",

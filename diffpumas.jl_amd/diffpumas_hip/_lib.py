@@ -98,6 +98,22 @@ class McResult(C.Structure):
                 ("n_events", C.c_int64)]
 
 
+class MaterialRecipe(C.Structure):
+    _fields_ = [("n_elements", C.c_int32), ("Z", dbl_p), ("A", dbl_p), ("I", dbl_p),
+                ("fraction", dbl_p), ("I_material", C.c_double), ("density", C.c_double),
+                ("ZoA", C.c_double)]
+
+
+class DedxColumns(C.Structure):
+    _fields_ = [("n", C.c_int32), ("kinetic_energy", dbl_p), ("ionization", dbl_p),
+                ("bremsstrahlung", dbl_p), ("pair_production", dbl_p), ("photonuclear", dbl_p),
+                ("stopping_power", dbl_p), ("csda_range", dbl_p), ("ZoA", C.c_double)]
+
+
+class PhysicsSettings(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("elastic_ratio", C.c_double), ("mass", C.c_double)]
+
+
 # Exported symbols and their argument types.
 _SIGS = {
     "dp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -128,6 +144,15 @@ _SIGS = {
     "dp_mc_flat_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),
                                       C.POINTER(McConfig), C.POINTER(McResult)]),
     "dp_last_kernel_ms": (C.c_double, [C.c_void_p]),
+    "dp_build_material_table": (C.c_int, [C.POINTER(MaterialRecipe), C.POINTER(DedxColumns),
+                                          C.POINTER(PhysicsSettings), dbl_p]),
+    "dp_build_composite_table": (C.c_int, [C.c_int32, C.POINTER(dbl_p), dbl_p, C.c_int32,
+                                           C.POINTER(MaterialRecipe), C.POINTER(PhysicsSettings),
+                                           dbl_p]),
+    "dp_dcs": (C.c_double, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                            C.c_double, C.c_double]),
+    "dp_dcs_integral": (C.c_double, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32]),
 }
 
 _lib = None

@@ -6,6 +6,7 @@ every material on one knot grid as a numpy array [material][array][knot]
 (the layout dp_upload_tables copies to HBM) and uploads itself lazily to
 the device of a Context.
 """
+import ctypes as C
 import math
 import os
 import re
@@ -272,3 +273,84 @@ def create_physics_csda(materials=("StandardRock", "Air"), data_dir=None, mdf=No
         rec = recipes.get(name)
         scal.append([rec.density if rec else 0.0, rec.I if rec else 0.0, d.ZoA, 0.0])
     return PhysicsTables(names, np.stack(cols), np.array(scal), recipes=recipes, complete=False)
+
+
+# ----------------------------------------------------- full table builder
+def _recipe(mat):
+    Z = np.array([e.Z for e in mat.elements])
+    A = np.array([e.A for e in mat.elements])
+    I = np.array([e.I for e in mat.elements])
+    f = np.array(mat.fractions, dtype=np.float64)
+    keep = (Z, A, I, f)
+    r = _lib.MaterialRecipe(len(Z), _lib.ptr(Z), _lib.ptr(A), _lib.ptr(I), _lib.ptr(f), mat.I,
+                            mat.density, mat.ZoA)
+    return r, keep
+
+
+def flatten_composite(name, components, fractions):
+    """flatten_to_base_material (src/Materials.jl:1081-1118): merged element
+    fractions, harmonic-mean density (composite_density, :1066-1074)."""
+    tot = sum(fractions)
+    fr = [f / tot for f in fractions]
+    merged = {}
+    order = []
+    for cf, m in zip(fr, components):
+        for e, ef in zip(m.elements, m.fractions):
+            if e.name not in merged:
+                merged[e.name] = [e, 0.0]
+                order.append(e.name)
+            merged[e.name][1] += cf * ef
+    els = [merged[n][0] for n in order]
+    efs = [merged[n][1] for n in order]
+    rho_inv = sum(f / m.density for f, m in zip(fr, components))
+    density = sum(fr) / rho_inv
+    return BaseMaterial(name, density, els, efs)
+
+
+def create_physics(materials=("StandardRock", "Water", "Air"), composites=(), data_dir=None,
+                   mdf=None, cutoff=0.05, elastic_ratio=0.05):
+    """create_physics (src/Physics.jl:534-599) for tabulated materials: every
+    MaterialTable array, built by the library's host table builder from the
+    PUMAS dE/dx files and the MDF element recipes."""
+    data_dir = data_dir or default_data_dir()
+    mdf = mdf or os.path.join(data_dir, "materials.xml")
+    _, recipes, comp_defs = load_mdf(mdf)
+    lib = _lib.load()
+    settings = _lib.PhysicsSettings(cutoff, elastic_ratio, MUON_MASS)
+    names, tabs, scal = [], [], []
+    K = None
+    for name in materials:
+        d = load_dedx_file(os.path.join(data_dir, "pumas-materials", "dedx", "muon",
+                                        dedx_file_name(name)))
+        n = len(d.kinetic_energy)
+        K = n if K is None else K
+        if n != K:
+            raise ValueError("all materials must share one energy grid")
+        rec = recipes[name]
+        r, keep = _recipe(rec)
+        cols = [d.kinetic_energy, d.ionization, d.bremsstrahlung, d.pair_production,
+                d.photonuclear, d.stopping_power, d.csda_range]
+        cols = [np.ascontiguousarray(c) for c in cols]
+        dc = _lib.DedxColumns(n, *[_lib.ptr(c) for c in cols], d.ZoA)
+        out = np.zeros((T_NARRAYS, n))
+        _lib.check(lib.dp_build_material_table(C.byref(r), C.byref(dc), C.byref(settings),
+                                               _lib.ptr(out)))
+        names.append(name)
+        tabs.append(out)
+        scal.append([rec.density, rec.I, d.ZoA, 0.0])
+    for cname in composites:
+        parts = comp_defs[cname]
+        comps = [recipes[p] for p, _ in parts]
+        fr = np.array([f for _, f in parts])
+        fr = fr / fr.sum()
+        flat = flatten_composite(cname, comps, list(fr))
+        r, keep = _recipe(flat)
+        ptrs = (_lib.dbl_p * len(parts))(*[_lib.ptr(tabs[names.index(p)]) for p, _ in parts])
+        out = np.zeros((T_NARRAYS, K))
+        _lib.check(lib.dp_build_composite_table(len(parts), ptrs, _lib.ptr(fr), K, C.byref(r),
+                                                C.byref(settings), _lib.ptr(out)))
+        names.append(cname)
+        tabs.append(out)
+        scal.append([flat.density, flat.I, flat.ZoA, 0.0])
+        recipes[cname] = flat
+    return PhysicsTables(names, np.stack(tabs), np.array(scal), recipes=recipes, complete=True)

@@ -75,6 +75,59 @@ int dp_device_info(dp_ctx ctx, int32_t* n_cu, int32_t* wave_size);
 int dp_upload_tables(dp_ctx ctx, const dp_table_desc* desc, dp_tables* out);
 int dp_free_tables(dp_ctx ctx, dp_tables tables);
 
+/* ------------------------------------------------------ table building
+ * Host-side construction of the MaterialTable arrays (setup, not the hot
+ * path).  Element recipe of a material as read from a PUMAS MDF file
+ * (src/Loader.jl:115-205): Z, A (g/mol), I (GeV), mass fractions.        */
+typedef struct {
+  int32_t n_elements;
+  const double* Z;
+  const double* A;
+  const double* I;
+  const double* fraction;
+  double I_material;         /* GeV */
+  double density;            /* kg/m3 */
+  double ZoA;
+} dp_material_recipe;
+
+/* Columns of a PUMAS dE/dx table in GeV and kg/m2 units
+ * (load_dedx_file, src/DEDXLoader.jl:100-191). */
+typedef struct {
+  int32_t n;
+  const double* kinetic_energy;
+  const double* ionization;
+  const double* bremsstrahlung;
+  const double* pair_production;
+  const double* photonuclear;
+  const double* stopping_power;
+  const double* csda_range;
+  double ZoA;                /* header <Z/A> of the file (used by the Coulomb tables) */
+} dp_dedx_columns;
+
+/* PhysicsSettings (src/Physics.jl:39-57) plus the particle mass. */
+typedef struct {
+  double cutoff;             /* relative soft/hard cutoff (0.05) */
+  double elastic_ratio;      /* 0.05 */
+  double mass;               /* GeV */
+} dp_physics_settings;
+
+/* Replaces: create_material_table_from_dedx (src/Physics.jl:146-273).
+ * out[DP_T_NARRAYS * n] receives the 19 arrays on the dE/dx knots. */
+int dp_build_material_table(const dp_material_recipe* recipe, const dp_dedx_columns* dedx,
+                            const dp_physics_settings* settings, double* out);
+/* Replaces: create_composite_table (src/Physics.jl:421-525); `flat` is the
+ * flattened element recipe (flatten_to_base_material, src/Materials.jl). */
+int dp_build_composite_table(int32_t n_components, const double* const* tables,
+                             const double* fractions, int32_t n_knots,
+                             const dp_material_recipe* flat, const dp_physics_settings* settings,
+                             double* out);
+/* Per-atom dσ/dq (m2/GeV) of process 0 brems, 1 pair, 2 photonuclear,
+ * 3 ionisation (src/Materials.jl:534-926), and its moment integral per kg
+ * of element (compute_dcs_integral, src/Materials.jl:939-1007). */
+double dp_dcs(int32_t process, double Z, double A, double I, double mass, double K, double q);
+double dp_dcs_integral(int32_t process, double Z, double A, double I, double mass, double K,
+                       double xlow, double xhigh, int32_t mode, int32_t npoints);
+
 /* ---------------------------------------------------------- traversal
  * Uniform cube grid, CubeGeometry (examples/diff_flux_cube.jl:85-129). */
 typedef struct {
