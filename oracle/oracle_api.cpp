@@ -3,6 +3,7 @@
 #include "oracle_core.hpp"
 #include "oracle_geometry.hpp"
 #include "oracle_philox.hpp"
+#include "oracle_flat_mc.hpp"
 #include <cstring>
 #include <map>
 #include <omp.h>
@@ -311,6 +312,75 @@ void orc_mc_flat_csda(int M, int K, const double* data, const double* scal, doub
   out[1] = g.rock_thickness > 0 ? std::sqrt(std::max(var, 0.0) / (double)n) : 0.0;
   out[2] = sgf / (double)n;
   out[3] = sg / (double)n;
+}
+
+
+// Stochastic flat backward MC (_compute_flux_estimate, src/Geometry.jl:818-896)
+// with a runtime-mixture rock layer.  recipes: n_el[M] elements per material,
+// then Z, A, I, f concatenated in material order.  Returns 0, or -2 on a
+// DomainError, -3 on another transport error (first failing event in
+// err_event).  w_events (optional) receives each event's contribution.
+int orc_mc_flat(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                const int32_t* n_el, const double* eZ, const double* eA, const double* eI,
+                const double* ef, int n_rock, const int32_t* rock_mats, const double* rock_fracs,
+                double rock_rho, double thick, int air, int porous, double porous_rho,
+                double porous_thick, double palt, double cos_t, double sin_t, double emin,
+                double emax, double threshold, double mode_switch, int straggling, int scattering,
+                double cutoff, double elastic_ratio, long long n, unsigned long long seed,
+                int threads, double* w_events, double* out, long long* err_event) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  std::vector<std::vector<ElementRec>> rec(M);
+  int o = 0;
+  for (int m = 0; m < M; ++m)
+    for (int i = 0; i < n_el[m]; ++i, ++o) rec[m].push_back({eZ[o], eA[o], eI[o], ef[o]});
+  Mixture mrock = make_mixture(T, std::vector<int>(rock_mats, rock_mats + n_rock),
+                               std::vector<double>(rock_fracs, rock_fracs + n_rock));
+  Mixture mair = make_mixture(T, {air}, {1.0});
+  Mixture mpor = make_mixture(T, {porous >= 0 ? porous : rock_mats[0]}, {1.0});
+  MixedKernel krock = prepare_mixed_kernel(T, mrock, rec, cutoff);
+  MixedKernel kair = prepare_mixed_kernel(T, mair, rec, cutoff);
+  MixedKernel kpor = prepare_mixed_kernel(T, mpor, rec, cutoff);
+  McSetup S;
+  S.T = &T;
+  S.rock = {&mrock, &krock};
+  S.air = {&mair, &kair};
+  S.porous = {&mpor, &kpor};
+  S.has_porous = porous >= 0 && porous_thick > 0;
+  S.thick = thick; S.rock_rho = rock_rho; S.porous_rho = porous_rho; S.porous_thick = porous_thick;
+  S.palt = palt; S.cutoff = cutoff; S.elastic_ratio = elastic_ratio;
+  if (threads > 0) omp_set_num_threads(threads);
+  double sw = 0, sw2 = 0;
+  int status = 0;
+  long long first_bad = -1;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : sw, sw2)
+  for (long long e = 0; e < n; ++e) {
+    Philox4x32 rng(seed, (uint64_t)e);
+    double k, w;
+    sample_loguniform(emin, emax, rng.uniform(), k, w);
+    const double q = rng.uniform() > 0.5 ? 1.0 : -1.0;
+    w *= 2.0;
+    double f = 0.0;
+    try {
+      f = flat_event(S, k, cos_t, sin_t, q, threshold, mode_switch, straggling != 0,
+                     scattering != 0, rng);
+    } catch (const DomainErr&) {
+#pragma omp critical
+      { if (status == 0 || e < first_bad) { status = -2; first_bad = e; } }
+    } catch (const std::exception&) {
+#pragma omp critical
+      { if (status == 0 || e < first_bad) { status = -3; first_bad = e; } }
+    }
+    const double wi = w * f;
+    if (w_events) w_events[e] = wi;
+    sw += wi;
+    sw2 += wi * wi;
+  }
+  const double flux = sw / (double)n;
+  const double var = sw2 / (double)n - flux * flux;
+  out[0] = flux;
+  out[1] = thick > 0 ? std::sqrt(std::max(var, 0.0) / (double)n) : 0.0;
+  if (err_event) *err_event = first_bad;
+  return status;
 }
 
 }  // extern "C"

@@ -216,3 +216,47 @@ def mc_flat_csda(ph, geo8, mats3, emin, emax, cos_t, n, seed, threshold=1e9, mod
     lib().orc_mc_flat_csda(*tabargs(ph), ph.mass, ph.ctau, P(g), P(m, C.c_int32), emin, emax,
                            cos_t, threshold, mode_switch, 0, n, seed, threads, int(gradient), P(out))
     return out
+
+
+def recipes_arrays(ph):
+    """Per-material element recipes of a PhysicsTables as the flat arrays
+    orc_mc_flat takes."""
+    n_el, Z, A, I, f = [], [], [], [], []
+    for name in ph.names:
+        r = ph.recipes[name]
+        n_el.append(len(r.elements))
+        for e, fr in zip(r.elements, r.fractions):
+            Z.append(e.Z); A.append(e.A); I.append(e.I); f.append(fr)
+    return (np.array(n_el, dtype=np.int32), np.array(Z), np.array(A), np.array(I), np.array(f))
+
+
+_SIGS["orc_mc_flat"] = (C.c_int, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, i_p, d_p,
+                                  d_p, d_p, d_p, C.c_int, i_p, d_p, C.c_double, C.c_double,
+                                  C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
+                                  C.c_longlong, C.c_ulonglong, C.c_int, d_p, d_p,
+                                  C.POINTER(C.c_longlong)])
+
+
+def mc_flat(ph, rock_mats, rock_fracs, rock_rho, thick, air, elevation, emin, emax, n, seed,
+            threshold=1e9, mode_switch=100.0, straggling=True, scattering=True, palt=3e4,
+            porous=-1, porous_rho=0.0, porous_thick=0.0, threads=0, per_event=False,
+            cutoff=0.05, elastic_ratio=0.05):
+    """Stochastic flat backward MC on the CPU oracle: (flux, sigma[, w_events])."""
+    import math
+    L = lib()
+    n_el, Z, A, I, f = recipes_arrays(ph)
+    rm, rf = i32(rock_mats), f64(rock_fracs)
+    theta = (90.0 - elevation) * math.pi / 180.0
+    out = np.zeros(2)
+    ev = np.zeros(n) if per_event else None
+    bad = C.c_longlong(-1)
+    rc = L.orc_mc_flat(*tabargs(ph), ph.mass, ph.ctau, P(n_el, C.c_int32), P(Z), P(A), P(I), P(f),
+                       len(rm), P(rm, C.c_int32), P(rf), rock_rho, thick, air, porous, porous_rho,
+                       porous_thick, palt, math.cos(theta), math.sin(theta), emin, emax,
+                       threshold, mode_switch, int(straggling), int(scattering), cutoff,
+                       elastic_ratio, n, seed, threads, P(ev), P(out), C.byref(bad))
+    if rc != 0:
+        raise RuntimeError(f"oracle MC error {rc} at event {bad.value}")
+    return (out[0], out[1], ev) if per_event else (out[0], out[1])
