@@ -121,6 +121,18 @@ int dp_upload_tables(dp_ctx c, const dp_table_desc* d, dp_tables* out) {
   t->mass = d->mass;
   t->ctau = d->ctau;
   t->host_scalars.assign(d->scalars, d->scalars + (size_t)M * 4);
+  t->host_data.assign(d->data, d->data + (size_t)M * DP_T_NARRAYS * K);
+  t->cutoff = d->cutoff > 0 ? d->cutoff : 0.05;
+  t->elastic_ratio = d->elastic_ratio > 0 ? d->elastic_ratio : 0.05;
+  t->recipes.assign(M, {});
+  if (d->n_elements) {
+    size_t o = 0;
+    for (int m = 0; m < M; ++m) {
+      if (d->n_elements[m] < 0) { delete t; return dph::fail(DP_E_ARGUMENT, "dp_upload_tables: negative element count"); }
+      for (int i = 0; i < d->n_elements[m]; ++i, ++o)
+        t->recipes[m].push_back({d->element_Z[o], d->element_A[o], d->element_I[o], d->element_fraction[o]});
+    }
+  }
   for (int m = 0; m < M; ++m)
     t->last_energy.push_back(d->data[((size_t)m * DP_T_NARRAYS + DP_T_ENERGY) * K + K - 1]);
   size_t nb = (size_t)M * DP_T_NARRAYS * K * sizeof(double);

@@ -29,13 +29,34 @@ void release(DevBuf& b);
 
 }  // namespace dph
 
+#include <map>
+#include <memory>
+
+namespace dph {
+// Element of a material recipe (Z, A, I in GeV, mass fraction).
+struct Element { double Z, A, I, f; };
+// A prepared object living in device memory (mixture tables or mixed kernel).
+struct Prepared {
+  void* dev = nullptr;        // one allocation holding all arrays
+  std::vector<double> host;   // host copy of the same arrays
+  int n = 0, pow2 = 0, n_el = 0;
+  double onset = 0, cutoff = 0;
+  ~Prepared() { if (dev) (void)hipFree(dev); }
+};
+}  // namespace dph
+
 struct dp_tables_s {
   int32_t n_materials = 0, n_knots = 0, pow2 = 0;
-  double mass = 0, ctau = 0;
+  double mass = 0, ctau = 0, cutoff = 0.05, elastic_ratio = 0.05;
   double* data = nullptr;      // device: M * NARR * K
   double* scalars = nullptr;   // device: M * 4
+  std::vector<double> host_data;     // M * NARR * K
   std::vector<double> host_scalars;  // M * 4
   std::vector<double> last_energy;   // per material, top of the table
+  std::vector<std::vector<dph::Element>> recipes;  // per material (may be empty)
+  // Lazily prepared per (components, fractions) key: mixture range tables
+  // and mixed kernels (MixedKernel snapshots).
+  std::map<std::vector<double>, std::unique_ptr<dph::Prepared>> mixtures, kernels;
 };
 
 struct dp_ctx_s {

@@ -12,6 +12,8 @@
 // through the adaptive step sizes (src/Geometry.jl:933-1028).
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
+#include "dp_transport.hpp"
+#include <algorithm>
 #include <cmath>
 
 using namespace dpk;
@@ -123,7 +125,7 @@ __device__ R flat_csda_flux(const MatView* mv, int K, int pow2, double mass, dou
       csda_step(m, K, pow2, mass, ctau, s, density, step, uz);
     }
   }
-  if (outer == 64) atomicOr(err, 2);
+  if (outer == 64) atomicOr(err, ERR_CAP);
   if (val(s.z) >= palt - 1.0 && val(s.energy) < threshold) {
     double cf = fmin(fmax(uz, 0.0), 1.0);
     if (cf <= 0) return R(0.0);
@@ -182,16 +184,318 @@ __global__ __launch_bounds__(256) void mc_flat_csda_kernel(McArgs a) {
                                        a.cos_t, q, a.thr, a.sw, false, a.err);
       a.out_gf[e] = w * fd.v;
       a.out_gd[e] = w * fd.d;
-      if (a.same_geo) {
-        a.out_w[e] = w * fd.v;
+      if (a.same_geo || !a.out_w) {
+        if (a.out_w) a.out_w[e] = w * fd.v;
         continue;
       }
     }
+    if (!a.out_w) continue;
     double f = flat_csda_flux<double>(mv, K, a.pow2, a.mass, a.ctau, a.g, a.g.rho, k, a.cos_t, q,
                                       a.thr, a.sw, a.scat_limit, a.err);
     a.out_w[e] = w * f;
   }
 }
+
+// ------------------------------------------------- stochastic transport
+struct StochArgs {
+  DevTables T;
+  Medium rock, air, porous;
+  bool has_porous;
+  double thick, rock_rho, porous_rho, porous_thick, palt;
+  double cos_t, sin_t, emin, emax, thr, sw;
+  bool straggling, scattering;
+  int64_t n;
+  uint64_t seed;
+  double* out_w;
+  int* err;            // [0] error bits, [1] spare
+  unsigned long long* bad_event;
+};
+
+// transport_backward_through_geometry with an RNG, src/Geometry.jl:534-672,
+// and the acceptance of compute_flux_single (:739-757).
+__device__ double flat_event(const StochArgs& a, double energy, double charge, Philox& rng,
+                             int& err) {
+  const DevTables& T = a.T;
+  McState st;
+  st.E = energy; st.w = 1.0; st.t = 0.0;
+  st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
+  st.dir[0] = -a.sin_t; st.dir[1] = 0.0; st.dir[2] = -a.cos_t;
+  const double palt = a.palt;
+  int outer = 0;
+  for (; outer < 64; ++outer) {
+    if (!(st.E < a.thr - EPS64)) break;
+    double z = st.pos[2];
+    if (z < 0 || z >= palt - EPS64) break;
+    if (st.w <= 0 || !isfinite(st.w)) break;
+    const bool below = st.E < a.sw - EPS64;
+    const bool use_str = below && a.straggling;
+    const bool use_mixed = !below && a.straggling;
+    const bool use_scat = below && a.scattering;
+    const double limit = below ? fmin(a.sw, a.thr) : a.thr;
+    for (int inner = 0; inner < 50000; ++inner) {
+      z = st.pos[2];
+      if (z < 0 || z >= palt - EPS64) break;
+      if (st.E >= limit - EPS64) break;
+      if (st.w <= 0 || !isfinite(st.w)) break;
+      const double uz = -st.dir[2];
+      const double porous_z = a.has_porous ? a.thick - a.porous_thick : a.thick;
+      const Medium* md;
+      double density, stb;
+      const bool in_air = !(z < a.thick);
+      if (!in_air) {
+        if (a.has_porous && z >= porous_z) { md = &a.porous; density = a.porous_rho; }
+        else { md = &a.rock; density = a.rock_rho; }
+        if (uz > EPS64) stb = (((a.has_porous && z < porous_z) ? porous_z : a.thick) - z) / uz;
+        else if (uz < -EPS64) stb = (((a.has_porous && z >= porous_z) ? porous_z : 0.0) - z) / uz;
+        else stb = 1000.0;
+      } else {
+        md = &a.air;
+        density = 1.205 * exp(-z / 12e3);
+        if (uz > EPS64) stb = (palt - z) / uz;
+        else if (uz < -EPS64) stb = (a.thick - z) / uz;
+        else stb = 1000.0;
+      }
+      // _backward_geometric_step_limit, src/Geometry.jl:463-476
+      const double le = log(st.E);
+      const double Xi = mix_range(T, md->mix, false, st.E, le);
+      double lim = 0.01 * Xi / fmax(density, 1e-6);
+      if (use_scat) {
+        const double lb1 = mix_transport_path(T, md->mix, st.E, le);
+        if (lb1 > 0 && density > 0) lim = fmin(lim, 0.01 * lb1 / density);
+      }
+      if (in_air) lim = fmin(lim, 0.05 * 12e3 / fmax(fabs(uz), 0.05));
+      double step = fmin(stb + 1e-7, lim);
+      step = fmin(fmax(step, 1e-7), 1000.0);
+      if (use_str) straggled_step(T, *md, st, density, step, use_scat, limit, rng, err);
+      else if (use_mixed) mixed_step(T, *md, st, density, step, limit, rng, err);
+      else csda_mix_step(T, *md, st, density, step);
+      if (err) return 0.0;
+    }
+  }
+  if (outer == 64) { err |= ERR_CAP; return 0.0; }
+  if (st.pos[2] >= palt - 1.0 && st.E < a.thr) {
+    const double cf = fmin(fmax(-st.dir[2], 0.0), 1.0);
+    if (cf <= 0) return 0.0;
+    return st.w * flux_gccly(cf, st.E, charge);
+  }
+  return 0.0;
+}
+
+__global__ __launch_bounds__(128, 4) void mc_flat_stochastic_kernel(StochArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += stride) {
+    Philox rng;
+    rng.init(a.seed, (uint64_t)e);
+    const double u = rng.uniform();
+    double k, w;
+    const double rk = log(a.emax / a.emin);
+    if (rk > 0) { k = a.emin * exp(rk * u); w = k * rk; }
+    else { k = a.emin; w = 1.0; }
+    const double q = rng.uniform() > 0.5 ? 1.0 : -1.0;
+    w *= 2.0;
+    int err = 0;
+    const double f = flat_event(a, k, q, rng, err);
+    if (err) {
+      atomicOr(a.err, err);
+      atomicMin(a.bad_event, (unsigned long long)e);
+      a.out_w[e] = 0.0;
+    } else {
+      a.out_w[e] = w * f;
+    }
+  }
+}
+
+// ---------------------------------------------------- host preparation
+// Mixture tables (src/Physics.jl:1049-1103) and MixedKernel snapshots
+// (src/MixedTransport.jl:55-97) for a medium, built once per physics object
+// and (components, fractions), kept in device memory.
+struct HostT {
+  const dp_tables_s* t;
+  const double* a(int m, int arr) const {
+    return t->host_data.data() + ((size_t)m * DP_T_NARRAYS + arr) * t->n_knots;
+  }
+  double interp(int m, int arr, double e) const {
+    const int K = t->n_knots;
+    const double* es = a(m, DP_T_ENERGY);
+    const double* les = a(m, DP_T_LOG_ENERGY);
+    const double* ys = a(m, arr);
+    if (e <= es[0]) return ys[0];
+    if (e >= es[K - 1]) return ys[K - 1];
+    const double le = std::log(e);
+    int lo = 0, hi = K - 1;
+    while (lo < hi - 1) {
+      const int mid = (lo + hi) >> 1;
+      if (le < les[mid]) hi = mid; else lo = mid;
+    }
+    const double tt = (le - les[lo]) / (les[lo + 1] - les[lo]);
+    return ys[lo] + tt * (ys[lo + 1] - ys[lo]);
+  }
+};
+
+std::vector<double> medium_key(const std::vector<int>& mats, const std::vector<double>& fr) {
+  std::vector<double> k;
+  for (size_t i = 0; i < mats.size(); ++i) { k.push_back(mats[i]); k.push_back(fr[i]); }
+  return k;
+}
+
+int upload_prepared(dph::Prepared& p) {
+  const size_t nb = p.host.size() * sizeof(double);
+  if (hipMalloc(&p.dev, nb) != hipSuccess) { p.dev = nullptr; return dph::fail(DP_E_HIP, "hipMalloc (prepared tables)"); }
+  if (hipMemcpy(p.dev, p.host.data(), nb, hipMemcpyHostToDevice) != hipSuccess)
+    return dph::fail(DP_E_HIP, "hipMemcpy (prepared tables)");
+  return DP_OK;
+}
+
+// Integrated mixture tables on the grid of material 0: [e, le, crange, mrange].
+int prepare_mixture(dp_tables_s* t, const std::vector<int>& mats, const std::vector<double>& fr,
+                    dph::Prepared** out) {
+  auto key = medium_key(mats, fr);
+  auto it = t->mixtures.find(key);
+  if (it != t->mixtures.end()) { *out = it->second.get(); return DP_OK; }
+  HostT H{t};
+  const int K = t->n_knots;
+  auto p = std::make_unique<dph::Prepared>();
+  p->n = K;
+  p->host.assign((size_t)4 * K, 0.0);
+  double* e = p->host.data();
+  double* le = e + K;
+  double* cr = e + 2 * K;
+  double* mr = e + 3 * K;
+  for (int i = 0; i < K; ++i) { e[i] = H.a(0, DP_T_ENERGY)[i]; le[i] = H.a(0, DP_T_LOG_ENERGY)[i]; }
+  auto stop = [&](bool mixed, double k) {
+    double v = 0;
+    for (size_t j = 0; j < mats.size(); ++j) v += fr[j] * H.interp(mats[j], mixed ? DP_T_MIXED_STOP : DP_T_CSDA_STOP, k);
+    return v;
+  };
+  double cp = stop(false, e[0]), mp = stop(true, e[0]);
+  cr[0] = e[0] / cp;
+  mr[0] = e[0] / mp;
+  for (int i = 1; i < K; ++i) {
+    const double dK = e[i] - e[i - 1];
+    const double c = stop(false, e[i]), m = stop(true, e[i]);
+    cr[i] = cr[i - 1] + dK / ((c + cp) / 2);
+    mr[i] = mr[i - 1] + dK / ((m + mp) / 2);
+    cp = c;
+    mp = m;
+  }
+  int rc = upload_prepared(*p);
+  if (rc) return rc;
+  *out = p.get();
+  t->mixtures[key] = std::move(p);
+  return DP_OK;
+}
+
+// MixedKernel: [e, stop, sigma, range, hazard, proper, el(4 per element)].
+int prepare_kernel(dp_tables_s* t, const std::vector<int>& mats, const std::vector<double>& fr,
+                   dph::Prepared** out) {
+  auto key = medium_key(mats, fr);
+  auto it = t->kernels.find(key);
+  if (it != t->kernels.end()) { *out = it->second.get(); return DP_OK; }
+  HostT H{t};
+  const int K = t->n_knots;
+  const double mass = t->mass;
+  for (int m : mats)
+    if (t->recipes[m].empty())
+      return dph::fail(DP_E_ARGUMENT, "stochastic transport needs the element recipe of material " + std::to_string(m));
+  double lo = -INFINITY, hi = INFINITY;
+  for (int m : mats) { lo = std::max(lo, H.a(m, DP_T_ENERGY)[0]); hi = std::min(hi, H.a(m, DP_T_ENERGY)[K - 1]); }
+  std::vector<double> grid;
+  for (int m : mats)
+    for (int i = 0; i < K; ++i) {
+      const double x = H.a(m, DP_T_ENERGY)[i];
+      if (lo <= x && x <= hi) grid.push_back(x);
+    }
+  std::sort(grid.begin(), grid.end());
+  grid.erase(std::unique(grid.begin(), grid.end()), grid.end());
+  const int refinement = 8;
+  std::vector<double> e;
+  for (size_t j = 0; j + 1 < grid.size(); ++j) {
+    const double a = std::log(grid[j]), b = std::log(grid[j + 1]);
+    for (int r = 0; r < refinement; ++r) e.push_back(std::exp(a + (b - a) * r / refinement));
+  }
+  e.push_back(hi);
+  e[0] = lo;
+  const int n = (int)e.size();
+  std::vector<double> st(n), sg(n), rg(n, 0.0), hz(n, 0.0), pr(n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double s = 0, x = 0;
+    for (size_t q = 0; q < mats.size(); ++q) {
+      s += fr[q] * H.interp(mats[q], DP_T_MIXED_STOP, e[j]);
+      x += fr[q] * H.interp(mats[q], DP_T_CROSS_SECTION, e[j]);
+    }
+    if (!(std::isfinite(s) && s > 0)) return dph::fail(DP_E_ARGUMENT, "invalid restricted stopping power");
+    if (!(std::isfinite(x) && x >= 0)) return dph::fail(DP_E_ARGUMENT, "invalid hard cross section");
+    st[j] = s;
+    sg[j] = x;
+  }
+  double onset = INFINITY;
+  for (double g : grid) {
+    double x = 0;
+    for (size_t q = 0; q < mats.size(); ++q) x += fr[q] * H.interp(mats[q], DP_T_CROSS_SECTION, g);
+    if (x > 0) {
+      onset = g;
+      for (int j = 0; j < n; ++j)
+        if (e[j] < g) sg[j] = x;
+      break;
+    }
+  }
+  for (int j = 1; j < n; ++j) {
+    const double a = e[j - 1], b = e[j], d = b - a;
+    rg[j] = rg[j - 1] + d / 2 * (1 / st[j - 1] + 1 / st[j]);
+    hz[j] = hz[j - 1] + d / 2 * (sg[j - 1] / st[j - 1] + sg[j] / st[j]);
+    pr[j] = pr[j - 1] + d / 2 * mass * (1 / (std::sqrt(a * (a + 2 * mass)) * st[j - 1]) +
+                                        1 / (std::sqrt(b * (b + 2 * mass)) * st[j]));
+  }
+  auto p = std::make_unique<dph::Prepared>();
+  p->n = n;
+  p->pow2 = dph::pow2_for(n);
+  p->onset = onset;
+  p->cutoff = t->cutoff;
+  for (auto* v : {&e, &st, &sg, &rg, &hz, &pr}) p->host.insert(p->host.end(), v->begin(), v->end());
+  int n_el = 0;
+  for (size_t q = 0; q < mats.size(); ++q)
+    for (const dph::Element& el : t->recipes[mats[q]]) {
+      p->host.push_back(el.Z); p->host.push_back(el.A); p->host.push_back(el.I);
+      p->host.push_back(fr[q] * el.f);
+      ++n_el;
+    }
+  p->n_el = n_el;
+  int rc = upload_prepared(*p);
+  if (rc) return rc;
+  *out = p.get();
+  t->kernels[key] = std::move(p);
+  return DP_OK;
+}
+
+int make_medium(dp_tables_s* t, std::vector<int> mats, std::vector<double> fr, Medium& md) {
+  double tot = 0;
+  for (double f : fr) tot += f;
+  if (!(tot > 0)) return dph::fail(DP_E_ARGUMENT, "mixture fractions must sum to a positive value");
+  for (double& f : fr) f /= tot;
+  md.mix.n = (int)mats.size();
+  for (int j = 0; j < 4; ++j) { md.mix.mat[j] = j < md.mix.n ? mats[j] : 0; md.mix.frac[j] = j < md.mix.n ? fr[j] : 0.0; }
+  md.mix.e = md.mix.le = md.mix.crange = md.mix.mrange = nullptr;
+  if (md.mix.n > 1) {
+    dph::Prepared* p;
+    int rc = prepare_mixture(t, mats, fr, &p);
+    if (rc) return rc;
+    const double* b = (const double*)p->dev;
+    const int K = t->n_knots;
+    md.mix.e = b; md.mix.le = b + K; md.mix.crange = b + 2 * K; md.mix.mrange = b + 3 * K;
+  }
+  dph::Prepared* k;
+  int rc = prepare_kernel(t, mats, fr, &k);
+  if (rc) return rc;
+  const double* b = (const double*)k->dev;
+  const int n = k->n;
+  md.kern.n = n; md.kern.pow2 = k->pow2; md.kern.n_el = k->n_el;
+  md.kern.e = b; md.kern.stop = b + n; md.kern.sigma = b + 2 * n; md.kern.range = b + 3 * n;
+  md.kern.hazard = b + 4 * n; md.kern.proper = b + 5 * n; md.kern.el = b + 6 * n;
+  md.kern.onset = k->onset; md.kern.cutoff = k->cutoff;
+  return DP_OK;
+}
+
+__global__ void reset_bad(unsigned long long* b) { if (threadIdx.x == 0) *b = ~0ull; }
 
 // Fixed-shape deterministic reduction of per-event values:
 // each block sums CHUNK consecutive events (every thread a contiguous run,
@@ -263,26 +567,37 @@ __global__ void reset_flags(int* f) { if (threadIdx.x < 4) f[threadIdx.x] = 0; }
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int dp_mc_flat_backward(dp_ctx c, dp_tables t, const dp_flat_geometry* geo,
-                        const dp_mc_config* cfg, dp_mc_result* out) {
+int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_config* cfg,
+           dp_mc_result* out, double* host_events) {
   if (!c || !t || !geo || !cfg || !out) return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: null argument");
   if (cfg->n_events <= 0) return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: n_events must be positive");
   const int M = t->n_materials;
-  if (geo->rock_material < 0 || geo->rock_material >= M || geo->air_material < 0 ||
-      geo->air_material >= M || geo->porous_material >= M)
+  const bool mixture = geo->rock_n_components > 0;
+  if (mixture) {
+    if (geo->rock_n_components > 4) return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: at most 4 rock components");
+    for (int j = 0; j < geo->rock_n_components; ++j)
+      if (geo->rock_components[j] < 0 || geo->rock_components[j] >= M || !(geo->rock_fractions[j] >= 0))
+        return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: invalid rock mixture component");
+  } else if (geo->rock_material < 0 || geo->rock_material >= M) {
+    return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: material index out of range");
+  }
+  if (geo->air_material < 0 || geo->air_material >= M || geo->porous_material >= M)
     return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: material index out of range");
   if (!(cfg->energy_min > 0) || !(cfg->energy_max >= cfg->energy_min))
     return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: need 0 < energy_min <= energy_max");
+  if (!(geo->rock_density > 0) || !(geo->primary_altitude > geo->rock_thickness))
+    return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: need a positive density and primary altitude above the rock");
   double supported = t->last_energy[0];
   for (double e : t->last_energy) supported = e < supported ? e : supported;
-  double thr = cfg->energy_threshold > 0 ? cfg->energy_threshold : fmin(1e9, supported);
+  const double thr = cfg->energy_threshold > 0 ? cfg->energy_threshold : fmin(1e9, supported);
   // compute_flux_single, src/Geometry.jl:725-727
   if (!(thr > 0 && thr <= supported))
     return dph::fail(DP_E_ARGUMENT, "source_energy_max must be positive and within the physics energy grid");
-  if (cfg->straggling || cfg->scattering)
-    return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: straggled/scattering transport needs prepared mixed kernels (dp_mc_flat_straggled)");
+  const bool stochastic = cfg->straggling || cfg->scattering;
+  if (cfg->compute_gradient && mixture)
+    return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward: the CSDA gradient surrogate needs a single rock material");
   DP_HIP(hipSetDevice(c->device));
   const int64_t n = cfg->n_events;
   const int64_t nparts = (n + RED_CHUNK - 1) / RED_CHUNK + 1;
@@ -291,38 +606,80 @@ int dp_mc_flat_backward(dp_ctx c, dp_tables t, const dp_flat_geometry* geo,
   size_t o_buf = dph::align_up(o_gd + 8 * (size_t)n, 256);
   size_t o_res = dph::align_up(o_buf + 8 * 4 * (size_t)nparts, 256);
   size_t o_flag = o_res + 256;
-  int rc = dph::ensure(c->scratch, o_flag + 256);
+  size_t o_bad = o_flag + 256;
+  int rc = dph::ensure(c->scratch, o_bad + 256);
   if (rc) return rc;
   char* b = (char*)c->scratch.ptr;
+  int* flags = (int*)(b + o_flag);
+  unsigned long long* bad = (unsigned long long*)(b + o_bad);
+  double* out_w = (double*)(b + o_w);
+  const double th = (90.0 - cfg->elevation) * M_PI / 180.0;
+  const double cos_t = cfg->cos_theta != 0 || cfg->sin_theta != 0 ? cfg->cos_theta : std::cos(th);
+  const double sin_t = cfg->cos_theta != 0 || cfg->sin_theta != 0 ? cfg->sin_theta : std::sin(th);
+  hipLaunchKernelGGL(reset_flags, dim3(1), dim3(64), 0, c->stream, flags);
+  hipLaunchKernelGGL(reset_bad, dim3(1), dim3(64), 0, c->stream, bad);
+  const int rock = mixture ? geo->rock_components[0] : geo->rock_material;
   McArgs a;
   a.tab = t->data; a.K = t->n_knots; a.pow2 = t->pow2; a.mass = t->mass; a.ctau = t->ctau;
-  a.g = {geo->rock_thickness, geo->rock_density, geo->rock_material, geo->air_material,
+  a.g = {geo->rock_thickness, geo->rock_density, rock, geo->air_material,
          geo->porous_material, geo->porous_density, geo->porous_thickness, geo->primary_altitude};
-  a.gg = {geo->rock_thickness, geo->rock_density, geo->rock_material, geo->air_material, -1, 0.0,
-          0.0, 3e4};
-  a.same_geo = !(geo->porous_material >= 0 && geo->porous_thickness > 0) &&
-               geo->primary_altitude == 3e4 && !cfg->scattering;
-  a.cos_t = cfg->cos_theta; a.emin = cfg->energy_min; a.emax = cfg->energy_max;
+  a.gg = {geo->rock_thickness, geo->rock_density, rock, geo->air_material, -1, 0.0, 0.0, 3e4};
+  a.same_geo = !stochastic && !(geo->porous_material >= 0 && geo->porous_thickness > 0) &&
+               geo->primary_altitude == 3e4;
+  a.cos_t = cos_t; a.emin = cfg->energy_min; a.emax = cfg->energy_max;
   a.thr = thr; a.sw = cfg->mode_switch; a.n = n; a.seed = cfg->seed;
-  a.grad = cfg->compute_gradient != 0; a.scat_limit = cfg->scattering != 0;
-  a.out_w = (double*)(b + o_w); a.out_gf = (double*)(b + o_gf); a.out_gd = (double*)(b + o_gd);
-  int* flags = (int*)(b + o_flag);
+  a.grad = cfg->compute_gradient != 0; a.scat_limit = false;
+  a.out_w = stochastic ? nullptr : out_w;
+  a.out_gf = (double*)(b + o_gf); a.out_gd = (double*)(b + o_gd);
   a.err = flags;
-  size_t lds = (size_t)3 * MC_ARRS * t->n_knots * sizeof(double);
-  int occ = 0;
-  DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_csda_kernel, 256, lds));
-  if (occ < 1) occ = 1;
-  int64_t want = (n + 255) / 256;
-  int64_t grid = (int64_t)c->n_cu * occ * 4;
-  if (grid > want) grid = want;
-  hipLaunchKernelGGL(reset_flags, dim3(1), dim3(64), 0, c->stream, flags);
-  (void)hipEventRecord(c->ev0, c->stream);
-  hipLaunchKernelGGL(mc_flat_csda_kernel, dim3((unsigned)grid), dim3(256), lds, c->stream, a);
-  (void)hipEventRecord(c->ev1, c->stream);
-  DP_HIP(hipGetLastError());
+  if (stochastic) {
+    StochArgs s;
+    s.T = {t->data, t->scalars, t->n_knots, t->pow2, t->mass, t->ctau, t->elastic_ratio};
+    std::vector<int> rm;
+    std::vector<double> rf;
+    if (mixture) {
+      for (int j = 0; j < geo->rock_n_components; ++j) { rm.push_back(geo->rock_components[j]); rf.push_back(geo->rock_fractions[j]); }
+    } else {
+      rm = {geo->rock_material}; rf = {1.0};
+    }
+    if ((rc = make_medium(t, rm, rf, s.rock))) return rc;
+    if ((rc = make_medium(t, {geo->air_material}, {1.0}, s.air))) return rc;
+    s.has_porous = geo->porous_material >= 0 && geo->porous_thickness > 0;
+    if ((rc = make_medium(t, {s.has_porous ? geo->porous_material : rm[0]}, {1.0}, s.porous))) return rc;
+    s.thick = geo->rock_thickness; s.rock_rho = geo->rock_density;
+    s.porous_rho = geo->porous_density; s.porous_thick = geo->porous_thickness;
+    s.palt = geo->primary_altitude;
+    s.cos_t = cos_t; s.sin_t = sin_t; s.emin = cfg->energy_min; s.emax = cfg->energy_max;
+    s.thr = thr; s.sw = cfg->mode_switch;
+    s.straggling = cfg->straggling != 0; s.scattering = cfg->scattering != 0;
+    s.n = n; s.seed = cfg->seed; s.out_w = out_w; s.err = flags; s.bad_event = bad;
+    int occ = 0;
+    DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel, 128, 0));
+    if (occ < 1) occ = 1;
+    int64_t want = (n + 127) / 128;
+    int64_t grid = (int64_t)c->n_cu * occ * 4;
+    if (grid > want) grid = want;
+    (void)hipEventRecord(c->ev0, c->stream);
+    hipLaunchKernelGGL(mc_flat_stochastic_kernel, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
+    (void)hipEventRecord(c->ev1, c->stream);
+    DP_HIP(hipGetLastError());
+  }
+  if (!stochastic || a.grad) {
+    size_t lds = (size_t)3 * MC_ARRS * t->n_knots * sizeof(double);
+    int occ = 0;
+    DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_csda_kernel, 256, lds));
+    if (occ < 1) occ = 1;
+    int64_t want = (n + 255) / 256;
+    int64_t grid = (int64_t)c->n_cu * occ * 4;
+    if (grid > want) grid = want;
+    if (!stochastic) (void)hipEventRecord(c->ev0, c->stream);
+    hipLaunchKernelGGL(mc_flat_csda_kernel, dim3((unsigned)grid), dim3(256), lds, c->stream, a);
+    if (!stochastic) (void)hipEventRecord(c->ev1, c->stream);
+    DP_HIP(hipGetLastError());
+  }
   double* res = (double*)(b + o_res);
   double* buf = (double*)(b + o_buf);
-  rc = reduce_pair(c->stream, a.out_w, nullptr, n, true, buf, res);
+  rc = reduce_pair(c->stream, out_w, nullptr, n, true, buf, res);
   if (rc) return rc;
   if (a.grad) {
     rc = reduce_pair(c->stream, a.out_gf, a.out_gd, n, false, buf, res + 2);
@@ -330,20 +687,41 @@ int dp_mc_flat_backward(dp_ctx c, dp_tables t, const dp_flat_geometry* geo,
   }
   double h[4] = {0, 0, 0, 0};
   int hf[4] = {0, 0, 0, 0};
+  unsigned long long hb = 0;
   DP_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   DP_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, c->stream));
+  DP_HIP(hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+  if (host_events) DP_HIP(hipMemcpyAsync(host_events, out_w, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
   DP_HIP(hipStreamSynchronize(c->stream));
   (void)hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1);
-  if (hf[0] & 2) return dph::fail(DP_E_EXHAUSTED, "dp_mc_flat_backward: outer stepping cap reached");
+  if (hf[0] & ERR_DOMAIN)
+    return dph::fail(DP_E_DOMAIN, "energy outside a prepared mixed kernel or table (first failing event " + std::to_string(hb) + ")");
+  if (hf[0] & (ERR_TRANSPORT | ERR_CAP))
+    return dph::fail(DP_E_EXHAUSTED, "transport cap or invalid hard-event weight (first failing event " + std::to_string(hb) + ")");
   const double dn = (double)n;
   out->flux = h[0] / dn;
-  double var = h[1] / dn - out->flux * out->flux;
+  const double var = h[1] / dn - out->flux * out->flux;
   out->sigma = geo->rock_thickness > 0 ? sqrt(fmax(var, 0.0) / dn) : 0.0;
   out->gradient_flux = a.grad ? h[2] / dn : NAN;
   out->gradient = a.grad ? h[3] / dn : NAN;
   out->source_energy_max = thr;
   out->n_events = n;
   return DP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dp_mc_flat_backward(dp_ctx c, dp_tables t, const dp_flat_geometry* geo,
+                        const dp_mc_config* cfg, dp_mc_result* out) {
+  return mc_run(c, t, geo, cfg, out, nullptr);
+}
+
+int dp_mc_flat_backward_events(dp_ctx c, dp_tables t, const dp_flat_geometry* geo,
+                               const dp_mc_config* cfg, dp_mc_result* out, double* ev) {
+  if (!ev) return dph::fail(DP_E_ARGUMENT, "dp_mc_flat_backward_events: null event buffer");
+  return mc_run(c, t, geo, cfg, out, ev);
 }
 
 }  // extern "C"

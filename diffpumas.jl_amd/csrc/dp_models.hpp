@@ -15,8 +15,10 @@
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
 #define DPM_HD __host__ __device__ inline
+#define DPM_HD_COLD __host__ __device__ inline __attribute__((noinline))
 #else
 #define DPM_HD inline
+#define DPM_HD_COLD inline
 #endif
 
 namespace dpm {
@@ -103,6 +105,7 @@ DPM_HD double dilog(double x) {
     A = PI6 + 0.5 * l * l;
   }
   double H = Y + Y - 1.0, ALFA = H + H, B1 = 0.0, B2 = 0.0;
+#pragma unroll
   for (int i = 19; i >= 0; --i) {
     double B0 = c[i] + ALFA * B1 - B2;
     B2 = B1;
@@ -111,22 +114,25 @@ DPM_HD double dilog(double x) {
   return -(S * (B1 - H * B2) + A);
 }
 
-// radiation_logarithm, src/Materials.jl:592-620 (0 = use the 182.7 default)
+// radiation_logarithm, src/Materials.jl:592-620 (elements without a tabulated
+// value use 182.7)
 DPM_HD double radiation_logarithm(double Z) {
-  const double t[92] = {202.4, 151.9, 159.9, 172.3, 177.9, 178.3, 176.6, 173.4, 170.0, 165.8,
-                        165.8, 167.1, 169.1, 170.8, 172.2, 173.4, 174.3, 174.8, 175.1, 175.6,
-                        176.2, 176.8, 0, 0, 0, 175.8, 0, 0, 173.1, 0, 0, 173.0, 0, 0, 173.5,
-                        0, 0, 0, 0, 0, 0, 175.9, 0, 0, 0, 0, 0, 0, 0, 177.4, 0, 0, 178.6, 0,
-                        0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 177.6, 0, 0,
-                        0, 0, 0, 0, 0, 178.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 179.8};
-  const int i = (int)rint(Z);
-  if (i >= 1 && i <= 92 && t[i - 1] > 0.0) return t[i - 1];
-  return 182.7;
+  switch ((int)rint(Z)) {
+    case 1: return 202.4; case 2: return 151.9; case 3: return 159.9; case 4: return 172.3;
+    case 5: return 177.9; case 6: return 178.3; case 7: return 176.6; case 8: return 173.4;
+    case 9: return 170.0; case 10: return 165.8; case 11: return 165.8; case 12: return 167.1;
+    case 13: return 169.1; case 14: return 170.8; case 15: return 172.2; case 16: return 173.4;
+    case 17: return 174.3; case 18: return 174.8; case 19: return 175.1; case 20: return 175.6;
+    case 21: return 176.2; case 22: return 176.8; case 26: return 175.8; case 29: return 173.1;
+    case 32: return 173.0; case 35: return 173.5; case 42: return 175.9; case 50: return 177.4;
+    case 53: return 178.6; case 74: return 177.6; case 82: return 178.0; case 92: return 179.8;
+    default: return 182.7;
+  }
 }
 
 // ------------------------------------------------------ bremsstrahlung
 // dcs_bremsstrahlung_ssr, src/Materials.jl:631-695
-DPM_HD double dcs_brems(double Z, double A, double m, double K, double q) {
+DPM_HD_COLD double dcs_brems(double Z, double A, double m, double K, double q) {
   const double SQRTE = 1.648721270700128, ME = 0.5109989461, RE = 2.8179403227e-13,
                MMU = 105.6583745;
   if (Z <= 0 || A <= 0 || m <= 0 || K <= 0 || q <= 0) return 0.0;
@@ -257,7 +263,7 @@ DPM_HD double dcs_pair_d2(double Z, double A, double mass, double K, double q, d
 }
 
 // dcs_pair_production_ssr, src/Materials.jl:802-832
-DPM_HD double dcs_pair(double Z, double A, double mass, double K, double q) {
+DPM_HD_COLD double dcs_pair(double Z, double A, double mass, double K, double q) {
   if (Z <= 0 || A <= 0 || mass <= 0 || K <= 0 || q <= 0) return 0.0;
   if (q <= 4.0 * ELECTRON_MASS) return 0.0;
   const double sqrte = 1.6487212707;
@@ -272,6 +278,7 @@ DPM_HD double dcs_pair(double Z, double A, double mass, double K, double q) {
   if (ri <= 0.0 || ri >= 1.0) return 0.0;
   const double tmin = log(ri);
   double I = 0.0;
+#pragma unroll
   for (int i = 0; i < 12; ++i) {
     const double r = exp(gq12_x(i) * tmin);
     I -= dcs_pair_d2(Z, A, mass, K, q, r) * r * gq12_w(i) * tmin;
@@ -327,7 +334,7 @@ DPM_HD double photo_d2(double Z, double A, double ml, double K, double q, double
   return cf * F2A * dds / q;
 }
 // dcs_photonuclear_drss, src/Materials.jl:897-926
-DPM_HD double dcs_photo(double Z, double A, double m, double K, double q) {
+DPM_HD_COLD double dcs_photo(double Z, double A, double m, double K, double q) {
   if (Z <= 0 || A <= 0 || m <= 0 || K <= 0 || q <= 0) return 0.0;
   const double M = 0.5 * (NEUTRON_MASS + PROTON_MASS);
   const double mpi = PION_MASS;
@@ -342,6 +349,7 @@ DPM_HD double dcs_photo(double Z, double A, double m, double K, double q) {
   const double lmin = log(Q2min);
   const double dx = log(Q2max) - lmin;
   double ds = 0.0;
+#pragma unroll
   for (int i = 0; i < 9; ++i) {
     const double Q2 = exp(lmin + dx * gq9_x(i));
     const double v = photo_d2(Z, A, m, K, q, Q2);
@@ -352,7 +360,7 @@ DPM_HD double dcs_photo(double Z, double A, double m, double K, double q) {
 
 // -------------------------------------------------------- ionisation
 // electronic_dcs, src/Materials.jl:534-548
-DPM_HD double dcs_ionisation(double Z, double I, double m, double K, double q) {
+DPM_HD_COLD double dcs_ionisation(double Z, double I, double m, double K, double q) {
   const double P2 = K * (K + 2.0 * m);
   const double E = K + m;
   const double Wmax = 2.0 * ELECTRON_MASS * P2 / (m * m + ELECTRON_MASS * (ELECTRON_MASS + 2.0 * E));
@@ -432,6 +440,7 @@ DPM_HD double dcs_integral(int proc, double Z, double A, double I, double mass, 
     const double a = lql + (it - 1) * (lqh - lql) / nint;
     const double b = lql + it * (lqh - lql) / nint;
     const double dx = b - a;
+#pragma unroll
     for (int j = 0; j < 9; ++j) {
       const double xi = exp(a + dx * gq9_x(j));
       const double wi = gq9_w(j) * dx * xi;

@@ -54,7 +54,10 @@ i32_p = C.POINTER(C.c_int32)
 
 class TableDesc(C.Structure):
     _fields_ = [("n_materials", C.c_int32), ("n_knots", C.c_int32), ("data", dbl_p),
-                ("scalars", dbl_p), ("mass", C.c_double), ("ctau", C.c_double)]
+                ("scalars", dbl_p), ("mass", C.c_double), ("ctau", C.c_double),
+                ("n_elements", i32_p), ("element_Z", dbl_p), ("element_A", dbl_p),
+                ("element_I", dbl_p), ("element_fraction", dbl_p), ("cutoff", C.c_double),
+                ("elastic_ratio", C.c_double)]
 
 
 class CubeGrid(C.Structure):
@@ -81,7 +84,9 @@ class FlatGeometry(C.Structure):
     _fields_ = [("rock_thickness", C.c_double), ("rock_density", C.c_double),
                 ("rock_material", C.c_int32), ("air_material", C.c_int32),
                 ("porous_material", C.c_int32), ("porous_density", C.c_double),
-                ("porous_thickness", C.c_double), ("primary_altitude", C.c_double)]
+                ("porous_thickness", C.c_double), ("primary_altitude", C.c_double),
+                ("rock_n_components", C.c_int32), ("rock_components", C.c_int32 * 4),
+                ("rock_fractions", C.c_double * 4)]
 
 
 class McConfig(C.Structure):
@@ -143,6 +148,8 @@ _SIGS = {
                                        C.c_void_p, C.c_int32, C.c_void_p]),
     "dp_mc_flat_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),
                                       C.POINTER(McConfig), C.POINTER(McResult)]),
+    "dp_mc_flat_backward_events": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),
+                                             C.POINTER(McConfig), C.POINTER(McResult), dbl_p]),
     "dp_last_kernel_ms": (C.c_double, [C.c_void_p]),
     "dp_build_material_table": (C.c_int, [C.POINTER(MaterialRecipe), C.POINTER(DedxColumns),
                                           C.POINTER(PhysicsSettings), dbl_p]),

@@ -22,8 +22,13 @@ def run_backward_mc(physics, rock_thickness=100.0, rock_density=2650.0, elevatio
                     energy_min=1.0, energy_max=None, n_samples=10000, compute_gradient=False,
                     seed=42, straggling=True, scattering=True, energy_threshold_low=100.0,
                     source_energy_max=None, porous_material=-1, porous_density=0.0,
-                    porous_thickness=0.0, primary_altitude=PRIMARY_ALTITUDE, ctx=None):
-    """run_backward_mc (src/Geometry.jl:1286-1338); returns the same fields."""
+                    porous_thickness=0.0, primary_altitude=PRIMARY_ALTITUDE, rock_mixture=None,
+                    return_events=False, ctx=None):
+    """run_backward_mc (src/Geometry.jl:1286-1338); returns the same fields.
+
+    rock_mixture: optional [(material index, mass fraction), ...] runtime
+    mixture for the rock layer (an extension: the reference flat geometry takes
+    one material).  return_events adds each event's contribution."""
     ctx = ctx or _lib.default_context()
     if energy_max is None:
         energy_max = energy_min
@@ -36,6 +41,13 @@ def run_backward_mc(physics, rock_thickness=100.0, rock_density=2650.0, elevatio
     geo = _lib.FlatGeometry(float(rock_thickness), float(rock_density), rock, air,
                             int(porous_material), float(porous_density), float(porous_thickness),
                             float(primary_altitude))
+    if rock_mixture:
+        if len(rock_mixture) > 4:
+            raise _lib.ArgumentError("at most 4 rock mixture components")
+        geo.rock_n_components = len(rock_mixture)
+        for j, (m, f) in enumerate(rock_mixture):
+            geo.rock_components[j] = int(m)
+            geo.rock_fractions[j] = float(f)
     c, s = _zenith(float(elevation))
     cfg = _lib.McConfig(float(elevation), c, s, float(energy_min), float(energy_max),
                         float(source_energy_max) if source_energy_max else 0.0,
@@ -43,13 +55,24 @@ def run_backward_mc(physics, rock_thickness=100.0, rock_density=2650.0, elevatio
                         int(bool(straggling)), int(bool(scattering)), int(bool(compute_gradient)))
     out = _lib.McResult()
     h = physics.device_handle(ctx)
-    _lib.check(ctx.lib.dp_mc_flat_backward(ctx.h, h, C.byref(geo), C.byref(cfg), C.byref(out)))
+    events = None
+    if return_events:
+        import numpy as np
+        events = np.zeros(int(n_samples))
+        _lib.check(ctx.lib.dp_mc_flat_backward_events(ctx.h, h, C.byref(geo), C.byref(cfg),
+                                                      C.byref(out), _lib.ptr(events)))
+    else:
+        _lib.check(ctx.lib.dp_mc_flat_backward(ctx.h, h, C.byref(geo), C.byref(cfg),
+                                               C.byref(out)))
     g = compute_gradient
-    return {"flux": out.flux, "sigma": out.sigma,
+    res = {"flux": out.flux, "sigma": out.sigma,
             "gradient": out.gradient if g else None,
             "gradient_flux": out.gradient_flux if g else None,
             "gradient_method": "deterministic_csda_sample_average" if g else None,
             "source_energy_max": out.source_energy_max}
+    if return_events:
+        res["events"] = events
+    return res
 
 
 def compute_flux(physics, rock_density, rock_thickness, elevation, energy_min, energy_max,

@@ -200,6 +200,8 @@ class PhysicsTables:
         self.ctau = float(ctau)
         self.recipes = recipes or {}
         self.complete = complete  # False: only the CSDA columns are filled
+        self.cutoff = 0.05
+        self.elastic_ratio = 0.05
         self._dev = {}
 
     @property
@@ -225,15 +227,31 @@ class PhysicsTables:
         key = id(ctx)
         h = self._dev.get(key)
         if h is None:
-            import ctypes as C
+            n_el, Z, A, I, f = self.recipe_arrays()
+            self._keep = (n_el, Z, A, I, f)
             desc = _lib.TableDesc(self.n_materials, self.n_knots, _lib.ptr(self.data),
-                                  _lib.ptr(self.scalars), self.mass, self.ctau)
+                                  _lib.ptr(self.scalars), self.mass, self.ctau,
+                                  _lib.ptr(n_el, C.c_int32), _lib.ptr(Z), _lib.ptr(A),
+                                  _lib.ptr(I), _lib.ptr(f), self.cutoff, self.elastic_ratio)
             out = C.c_void_p()
             _lib.check(ctx.lib.dp_upload_tables(ctx.h, C.byref(desc), C.byref(out)))
             h = out
             self._dev[key] = (ctx, h)
             return h
         return h[1]
+
+    def recipe_arrays(self):
+        """Element recipes in the flat layout of dp_table_desc (materials
+        without a recipe get zero elements)."""
+        n_el, Z, A, I, f = [], [], [], [], []
+        for name in self.names:
+            r = self.recipes.get(name)
+            els = list(zip(r.elements, r.fractions)) if r is not None else []
+            n_el.append(len(els))
+            for e, fr in els:
+                Z.append(e.Z); A.append(e.A); I.append(e.I); f.append(fr)
+        return (np.array(n_el, dtype=np.int32), np.array(Z + [0.0]), np.array(A + [0.0]),
+                np.array(I + [0.0]), np.array(f + [0.0]))
 
     def release(self):
         for ctx, h in self._dev.values():
@@ -353,4 +371,6 @@ def create_physics(materials=("StandardRock", "Water", "Air"), composites=(), da
         tabs.append(out)
         scal.append([flat.density, flat.I, flat.ZoA, 0.0])
         recipes[cname] = flat
-    return PhysicsTables(names, np.stack(tabs), np.array(scal), recipes=recipes, complete=True)
+    ph = PhysicsTables(names, np.stack(tabs), np.array(scal), recipes=recipes, complete=True)
+    ph.cutoff, ph.elastic_ratio = cutoff, elastic_ratio
+    return ph

@@ -57,6 +57,16 @@ typedef struct {
   const double* scalars;    /* n_materials * 4: density kg/m3, I GeV, Z/A, 0   */
   double mass;              /* particle mass, GeV                               */
   double ctau;              /* decay length, m                                  */
+  /* Element recipes (needed by stochastic transport: hard-event DCS weights
+   * and targets).  n_elements[m] elements per material, then Z, A, I (GeV)
+   * and mass fraction concatenated in material order.  NULL = none.       */
+  const int32_t* n_elements;
+  const double* element_Z;
+  const double* element_A;
+  const double* element_I;
+  const double* element_fraction;
+  double cutoff;            /* PhysicsSettings.cutoff (0.05)        */
+  double elastic_ratio;     /* PhysicsSettings.elastic_ratio (0.05) */
 } dp_table_desc;
 
 /* ------------------------------------------------------------- lifecycle */
@@ -218,6 +228,11 @@ typedef struct {
   double porous_density;
   double porous_thickness;
   double primary_altitude;
+  /* Optional runtime mixture for the rock layer (MaterialMixture,
+   * src/Types.jl:357-391): rock_n_components > 0 replaces rock_material. */
+  int32_t rock_n_components;
+  int32_t rock_components[4];
+  double rock_fractions[4];
 } dp_flat_geometry;
 
 typedef struct {
@@ -243,9 +258,16 @@ typedef struct {
 
 /* Replaces: _compute_flux_estimate (src/Geometry.jl:818-896) and, with
  * compute_gradient, _sampled_csda_flux_gradient (:933-945) — the body of
- * run_backward_mc (:1286-1338).  CSDA mode (straggling == scattering == 0). */
+ * run_backward_mc (:1286-1338).  straggling selects straggled stepping below
+ * mode_switch and mixed stepping above (transport_backward_through_geometry,
+ * :534-672); scattering adds EHS, DEL deflections and soft multiple
+ * scattering.  If event_weights != NULL it receives each event's
+ * contribution (n_events doubles, host memory). */
 int dp_mc_flat_backward(dp_ctx ctx, dp_tables tables, const dp_flat_geometry* geo,
                         const dp_mc_config* cfg, dp_mc_result* out);
+int dp_mc_flat_backward_events(dp_ctx ctx, dp_tables tables, const dp_flat_geometry* geo,
+                               const dp_mc_config* cfg, dp_mc_result* out,
+                               double* event_weights);
 
 /* Last kernel duration in ms, timed with HIP events on the context's stream
  * around the dominant kernel of the previous call (0 if none). */

@@ -73,6 +73,41 @@ inline Dual pow(Dual a, double p) {
   return {std::pow(a.v, p), p * std::pow(a.v, p - 1.0) * a.d};
 }
 inline Dual abs(Dual a) { return a.v < 0 ? -a : a; }
+// Forward mode with two directions (used for segment Jacobians in (w, E)).
+struct Dual2 {
+  double v, a, b;
+  Dual2() : v(0), a(0), b(0) {}
+  Dual2(double x) : v(x), a(0), b(0) {}
+  Dual2(double x, double da, double db) : v(x), a(da), b(db) {}
+};
+inline double val(const Dual2& x) { return x.v; }
+inline Dual2 operator+(Dual2 x, Dual2 y) { return {x.v + y.v, x.a + y.a, x.b + y.b}; }
+inline Dual2 operator-(Dual2 x, Dual2 y) { return {x.v - y.v, x.a - y.a, x.b - y.b}; }
+inline Dual2 operator-(Dual2 x) { return {-x.v, -x.a, -x.b}; }
+inline Dual2 operator*(Dual2 x, Dual2 y) { return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b}; }
+inline Dual2 operator/(Dual2 x, Dual2 y) {
+  double q = x.v / y.v;
+  return {q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v};
+}
+inline Dual2 operator+(Dual2 x, double y) { return {x.v + y, x.a, x.b}; }
+inline Dual2 operator+(double x, Dual2 y) { return {x + y.v, y.a, y.b}; }
+inline Dual2 operator-(Dual2 x, double y) { return {x.v - y, x.a, x.b}; }
+inline Dual2 operator-(double x, Dual2 y) { return {x - y.v, -y.a, -y.b}; }
+inline Dual2 operator*(Dual2 x, double y) { return {x.v * y, x.a * y, x.b * y}; }
+inline Dual2 operator*(double x, Dual2 y) { return {x * y.v, x * y.a, x * y.b}; }
+inline Dual2 operator/(Dual2 x, double y) { return {x.v / y, x.a / y, x.b / y}; }
+inline Dual2 operator/(double x, Dual2 y) {
+  double q = x / y.v;
+  return {q, -q * y.a / y.v, -q * y.b / y.v};
+}
+inline Dual2 log(Dual2 x) { return {std::log(x.v), x.a / x.v, x.b / x.v}; }
+inline Dual2 exp(Dual2 x) { double e = std::exp(x.v); return {e, e * x.a, e * x.b}; }
+inline Dual2 sqrt(Dual2 x) { double s = std::sqrt(x.v); return {s, x.a / (2.0 * s), x.b / (2.0 * s)}; }
+inline Dual2 pow(Dual2 x, double p) {
+  double d = p * std::pow(x.v, p - 1.0);
+  return {std::pow(x.v, p), d * x.a, d * x.b};
+}
+inline bool isfinite_(const Dual2& x) { return std::isfinite(x.v); }
 inline bool isfinite_(double x) { return std::isfinite(x); }
 inline bool isfinite_(const Dual& x) { return std::isfinite(x.v) && std::isfinite(x.d); }
 using std::log; using std::exp; using std::sqrt; using std::pow; using std::abs;

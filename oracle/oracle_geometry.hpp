@@ -321,4 +321,154 @@ inline R flat_csda_flux(const Tables& T, const Layered& g, R rho, double energy,
   return R(0.0);
 }
 
+
+// ============================================= w-parameterised tomography CSDA
+// MaterialConfig (src/Tomography.jl:179-188), materials 0-based (porous < 0
+// disables the shallow porous recipe) and SiteConfig (:198-201).
+struct MatCfg {
+  int rock, water, air, porous;
+  double rho_rock, rho_water, rho_porous, porous_top;
+};
+struct Site { double detector_elevation, primary_altitude; };
+
+// _csda_min / _csda_max, src/Tomography.jl:421-434: primal comparison with
+// the left operand kept at ties.
+template <class A, class B> inline A cmin(A a, B b) { return val(a) <= val(b) ? a : A(b); }
+template <class A, class B> inline A cmax(A a, B b) { return val(a) >= val(b) ? a : A(b); }
+
+inline bool shallow_porous(bool shallow, const MatCfg& m) {
+  return shallow && m.porous >= 0 && m.porous_top > 0.0;
+}
+// cell_density, src/Tomography.jl:295-302
+template <class R> inline R cell_density(R w, bool shallow, const MatCfg& m) {
+  if (shallow_porous(shallow, m))
+    return 1.0 / (0.5 * (1.0 - w) / m.rho_porous + 0.5 * (1.0 - w) / m.rho_rock + w / m.rho_water);
+  return 1.0 / ((1.0 - w) / m.rho_rock + w / m.rho_water);
+}
+// cell_stopping_power, src/Tomography.jl:304-314
+template <class R, class E> inline R cell_stop(const Tables& T, R w, bool shallow, const MatCfg& m, E e) {
+  if (shallow_porous(shallow, m)) {
+    R sp = prop_stop(T, false, m.porous, R(e)), sr = prop_stop(T, false, m.rock, R(e)),
+      sw = prop_stop(T, false, m.water, R(e));
+    return 0.5 * (1.0 - w) * sp + 0.5 * (1.0 - w) * sr + w * sw;
+  }
+  R sr = prop_stop(T, false, m.rock, R(e)), sw = prop_stop(T, false, m.water, R(e));
+  return (1.0 - w) * sr + w * sw;
+}
+
+constexpr double CSDA_MAX_RELATIVE_GAIN = 0.02;  // examples/lvd_tomography.jl:155
+constexpr double CSDA_MAX_STEP_M = 60.0;         // examples/lvd_tomography.jl:156
+
+// propagate_csda_cell_segment, src/Tomography.jl:444-473 (Heun sub-steps)
+template <class R>
+inline void cell_segment(const Tables& T, R w, bool shallow, const MatCfg& m, double distance,
+                         R& energy, R& weight, bool& ok) {
+  R density = cell_density(w, shallow, m);
+  R remaining = R(distance);
+  ok = true;
+  while (val(remaining) > 1e-8) {
+    R dedx0 = cell_stop(T, w, shallow, m, energy);
+    if (!std::isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); ok = false; return; }
+    R grel = CSDA_MAX_RELATIVE_GAIN * cmax(energy, 1e-6) / dedx0;
+    R gstep = cmin(cmin(remaining * density, cmax(grel, 1e-6)), CSDA_MAX_STEP_M * density);
+    R ds = cmin(remaining, gstep / cmax(density, 1e-12));
+    if (val(ds) <= 0.0) { weight = R(0.0); ok = false; return; }
+    R gram = ds * density;
+    R epred = energy + dedx0 * gram;
+    R dedx1 = cell_stop(T, w, shallow, m, epred);
+    R dE = 0.5 * (dedx0 + dedx1) * gram;
+    R enext = energy + dE;
+    if (!std::isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); ok = false; return; }
+    R dnext = cell_stop(T, w, shallow, m, enext);
+    weight = weight * (dnext / cmax(dedx0, 1e-30));
+    weight = weight * decay_weight(T, ds, 0.5 * (energy + enext));
+    energy = enext;
+    remaining = remaining - ds;
+  }
+}
+
+// propagate_csda_uniform_segment, src/Tomography.jl:481-509
+template <class R>
+inline void uniform_segment(const Tables& T, int mat, double density, double distance, R& energy,
+                            R& weight, bool& ok) {
+  R remaining = R(distance);
+  ok = true;
+  while (val(remaining) > 1e-8) {
+    R dedx0 = prop_stop(T, false, mat, energy);
+    if (!std::isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); ok = false; return; }
+    R grel = CSDA_MAX_RELATIVE_GAIN * cmax(energy, 1e-6) / dedx0;
+    R gstep = cmin(cmin(remaining * density, cmax(grel, 1e-6)), R(CSDA_MAX_STEP_M * density));
+    R ds = cmin(remaining, gstep / std::max(density, 1e-12));
+    if (val(ds) <= 0.0) { weight = R(0.0); ok = false; return; }
+    R gram = ds * density;
+    R epred = energy + dedx0 * gram;
+    R dedx1 = prop_stop(T, false, mat, epred);
+    R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
+    if (!std::isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); ok = false; return; }
+    R dnext = prop_stop(T, false, mat, enext);
+    weight = weight * (dnext / cmax(dedx0, 1e-30));
+    weight = weight * decay_weight(T, ds, 0.5 * (energy + enext));
+    energy = enext;
+    remaining = remaining - ds;
+  }
+}
+
+// propagate_csda_air_segment, src/Tomography.jl:516-551
+template <class R>
+inline void air_segment(const Tables& T, int air, const Site& site, double start_z, double cos_t,
+                        double distance, R& energy, R& weight, bool& ok) {
+  R remaining = R(distance);
+  R traversed = R(0.0);
+  ok = true;
+  while (val(remaining) > 1e-8) {
+    R dedx0 = prop_stop(T, false, air, energy);
+    if (!std::isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); ok = false; return; }
+    R grel = CSDA_MAX_RELATIVE_GAIN * cmax(energy, 1e-6) / dedx0;
+    // the first density only feeds the step choice through the same formula
+    R alt0 = site.detector_elevation + start_z + cos_t * (traversed + 0.5 * cmin(remaining, CSDA_MAX_STEP_M));
+    R dens0 = 1.205 * exp(-alt0 / 12e3);
+    R ds = cmin(cmin(remaining, CSDA_MAX_STEP_M), grel / cmax(dens0, 1e-12));
+    ds = cmax(ds, 1e-3);
+    R alt = site.detector_elevation + start_z + cos_t * (traversed + 0.5 * ds);
+    R density = 1.205 * exp(-alt / 12e3);
+    R gram = ds * density;
+    R epred = energy + dedx0 * gram;
+    R dedx1 = prop_stop(T, false, air, epred);
+    R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
+    if (!std::isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); ok = false; return; }
+    R dnext = prop_stop(T, false, air, enext);
+    weight = weight * (dnext / cmax(dedx0, 1e-30));
+    weight = weight * decay_weight(T, ds, 0.5 * (energy + enext));
+    energy = enext;
+    traversed = traversed + ds;
+    remaining = remaining - ds;
+  }
+}
+
+// compute_directional_flux_csda with the corrections disabled (their default,
+// src/Tomography.jl:139,167): sum over energy nodes of 2 w_n flux / N.
+// R is the water-fraction type (double, or a dual seeded on cells).
+template <class R>
+inline R directional_flux_w(const Tables& T, const MatCfg& m, const Site& site, const int* cells,
+                            const double* lens, int nseg, double remaining_rock, double exit_z,
+                            double cos_t, const R* w, const uint8_t* shallow, int n_nodes,
+                            const double* ne, const double* nw, const double* nq) {
+  R sum = R(0.0);
+  for (int n = 0; n < n_nodes; ++n) {
+    R energy = R(ne[n]);
+    R weight = R(1.0);
+    bool ok = true;
+    for (int s = 0; s < nseg && ok; ++s)
+      cell_segment(T, w[cells[s]], shallow[cells[s]] != 0, m, lens[s], energy, weight, ok);
+    if (ok && remaining_rock > 1e-8) uniform_segment(T, m.rock, m.rho_rock, remaining_rock, energy, weight, ok);
+    if (ok) {
+      const double air_d = std::max(0.0, site.primary_altitude - exit_z) / std::max(cos_t, 1e-3);
+      air_segment(T, m.air, site, exit_z, cos_t, air_d, energy, weight, ok);
+    }
+    R fs = ok ? weight * flux_gccly(cos_t, energy, nq[n]) : R(0.0);
+    sum = sum + 2.0 * nw[n] * fs;
+  }
+  return sum / (double)n_nodes;
+}
+
 }  // namespace orc
