@@ -260,6 +260,45 @@ __device__ __forceinline__ DualD sqrt(DualD a) { double s = ::sqrt(a.v); return 
 __device__ __forceinline__ DualD pow(DualD a, double p) {
   return {::pow(a.v, p), p * ::pow(a.v, p - 1.0) * a.d};
 }
+// Two tangent directions (segment Jacobians in (w, E_in)).
+struct D2 {
+  double v, a, b;
+  __device__ D2() : v(0), a(0), b(0) {}
+  __device__ D2(double x) : v(x), a(0), b(0) {}
+  __device__ D2(double x, double da, double db) : v(x), a(da), b(db) {}
+};
+__device__ __forceinline__ double val(const D2& x) { return x.v; }
+__device__ __forceinline__ D2 operator+(D2 x, D2 y) { return {x.v + y.v, x.a + y.a, x.b + y.b}; }
+__device__ __forceinline__ D2 operator-(D2 x, D2 y) { return {x.v - y.v, x.a - y.a, x.b - y.b}; }
+__device__ __forceinline__ D2 operator-(D2 x) { return {-x.v, -x.a, -x.b}; }
+__device__ __forceinline__ D2 operator*(D2 x, D2 y) {
+  return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b};
+}
+__device__ __forceinline__ D2 operator/(D2 x, D2 y) {
+  double q = x.v / y.v;
+  return {q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v};
+}
+__device__ __forceinline__ D2 operator+(D2 x, double y) { return {x.v + y, x.a, x.b}; }
+__device__ __forceinline__ D2 operator+(double x, D2 y) { return {x + y.v, y.a, y.b}; }
+__device__ __forceinline__ D2 operator-(D2 x, double y) { return {x.v - y, x.a, x.b}; }
+__device__ __forceinline__ D2 operator-(double x, D2 y) { return {x - y.v, -y.a, -y.b}; }
+__device__ __forceinline__ D2 operator*(D2 x, double y) { return {x.v * y, x.a * y, x.b * y}; }
+__device__ __forceinline__ D2 operator*(double x, D2 y) { return {x * y.v, x * y.a, x * y.b}; }
+__device__ __forceinline__ D2 operator/(D2 x, double y) { return {x.v / y, x.a / y, x.b / y}; }
+__device__ __forceinline__ D2 operator/(double x, D2 y) {
+  double q = x / y.v;
+  return {q, -q * y.a / y.v, -q * y.b / y.v};
+}
+__device__ __forceinline__ D2 log(D2 x) { return {::log(x.v), x.a / x.v, x.b / x.v}; }
+__device__ __forceinline__ D2 exp(D2 x) { double e = ::exp(x.v); return {e, e * x.a, e * x.b}; }
+__device__ __forceinline__ D2 sqrt(D2 x) {
+  double s = ::sqrt(x.v);
+  return {s, x.a / (2.0 * s), x.b / (2.0 * s)};
+}
+__device__ __forceinline__ D2 pow(D2 x, double p) {
+  double d = p * ::pow(x.v, p - 1.0);
+  return {::pow(x.v, p), d * x.a, d * x.b};
+}
 using ::log;
 using ::exp;
 using ::sqrt;

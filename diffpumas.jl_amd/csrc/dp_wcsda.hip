@@ -1,0 +1,584 @@
+// Water-fraction (w) tomography CSDA: directional flux and its sparse
+// Jacobian d flux / d w.
+//
+// Work decomposition: persistent blocks take one path (bin) at a time from a
+// work queue; the block's lanes are the energy samples.  Every lane walks the
+// same segment list, so the segment data are wave-uniform loads.  Rock,
+// water, porous and air stopping-power tables are staged in LDS.
+//
+// Per sample (lane) the forward sweep runs the Heun sub-stepping of
+// propagate_csda_cell_segment (src/Tomography.jl:444-473) on each cell with
+// a two-direction dual (w, E_in), giving the segment partials
+//   ew = dE_out/dw, ee = dE_out/dE_in, lw = dlog G/dw, le = dlog G/dE_in
+// (G = the segment's weight factor), i.e. _cell_segment_partials of
+// src/Tomography_rules.jl:33-48.  The dry-rock remainder and the atmosphere
+// (:481-551) carry one energy tangent.  The reverse sweep chains them:
+//   d flux/d w_s = f (lw_s + T ew_s),   T <- le_s + T ee_s
+// with T = d log(flux)/dE at the exit of segment s and f the sample's flux
+// term, which is the cotangent algebra of the rrules written for a
+// multiplicative weight.  Per-segment sums over samples use a fixed wave/LDS
+// tree, so the Jacobian is deterministic and independent of launch shape.
+#include "dp_internal.hpp"
+#include "dp_device.hpp"
+#include <cstring>
+
+using namespace dpk;
+
+namespace {
+
+constexpr double MAX_REL_GAIN = 0.02;  // CSDA_MAX_RELATIVE_GAIN, src/Tomography.jl:90
+constexpr double MAX_STEP_M = 60.0;    // CSDA_MAX_STEP_M, src/Tomography.jl:91
+constexpr int ERR_CAP = 1, ERR_CELL = 2;
+
+// _csda_min / _csda_max (src/Tomography.jl:421-434): branch on primal values,
+// the left operand wins ties.
+template <class A, class B>
+__device__ __forceinline__ A cmin(A a, B b) { return val(a) <= val(b) ? a : A(b); }
+template <class A, class B>
+__device__ __forceinline__ A cmax(A a, B b) { return val(a) >= val(b) ? a : A(b); }
+
+struct WTab {
+  const double *rE, *rLE, *rS;  // rock
+  const double *wE, *wLE, *wS;  // water
+  const double *pE, *pLE, *pS;  // porous (aliases rock when unused)
+  const double *aE, *aLE, *aS;  // air
+  int K, pow2, shared;          // shared: rock/water/porous energy grids identical
+  double mass, ctau;
+};
+struct WMat {
+  double rho_r, rho_w, rho_p;
+  int porous_on;  // porous material set and porous_top_thickness > 0
+};
+
+template <class R>
+__device__ __forceinline__ R lerp_at(const double* ys, int i, R t) {
+  return ys[i] + t * (ys[i + 1] - ys[i]);
+}
+
+// cell_stopping_power, src/Tomography.jl:304-314.  With a shared energy grid
+// the three interpolations reuse one clamp test and one bracket search, which
+// returns the same values as three separate lookups.
+template <class R>
+__device__ __forceinline__ R cell_stop(const WTab& T, R w, bool porous, R e) {
+  const int K = T.K;
+  R le = log(e);
+  R sr, sw, sp;
+  if (T.shared) {
+    if (val(e) <= T.rE[0]) {
+      sr = R(T.rS[0]); sw = R(T.wS[0]); sp = R(T.pS[0]);
+    } else if (val(e) >= T.rE[K - 1]) {
+      sr = R(T.rS[K - 1]); sw = R(T.wS[K - 1]); sp = R(T.pS[K - 1]);
+    } else {
+      const int i = bracket(T.rLE, K, T.pow2, val(le));
+      R t = (le - T.rLE[i]) / (T.rLE[i + 1] - T.rLE[i]);
+      sr = lerp_at(T.rS, i, t);
+      sw = lerp_at(T.wS, i, t);
+      sp = porous ? lerp_at(T.pS, i, t) : R(0.0);
+    }
+  } else {
+    sr = interp_r(T.rE, T.rLE, T.rS, K, T.pow2, e, le);
+    sw = interp_r(T.wE, T.wLE, T.wS, K, T.pow2, e, le);
+    sp = porous ? interp_r(T.pE, T.pLE, T.pS, K, T.pow2, e, le) : R(0.0);
+  }
+  if (porous) return 0.5 * (1.0 - w) * sp + 0.5 * (1.0 - w) * sr + w * sw;
+  return (1.0 - w) * sr + w * sw;
+}
+
+// cell_density, src/Tomography.jl:295-302 (harmonic in the mass fraction).
+template <class R>
+__device__ __forceinline__ R cell_density(const WMat& m, R w, bool porous) {
+  if (porous) return 1.0 / (0.5 * (1.0 - w) / m.rho_p + 0.5 * (1.0 - w) / m.rho_r + w / m.rho_w);
+  return 1.0 / ((1.0 - w) / m.rho_r + w / m.rho_w);
+}
+
+// compute_decay_weight_from_path, src/Geometry.jl:1083-1107 (the beta floor
+// branch is chosen on the primal value).
+template <class R, class P>
+__device__ __forceinline__ R decay_r(const WTab& T, P path, R eavg) {
+  R g = 1.0 + eavg / T.mass;
+  R b2 = 1.0 - 1.0 / (g * g);
+  R beta = sqrt(val(b2) >= 1e-12 ? b2 : R(1e-12));
+  R L = beta * g * T.ctau;
+  return exp(-path / L);
+}
+
+template <class R>
+__device__ __forceinline__ R air_stop(const WTab& T, R e) {
+  return interp_r(T.aE, T.aLE, T.aS, T.K, T.pow2, e, log(e));
+}
+template <class R>
+__device__ __forceinline__ R rock_stop(const WTab& T, R e) {
+  return interp_r(T.rE, T.rLE, T.rS, T.K, T.pow2, e, log(e));
+}
+
+// propagate_csda_cell_segment, src/Tomography.jl:444-473.  The stopping power
+// at the end of a sub-step is the next sub-step's starting value (same
+// function of the same energy), so it is evaluated once.
+template <class R>
+__device__ bool cell_segment(const WTab& T, const WMat& m, R w, bool porous, double distance,
+                             R& energy, R& weight) {
+  R density = cell_density(m, w, porous);
+  R remaining = R(distance);
+  R dedx0 = cell_stop(T, w, porous, energy);
+  while (val(remaining) > 1e-8) {
+    if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
+    R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
+    R gstep = cmin(cmin(remaining * density, cmax(grel, 1e-6)), MAX_STEP_M * density);
+    R ds = cmin(remaining, gstep / cmax(density, 1e-12));
+    if (val(ds) <= 0.0) { weight = R(0.0); return false; }
+    R gram = ds * density;
+    R epred = energy + dedx0 * gram;
+    R dedx1 = cell_stop(T, w, porous, epred);
+    R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
+    if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
+    R dnext = cell_stop(T, w, porous, enext);
+    weight = weight * (dnext / cmax(dedx0, 1e-30));
+    weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
+    energy = enext;
+    remaining = remaining - ds;
+    dedx0 = dnext;
+  }
+  return true;
+}
+
+// propagate_csda_uniform_segment, src/Tomography.jl:481-509 (rock remainder).
+template <class R>
+__device__ bool rock_segment(const WTab& T, double density, double distance, R& energy, R& weight) {
+  R remaining = R(distance);
+  R dedx0 = rock_stop(T, energy);
+  while (val(remaining) > 1e-8) {
+    if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
+    R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
+    R gstep = cmin(cmin(remaining * density, cmax(grel, 1e-6)), R(MAX_STEP_M * density));
+    R ds = cmin(remaining, gstep / fmax(density, 1e-12));
+    if (val(ds) <= 0.0) { weight = R(0.0); return false; }
+    R gram = ds * density;
+    R epred = energy + dedx0 * gram;
+    R dedx1 = rock_stop(T, epred);
+    R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
+    if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
+    R dnext = rock_stop(T, enext);
+    weight = weight * (dnext / cmax(dedx0, 1e-30));
+    weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
+    energy = enext;
+    remaining = remaining - ds;
+    dedx0 = dnext;
+  }
+  return true;
+}
+
+// propagate_csda_air_segment, src/Tomography.jl:516-551: exponential
+// atmosphere at the mid-step altitude (locals_air_at_altitude,
+// src/Geometry.jl:95-106), 1 mm minimum step.
+template <class R>
+__device__ bool air_segment(const WTab& T, double z0, double cos_t, double distance, R& energy,
+                            R& weight) {
+  R remaining = R(distance);
+  R traversed = R(0.0);
+  R dedx0 = air_stop(T, energy);
+  while (val(remaining) > 1e-8) {
+    if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
+    R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
+    R alt0 = z0 + cos_t * (traversed + 0.5 * cmin(remaining, MAX_STEP_M));
+    R dens0 = 1.205 * exp(-alt0 / 12e3);
+    R ds = cmin(cmin(remaining, MAX_STEP_M), grel / cmax(dens0, 1e-12));
+    ds = cmax(ds, 1e-3);
+    R alt = z0 + cos_t * (traversed + 0.5 * ds);
+    R density = 1.205 * exp(-alt / 12e3);
+    R gram = ds * density;
+    R epred = energy + dedx0 * gram;
+    R dedx1 = air_stop(T, epred);
+    R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
+    if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
+    R dnext = air_stop(T, enext);
+    weight = weight * (dnext / cmax(dedx0, 1e-30));
+    weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
+    energy = enext;
+    traversed = traversed + ds;
+    remaining = remaining - ds;
+    dedx0 = dnext;
+  }
+  return true;
+}
+
+struct WArgs {
+  const double* tab;
+  int K, pow2, shared;
+  int rock, water, air, porous;
+  WMat mat;
+  double det_elev, palt, mass, ctau;
+  int n_rays, max_seg, n_cells;
+  const int32_t* row_ptr;
+  const int32_t* cell;
+  const double* len;
+  const double* cos_t;
+  const double* remaining;
+  const double* exit_z;
+  const uint8_t* valid;
+  const double* w;
+  const uint8_t* shallow;
+  int n_nodes;
+  const double* ne;
+  const double* nw;
+  const double* nq;
+  double* flux;
+  double* jac;
+  double* scratch;
+  int* queue;
+  int* err;
+};
+
+template <bool JAC>
+__global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
+  extern __shared__ double lds[];
+  const int K = a.K;
+  const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwv = bd >> 6;
+  double* wsum = lds + 12 * K;                            // [nwv][max_seg]
+  double* wflux = wsum + (size_t)nwv * a.max_seg;         // [nwv]
+  int* scell = (int*)(wflux + nwv);                       // [max_seg]
+  __shared__ int s_ray;
+  {
+    const int mats[4] = {a.rock, a.water, a.porous >= 0 ? a.porous : a.rock, a.air};
+    for (int j = 0; j < 4; ++j) {
+      const double* t = a.tab + (size_t)mats[j] * 19 * K;
+      double* d = lds + 3 * j * K;
+      for (int i = tid; i < K; i += bd) {
+        d[i] = t[i];                 // energies
+        d[K + i] = t[K + i];         // log energies
+        d[2 * K + i] = t[2 * K + i]; // CSDA stopping power
+      }
+    }
+  }
+  WTab T;
+  T.rE = lds; T.rLE = lds + K; T.rS = lds + 2 * K;
+  T.wE = lds + 3 * K; T.wLE = lds + 4 * K; T.wS = lds + 5 * K;
+  T.pE = lds + 6 * K; T.pLE = lds + 7 * K; T.pS = lds + 8 * K;
+  T.aE = lds + 9 * K; T.aLE = lds + 10 * K; T.aS = lds + 11 * K;
+  T.K = K; T.pow2 = a.pow2; T.shared = a.shared; T.mass = a.mass; T.ctau = a.ctau;
+  const WMat M = a.mat;
+  double* sc = JAC ? a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd : nullptr;
+
+  for (;;) {
+    if (tid == 0) s_ray = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int r = s_ray;
+    if (r >= a.n_rays) break;
+    const int b0 = a.row_ptr[r];
+    const int ns = a.row_ptr[r + 1] - b0;
+    if (ns < 0 || ns > a.max_seg) {
+      if (tid == 0) { atomicOr(a.err, ERR_CAP); a.flux[r] = __builtin_nan(""); }
+      __syncthreads();
+      continue;
+    }
+    const double ct = a.cos_t[r];
+    const bool valid = (a.valid ? a.valid[r] != 0 : true) && ct > 0.0;
+    if (!valid) {
+      if (tid == 0) a.flux[r] = __builtin_nan("");
+      if (JAC) for (int s = tid; s < ns; s += bd) a.jac[b0 + s] = 0.0;
+      __syncthreads();
+      continue;
+    }
+    for (int s = tid; s < ns; s += bd) {
+      int c = a.cell[b0 + s];
+      if (c < 0 || c >= a.n_cells) { atomicOr(a.err, ERR_CELL); c = -1; }
+      scell[s] = c;
+    }
+    if (JAC) for (int s = lane; s < ns; s += 64) wsum[wave * a.max_seg + s] = 0.0;
+    if (lane == 0) wflux[wave] = 0.0;
+    __syncthreads();
+
+    const double rem = a.remaining[r];
+    const double zex = a.exit_z[r];
+    const double air_d = fmax(0.0, a.palt - zex) / fmax(ct, 1e-3);
+    const double z0 = a.det_elev + zex;
+
+    for (int n0 = 0; n0 < a.n_nodes; n0 += bd) {
+      const int n = n0 + tid;
+      const bool active = n < a.n_nodes;
+      double E = active ? a.ne[n] : a.ne[0];
+      double W = 1.0;
+      bool ok = true;
+      // ------------------------------------------------ forward sweep
+      for (int s = 0; s < ns && ok; ++s) {
+        const int c = scell[s];
+        if (c < 0) { ok = false; break; }
+        const double wv = a.w[c];
+        const bool porous = M.porous_on && a.shallow[c] != 0;
+        const double d = a.len[b0 + s];
+        if (JAC) {
+          D2 e(E, 0.0, 1.0), g(W);
+          ok = cell_segment(T, M, D2(wv, 1.0, 0.0), porous, d, e, g);
+          double* p = sc + (size_t)s * 4 * bd + tid;
+          p[0] = e.a;
+          p[bd] = e.b;
+          p[2 * bd] = g.a / g.v;
+          p[3 * bd] = g.b / g.v;
+          E = e.v;
+          W = g.v;
+        } else {
+          ok = cell_segment(T, M, wv, porous, d, E, W);
+        }
+      }
+      double rc = 1.0, rl = 0.0, ac = 1.0, al = 0.0;
+      if (ok && rem > 1e-8) {
+        if (JAC) {
+          DualD e(E, 1.0), g(W);
+          ok = rock_segment(T, M.rho_r, rem, e, g);
+          rc = e.d; rl = g.d / g.v; E = e.v; W = g.v;
+        } else {
+          ok = rock_segment(T, M.rho_r, rem, E, W);
+        }
+      }
+      if (ok) {
+        if (JAC) {
+          DualD e(E, 1.0), g(W);
+          ok = air_segment(T, z0, ct, air_d, e, g);
+          ac = e.d; al = g.d / g.v; E = e.v; W = g.v;
+        } else {
+          ok = air_segment(T, z0, ct, air_d, E, W);
+        }
+      }
+      double f = 0.0, Tad = 0.0;
+      if (active && ok) {
+        if (JAC) {
+          Dd phi = flux_gccly_d(ct, E, a.nq[n]);
+          f = 2.0 * a.nw[n] * (W * phi.v);
+          Tad = phi.d / phi.v;
+          Tad = al + Tad * ac;
+          if (rem > 1e-8) Tad = rl + Tad * rc;
+        } else {
+          f = 2.0 * a.nw[n] * (W * flux_gccly(ct, E, a.nq[n]));
+        }
+      }
+      const double fs = wave_sum(f);
+      if (lane == 0) wflux[wave] += fs;
+      if (JAC) {
+        // ---------------------------------------------- reverse sweep
+        for (int s = ns - 1; s >= 0; --s) {
+          const double* p = sc + (size_t)s * 4 * bd + tid;
+          double v = 0.0;
+          if (f != 0.0) {
+            v = f * (p[2 * bd] + Tad * p[0]);
+            Tad = p[3 * bd] + Tad * p[bd];
+          }
+          const double vs = wave_sum(v);
+          if (lane == 0) wsum[wave * a.max_seg + s] += vs;
+        }
+      }
+    }
+    __syncthreads();
+    if (JAC) {
+      // total per segment, then repeated cells merged into the first occurrence
+      for (int s = tid; s < ns; s += bd) {
+        double v = 0.0;
+        for (int j = 0; j < nwv; ++j) v += wsum[j * a.max_seg + s];
+        wsum[s] = v / a.n_nodes;  // row 0 holds the totals; each s is read then written by one thread
+      }
+      __syncthreads();
+      for (int s = tid; s < ns; s += bd) {
+        const int c = scell[s];
+        bool first = true;
+        for (int t = 0; t < s; ++t)
+          if (scell[t] == c) { first = false; break; }
+        double v = 0.0;
+        if (first) {
+          v = wsum[s];
+          for (int t = s + 1; t < ns; ++t)
+            if (scell[t] == c) v += wsum[t];
+        }
+        a.jac[b0 + s] = v;
+      }
+    }
+    if (tid == 0) {
+      double fsum = 0.0;
+      for (int j = 0; j < nwv; ++j) fsum += wflux[j];
+      a.flux[r] = fsum / a.n_nodes;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void reset_counters_w(int* q, int n) {
+  int i = threadIdx.x;
+  if (i < n) q[i] = 0;
+}
+
+size_t w_lds_bytes(int K, int nwv, int max_seg) {
+  return ((size_t)12 * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
+         (size_t)max_seg * sizeof(int);
+}
+
+int check_w(const char* fn, dp_tables t, const dp_material_config* m, const dp_site_config* site,
+            const dp_directional_paths* p, int32_t n_cells, const double* w,
+            const uint8_t* shallow, const dp_energy_samples* es, double* flux) {
+  std::string f(fn);
+  if (!t || !m || !site || !p || !es || !flux || !w || !shallow)
+    return dph::fail(DP_E_ARGUMENT, f + ": null argument");
+  if (p->n_rays < 0 || p->n_segments < 0 || p->max_segments < 0)
+    return dph::fail(DP_E_ARGUMENT, f + ": negative sizes");
+  if (es->n <= 0) return dph::fail(DP_E_ARGUMENT, f + ": need at least one energy sample");
+  if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, f + ": need at least one cell");
+  const int M = t->n_materials;
+  if (m->rock_material < 0 || m->rock_material >= M || m->water_material < 0 ||
+      m->water_material >= M || m->air_material < 0 || m->air_material >= M ||
+      m->porous_material >= M)
+    return dph::fail(DP_E_ARGUMENT, f + ": material index out of range");
+  if (!(m->rock_density > 0) || !(m->water_density > 0) ||
+      (m->porous_material >= 0 && m->porous_top_thickness > 0 && !(m->porous_density > 0)))
+    return dph::fail(DP_E_ARGUMENT, f + ": densities must be positive");
+  return DP_OK;
+}
+
+bool same_grid(dp_tables t, int a, int b) {
+  const size_t K = t->n_knots;
+  const double* x = t->host_data.data() + (size_t)a * 19 * K;
+  const double* y = t->host_data.data() + (size_t)b * 19 * K;
+  return std::memcmp(x, y, 2 * K * sizeof(double)) == 0;
+}
+
+int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_config* site,
+             const dp_directional_paths* p, int32_t n_cells, const double* w,
+             const uint8_t* shallow, const dp_energy_samples* es, double* flux, double* jac,
+             int* qerr) {
+  if (p->n_rays == 0) return DP_OK;
+  const int bd = es->n >= 256 ? 256 : ((es->n + 63) / 64) * 64;
+  const int nwv = bd / 64;
+  const int max_seg = p->max_segments > 0 ? p->max_segments : 1;
+  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg);
+  if (lds > 160 * 1024)
+    return dph::fail(DP_E_CAPACITY, "dp_csda_w_flux_jacobian: max_segments " +
+                                        std::to_string(max_seg) + " exceeds the LDS budget");
+  const bool J = jac != nullptr;
+  int occ = 0;
+  if (J) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcsda_kernel<true>, bd, lds));
+  else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcsda_kernel<false>, bd, lds));
+  if (occ < 1) occ = 1;
+  int grid = c->n_cu * occ;
+  if (grid > p->n_rays) grid = p->n_rays;
+  WArgs a;
+  a.scratch = nullptr;
+  if (J) {
+    int rc = dph::ensure(c->scratch, (size_t)grid * max_seg * 4 * bd * sizeof(double));
+    if (rc) return rc;
+    a.scratch = (double*)c->scratch.ptr;
+  }
+  const int porous = m->porous_material;
+  a.tab = t->data; a.K = t->n_knots; a.pow2 = t->pow2;
+  a.rock = m->rock_material; a.water = m->water_material; a.air = m->air_material;
+  a.porous = porous;
+  a.shared = same_grid(t, a.rock, a.water) && (porous < 0 || same_grid(t, a.rock, porous));
+  a.mat.rho_r = m->rock_density; a.mat.rho_w = m->water_density; a.mat.rho_p = m->porous_density;
+  a.mat.porous_on = porous >= 0 && m->porous_top_thickness > 0.0;
+  a.det_elev = site->detector_elevation; a.palt = site->primary_altitude;
+  a.mass = t->mass; a.ctau = t->ctau;
+  a.n_rays = p->n_rays; a.max_seg = max_seg; a.n_cells = n_cells;
+  a.row_ptr = p->row_ptr; a.cell = p->cell; a.len = p->length; a.cos_t = p->cos_theta;
+  a.remaining = p->remaining_rock; a.exit_z = p->exit_z; a.valid = p->valid;
+  a.w = w; a.shallow = shallow;
+  a.n_nodes = es->n; a.ne = es->energy; a.nw = es->weight; a.nq = es->charge;
+  a.flux = flux; a.jac = jac;
+  a.queue = qerr; a.err = qerr + 1;
+  hipLaunchKernelGGL(reset_counters_w, dim3(1), dim3(64), 0, c->stream, qerr, 2);
+  (void)hipEventRecord(c->ev0, c->stream);
+  if (J) hipLaunchKernelGGL(wcsda_kernel<true>, dim3(grid), dim3(bd), lds, c->stream, a);
+  else hipLaunchKernelGGL(wcsda_kernel<false>, dim3(grid), dim3(bd), lds, c->stream, a);
+  (void)hipEventRecord(c->ev1, c->stream);
+  DP_HIP(hipGetLastError());
+  return DP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dp_csda_w_flux_jacobian_dev(dp_ctx c, dp_tables t, const dp_material_config* m,
+                                const dp_site_config* site, const dp_directional_paths* p,
+                                int32_t n_cells, const double* w, const uint8_t* shallow,
+                                const dp_energy_samples* es, double* flux, double* jac) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_csda_w_flux_jacobian_dev: null context");
+  int rc = check_w("dp_csda_w_flux_jacobian_dev", t, m, site, p, n_cells, w, shallow, es, flux);
+  if (rc) return rc;
+  DP_HIP(hipSetDevice(c->device));
+  rc = dph::ensure(c->counter, 256);
+  if (rc) return rc;
+  return launch_w(c, t, m, site, p, n_cells, w, shallow, es, flux, jac, (int*)c->counter.ptr);
+}
+
+int dp_csda_w_flux_jacobian(dp_ctx c, dp_tables t, const dp_material_config* m,
+                            const dp_site_config* site, const dp_directional_paths* p,
+                            int32_t n_cells, const double* w, const uint8_t* shallow,
+                            const dp_energy_samples* es, double* flux, double* jac) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_csda_w_flux_jacobian: null context");
+  int rc = check_w("dp_csda_w_flux_jacobian", t, m, site, p, n_cells, w, shallow, es, flux);
+  if (rc) return rc;
+  const int nr = p->n_rays, ns = p->n_segments, nn = es->n;
+  if (nr == 0) return DP_OK;
+  if (p->row_ptr[0] != 0 || p->row_ptr[nr] != ns)
+    return dph::fail(DP_E_ARGUMENT,
+                     "dp_csda_w_flux_jacobian: row_ptr must start at 0 and end at n_segments");
+  int mx = 0;
+  for (int r = 0; r < nr; ++r) {
+    const int k = p->row_ptr[r + 1] - p->row_ptr[r];
+    if (k < 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_w_flux_jacobian: row_ptr must be non-decreasing");
+    mx = k > mx ? k : mx;
+  }
+  for (int s = 0; s < ns; ++s)
+    if (p->cell[s] < 0 || p->cell[s] >= n_cells)
+      return dph::fail(DP_E_ARGUMENT, "dp_csda_w_flux_jacobian: cell index out of range at segment " +
+                                          std::to_string(s));
+  for (int i = 0; i < nn; ++i)
+    if (!(es->energy[i] > 0))
+      return dph::fail(DP_E_ARGUMENT, "dp_csda_w_flux_jacobian: sample energies must be positive");
+  DP_HIP(hipSetDevice(c->device));
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
+  const size_t o_rp = take(4 * (size_t)(nr + 1)), o_cell = take(4 * (size_t)ns),
+               o_len = take(8 * (size_t)ns), o_cos = take(8 * (size_t)nr),
+               o_rem = take(8 * (size_t)nr), o_zex = take(8 * (size_t)nr),
+               o_val = take((size_t)nr), o_w = take(8 * (size_t)n_cells),
+               o_sh = take((size_t)n_cells), o_ne = take(8 * (size_t)nn),
+               o_nw = take(8 * (size_t)nn), o_nq = take(8 * (size_t)nn),
+               o_flux = take(8 * (size_t)nr), o_jac = take(8 * (size_t)ns), o_q = take(256);
+  rc = dph::ensure(c->staging, o);
+  if (rc) return rc;
+  char* b = (char*)c->staging.ptr;
+  hipStream_t st = c->stream;
+  DP_HIP(hipMemcpyAsync(b + o_rp, p->row_ptr, 4 * (size_t)(nr + 1), hipMemcpyHostToDevice, st));
+  if (ns) {
+    DP_HIP(hipMemcpyAsync(b + o_cell, p->cell, 4 * (size_t)ns, hipMemcpyHostToDevice, st));
+    DP_HIP(hipMemcpyAsync(b + o_len, p->length, 8 * (size_t)ns, hipMemcpyHostToDevice, st));
+  }
+  DP_HIP(hipMemcpyAsync(b + o_cos, p->cos_theta, 8 * (size_t)nr, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_rem, p->remaining_rock, 8 * (size_t)nr, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_zex, p->exit_z, 8 * (size_t)nr, hipMemcpyHostToDevice, st));
+  if (p->valid) DP_HIP(hipMemcpyAsync(b + o_val, p->valid, (size_t)nr, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_w, w, 8 * (size_t)n_cells, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_sh, shallow, (size_t)n_cells, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_ne, es->energy, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_nw, es->weight, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_nq, es->charge, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
+  dp_directional_paths dp = *p;
+  dp.max_segments = mx;
+  dp.row_ptr = (int32_t*)(b + o_rp); dp.cell = (int32_t*)(b + o_cell);
+  dp.length = (double*)(b + o_len); dp.cos_theta = (double*)(b + o_cos);
+  dp.remaining_rock = (double*)(b + o_rem); dp.exit_z = (double*)(b + o_zex);
+  dp.valid = p->valid ? (uint8_t*)(b + o_val) : nullptr;
+  dp_energy_samples de = *es;
+  de.energy = (double*)(b + o_ne); de.weight = (double*)(b + o_nw); de.charge = (double*)(b + o_nq);
+  int* qerr = (int*)(b + o_q);
+  rc = launch_w(c, t, m, site, &dp, n_cells, (double*)(b + o_w), (uint8_t*)(b + o_sh), &de,
+                (double*)(b + o_flux), jac ? (double*)(b + o_jac) : nullptr, qerr);
+  if (rc) return rc;
+  int herr[2] = {0, 0};
+  DP_HIP(hipMemcpyAsync(flux, b + o_flux, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));
+  if (jac && ns) DP_HIP(hipMemcpyAsync(jac, b + o_jac, 8 * (size_t)ns, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipMemcpyAsync(herr, qerr, 8, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipStreamSynchronize(st));
+  (void)hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1);
+  if (herr[1] & ERR_CAP) return dph::fail(DP_E_CAPACITY, "dp_csda_w_flux_jacobian: a path exceeded max_segments");
+  if (herr[1] & ERR_CELL) return dph::fail(DP_E_ARGUMENT, "dp_csda_w_flux_jacobian: cell index out of range");
+  return DP_OK;
+}
+
+}  // extern "C"

@@ -11,6 +11,11 @@ from .physics import (PhysicsTables, create_physics, create_physics_csda, load_d
 from .cube import (CubeGeometry, RayPaths, compute_flux_gradient_cube, csda_cube_flux,
                    csda_cube_vjp, ray_direction, trace_cube_paths, trace_edge_paths)
 from .geometry import PRIMARY_ALTITUDE, compute_flux, run_backward_mc
+from .tomography import (DirectionalPaths, EnergySamples, MaterialConfig, SiteConfig,
+                         assemble_forward_and_jacobian, cell_density,
+                         compute_directional_flux_csda, csda_energy_quadrature,
+                         direction_vector, directional_flux_and_grad_csda,
+                         flux_and_segment_jacobian, sample_energy_set, trace_voxel_paths)
 
 __all__ = [
     "ArgumentError", "CapacityError", "Context", "DomainError", "HipError", "TransportError",
@@ -18,5 +23,8 @@ __all__ = [
     "load_mdf", "MUON_MASS", "MUON_C_TAU", "CubeGeometry", "RayPaths",
     "compute_flux_gradient_cube", "csda_cube_flux", "csda_cube_vjp", "ray_direction",
     "trace_cube_paths", "trace_edge_paths", "PRIMARY_ALTITUDE", "compute_flux",
-    "run_backward_mc",
+    "run_backward_mc", "DirectionalPaths", "EnergySamples", "MaterialConfig", "SiteConfig",
+    "assemble_forward_and_jacobian", "cell_density", "compute_directional_flux_csda",
+    "csda_energy_quadrature", "direction_vector", "directional_flux_and_grad_csda",
+    "flux_and_segment_jacobian", "sample_energy_set", "trace_voxel_paths",
 ]

@@ -119,6 +119,27 @@ class PhysicsSettings(C.Structure):
     _fields_ = [("cutoff", C.c_double), ("elastic_ratio", C.c_double), ("mass", C.c_double)]
 
 
+class MaterialConfig(C.Structure):
+    _fields_ = [("rock_material", C.c_int32), ("water_material", C.c_int32),
+                ("air_material", C.c_int32), ("porous_material", C.c_int32),
+                ("rock_density", C.c_double), ("water_density", C.c_double),
+                ("porous_density", C.c_double), ("porous_top_thickness", C.c_double)]
+
+
+class SiteConfig(C.Structure):
+    _fields_ = [("detector_elevation", C.c_double), ("primary_altitude", C.c_double)]
+
+
+class DirectionalPaths(C.Structure):
+    _fields_ = [("n_rays", C.c_int32), ("n_segments", C.c_int32), ("max_segments", C.c_int32),
+                ("row_ptr", i32_p), ("cell", i32_p), ("length", dbl_p), ("cos_theta", dbl_p),
+                ("remaining_rock", dbl_p), ("exit_z", dbl_p), ("valid", C.POINTER(C.c_uint8))]
+
+
+class EnergySamples(C.Structure):
+    _fields_ = [("n", C.c_int32), ("energy", dbl_p), ("weight", dbl_p), ("charge", dbl_p)]
+
+
 # Exported symbols and their argument types.
 _SIGS = {
     "dp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -151,6 +172,14 @@ _SIGS = {
     "dp_mc_flat_backward_events": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),
                                              C.POINTER(McConfig), C.POINTER(McResult), dbl_p]),
     "dp_last_kernel_ms": (C.c_double, [C.c_void_p]),
+    "dp_csda_w_flux_jacobian": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(MaterialConfig),
+                                          C.POINTER(SiteConfig), C.POINTER(DirectionalPaths),
+                                          C.c_int32, dbl_p, C.POINTER(C.c_uint8),
+                                          C.POINTER(EnergySamples), dbl_p, dbl_p]),
+    "dp_csda_w_flux_jacobian_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(MaterialConfig),
+                                              C.POINTER(SiteConfig), C.POINTER(DirectionalPaths),
+                                              C.c_int32, C.c_void_p, C.c_void_p,
+                                              C.POINTER(EnergySamples), C.c_void_p, C.c_void_p]),
     "dp_build_material_table": (C.c_int, [C.POINTER(MaterialRecipe), C.POINTER(DedxColumns),
                                           C.POINTER(PhysicsSettings), dbl_p]),
     "dp_build_composite_table": (C.c_int, [C.c_int32, C.POINTER(dbl_p), dbl_p, C.c_int32,

@@ -1,0 +1,239 @@
+"""Water-fraction (w) tomography: CSDA directional flux and its sparse
+Jacobian on the GPU.
+
+Host mirror of the Tomography module's forward model (src/Tomography.jl:
+179-820).  Cells and materials are 0-based.  Every flux and Jacobian value
+comes from the HIP kernel behind dp_csda_w_flux_jacobian; there is no CPU
+path.  The MC-calibrated corrections (CsdaCorrection, CsdaMaterialCorrection)
+stay at the reference default, disabled.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, f64, i32, ptr
+from .cube import trace_edge_paths
+
+
+@dataclass
+class MaterialConfig:
+    """MaterialConfig (src/Tomography.jl:179-188); porous_material < 0 disables
+    the shallow porous recipe."""
+    rock_material: int
+    water_material: int
+    air_material: int
+    porous_material: int
+    rock_density: float
+    water_density: float
+    porous_density: float = 0.0
+    porous_top_thickness: float = 0.0
+
+    def c(self):
+        return _lib.MaterialConfig(int(self.rock_material), int(self.water_material),
+                                   int(self.air_material), int(self.porous_material),
+                                   float(self.rock_density), float(self.water_density),
+                                   float(self.porous_density), float(self.porous_top_thickness))
+
+
+@dataclass
+class SiteConfig:
+    """SiteConfig (src/Tomography.jl:198-201)."""
+    detector_elevation: float
+    primary_altitude: float
+
+
+@dataclass
+class EnergySamples:
+    """Vector{EnergySample} (src/Tomography.jl:203-207) as three arrays."""
+    energy: np.ndarray
+    weight: np.ndarray
+    charge: np.ndarray
+
+    def __len__(self):
+        return len(self.energy)
+
+
+def csda_energy_quadrature(n, energy_min, energy_max):
+    """csda_energy_quadrature (src/Tomography.jl:263-274): log-midpoint nodes,
+    charge 0 (charge-summed flux), weight 0.5 E span."""
+    lo, hi = float(energy_min), float(energy_max)
+    if int(n) <= 0:
+        raise ArgumentError("CSDA quadrature requires at least one energy node")
+    if not (math.isfinite(lo) and math.isfinite(hi) and 0 < lo < hi):
+        raise ArgumentError("CSDA quadrature requires finite 0 < energy_min < energy_max")
+    log_lo = math.log(lo)
+    span = math.log(hi) - log_lo
+    e = np.array([math.exp(log_lo + span * (i - 0.5) / n) for i in range(1, int(n) + 1)])
+    return EnergySamples(e, 0.5 * e * span, np.zeros(int(n)))
+
+
+def sample_energy_set(n, energy_min, energy_max, seed):
+    """sample_energy_set (src/Tomography.jl:236-248): log-uniform energies with
+    weight E log(emax/emin) and charges +/-1.  The stream is numpy's PCG64
+    seeded with `seed`, not Julia's MersenneTwister."""
+    rng = np.random.default_rng(int(seed))
+    rk = math.log(float(energy_max) / float(energy_min))
+    e = np.empty(int(n))
+    w = np.empty(int(n))
+    q = np.empty(int(n))
+    for i in range(int(n)):
+        if rk > 0:
+            e[i] = energy_min * math.exp(rk * rng.random())
+            w[i] = e[i] * rk
+        else:
+            e[i] = energy_min
+            w[i] = 1.0
+        q[i] = 1.0 if rng.random() > 0.5 else -1.0
+    return EnergySamples(e, w, q)
+
+
+def direction_vector(zenith_deg, azimuth_deg):
+    """direction_vector (src/Tomography.jl:229-234)."""
+    t, p = math.radians(zenith_deg), math.radians(azimuth_deg)
+    dh = math.sin(t)
+    return (dh * math.sin(p), dh * math.cos(p), math.cos(t))
+
+
+def cell_density(w, shallow, matcfg):
+    """cell_density (src/Tomography.jl:295-302): 1/rho = sum f_i/rho_i."""
+    if shallow and matcfg.porous_material >= 0 and matcfg.porous_top_thickness > 0.0:
+        return 1.0 / (0.5 * (1.0 - w) / matcfg.porous_density +
+                      0.5 * (1.0 - w) / matcfg.rock_density + w / matcfg.water_density)
+    return 1.0 / ((1.0 - w) / matcfg.rock_density + w / matcfg.water_density)
+
+
+class DirectionalPaths:
+    """Many DirectionalPath (src/Tomography.jl:214-223) in CSR form."""
+
+    def __init__(self, row_ptr, cells, lengths, zenith, azimuth, remaining_rock, exit_z,
+                 valid=None, distance_in_volume=None, surface_distance=None):
+        self.row_ptr = i32(row_ptr)
+        self.cells = i32(cells)
+        self.lengths = f64(lengths)
+        self.zenith = f64(zenith)
+        self.azimuth = f64(azimuth)
+        self.remaining_rock = f64(remaining_rock)
+        self.exit_z = f64(exit_z)
+        n = len(self.row_ptr) - 1
+        self.valid = np.ones(n, np.uint8) if valid is None else np.ascontiguousarray(valid, np.uint8)
+        self.distance_in_volume = distance_in_volume
+        self.surface_distance = surface_distance
+        self.cos_theta = np.array([math.cos(math.radians(z)) for z in self.zenith])
+
+    @classmethod
+    def from_list(cls, paths):
+        """paths: iterable of (zenith, azimuth, [(cell, distance), ...],
+        remaining_rock, exit_z[, valid])."""
+        rp, cells, lens, zen, az, rem, ez, va = [0], [], [], [], [], [], [], []
+        for p in paths:
+            z, a, segs, r, e = p[:5]
+            for c, d in segs:
+                cells.append(int(c))
+                lens.append(float(d))
+            rp.append(len(cells))
+            zen.append(z); az.append(a); rem.append(r); ez.append(e)
+            va.append(p[5] if len(p) > 5 else 1)
+        return cls(rp, cells, lens, zen, az, rem, ez, va)
+
+    @property
+    def n_rays(self):
+        return len(self.row_ptr) - 1
+
+    @property
+    def n_segments(self):
+        return int(self.row_ptr[-1])
+
+    def max_segments(self):
+        return int(np.diff(self.row_ptr).max()) if self.n_rays else 0
+
+    def c_paths(self):
+        return _lib.DirectionalPaths(self.n_rays, self.n_segments, self.max_segments(),
+                                     ptr(self.row_ptr, C.c_int32), ptr(self.cells, C.c_int32),
+                                     ptr(self.lengths), ptr(self.cos_theta),
+                                     ptr(self.remaining_rock), ptr(self.exit_z),
+                                     ptr(self.valid, C.c_uint8))
+
+
+def trace_voxel_paths(x_edges, y_edges, z_edges, zeniths, azimuths, surface_distance, exit_z,
+                      ctx=None):
+    """trace_path(::VoxelVolume) (examples/lvd_tomography.jl:738-779) from a
+    detector at the origin for many (zenith, azimuth) bins; surface_distance
+    and exit_z come from the topography (ray_surface_exit_distance)."""
+    zen, az = f64(zeniths), f64(azimuths)
+    dirs = np.array([direction_vector(z, a) for z, a in zip(zen, az)]).reshape(-1, 3)
+    origins = np.zeros_like(dirs)
+    rp, cells, lens, traveled, remaining = trace_edge_paths(x_edges, y_edges, z_edges, origins,
+                                                            dirs, surface_distance, ctx=ctx)
+    return DirectionalPaths(rp, cells, lens, zen, az, remaining, exit_z,
+                            distance_in_volume=traveled, surface_distance=f64(surface_distance))
+
+
+def _run(physics, shallow_flags, matcfg, site, paths, water_fractions, samples, jacobian, ctx):
+    ctx = ctx or _lib.default_context()
+    h = physics.device_handle(ctx)
+    w = f64(water_fractions)
+    sh = np.ascontiguousarray(shallow_flags, dtype=np.uint8)
+    if len(sh) != len(w):
+        raise ArgumentError("shallow_flags and water_fractions must have one entry per cell")
+    ne, nw, nq = f64(samples.energy), f64(samples.weight), f64(samples.charge)
+    es = _lib.EnergySamples(len(ne), ptr(ne), ptr(nw), ptr(nq))
+    mc = matcfg.c()
+    sc = _lib.SiteConfig(float(site.detector_elevation), float(site.primary_altitude))
+    cp = paths.c_paths()
+    flux = np.zeros(paths.n_rays)
+    jac = np.zeros(max(paths.n_segments, 1)) if jacobian else None
+    _lib.check(ctx.lib.dp_csda_w_flux_jacobian(ctx.h, h, C.byref(mc), C.byref(sc), C.byref(cp),
+                                               len(w), ptr(w), ptr(sh, C.c_uint8), C.byref(es),
+                                               ptr(flux), ptr(jac)))
+    return flux, (jac[:paths.n_segments] if jacobian else None)
+
+
+def compute_directional_flux_csda(physics, shallow_flags, matcfg, site, paths, water_fractions,
+                                  samples, ctx=None):
+    """compute_directional_flux_csda (src/Tomography.jl:667-726) for every path
+    (NaN for an invalid path or cos(zenith) <= 0)."""
+    return _run(physics, shallow_flags, matcfg, site, paths, water_fractions, samples, False,
+                ctx)[0]
+
+
+def flux_and_segment_jacobian(physics, shallow_flags, matcfg, site, paths, water_fractions,
+                              samples, ctx=None):
+    """(flux[n_rays], jac[n_segments]): jac holds d flux / d w of each path's
+    cell at its first occurrence on the path, 0 at later occurrences."""
+    return _run(physics, shallow_flags, matcfg, site, paths, water_fractions, samples, True, ctx)
+
+
+def directional_flux_and_grad_csda(physics, shallow_flags, matcfg, site, paths, water_fractions,
+                                   samples, ray=0, ctx=None):
+    """directional_flux_and_grad_csda (src/Tomography.jl:736-762) for one path:
+    (flux, cellids, values) with the on-path cells in first-appearance order."""
+    flux, jac = flux_and_segment_jacobian(physics, shallow_flags, matcfg, site, paths,
+                                          water_fractions, samples, ctx)
+    if not paths.valid[ray] or not paths.cos_theta[ray] > 0.0:
+        return math.nan, np.zeros(0, np.int64), np.zeros(0)
+    a, b = paths.row_ptr[ray], paths.row_ptr[ray + 1]
+    cells, vals, seen = [], [], set()
+    for k in range(a, b):
+        c = int(paths.cells[k])
+        if c not in seen:
+            seen.add(c)
+            cells.append(c)
+            vals.append(jac[k])
+    return float(flux[ray]), np.array(cells, np.int64), np.array(vals)
+
+
+def assemble_forward_and_jacobian(physics, shallow_flags, matcfg, site, paths, water_fractions,
+                                  samples, n_cells, ctx=None):
+    """assemble_forward_and_jacobian (src/Tomography.jl:773-820): flux per bin
+    and J[bin, cell] = d flux / d w as a scipy.sparse CSR matrix, keeping only
+    finite non-zero entries."""
+    import scipy.sparse as sp
+    flux, jac = flux_and_segment_jacobian(physics, shallow_flags, matcfg, site, paths,
+                                          water_fractions, samples, ctx)
+    rows = np.repeat(np.arange(paths.n_rays), np.diff(paths.row_ptr))
+    keep = np.isfinite(jac) & (jac != 0.0)
+    J = sp.csr_matrix((jac[keep], (rows[keep], paths.cells[keep])), shape=(paths.n_rays, n_cells))
+    return flux, J

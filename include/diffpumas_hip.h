@@ -269,6 +269,63 @@ int dp_mc_flat_backward_events(dp_ctx ctx, dp_tables tables, const dp_flat_geome
                                const dp_mc_config* cfg, dp_mc_result* out,
                                double* event_weights);
 
+/* ------------------------------- water-fraction tomography CSDA (w-model)
+ * MaterialConfig (src/Tomography.jl:179-188), materials 0-based;
+ * porous_material < 0 disables the shallow porous recipe.                  */
+typedef struct {
+  int32_t rock_material, water_material, air_material, porous_material;
+  double rock_density, water_density, porous_density, porous_top_thickness;
+} dp_material_config;
+
+/* SiteConfig (src/Tomography.jl:198-201). */
+typedef struct {
+  double detector_elevation;
+  double primary_altitude;
+} dp_site_config;
+
+/* DirectionalPath (src/Tomography.jl:214-223) for many bins in CSR form. */
+typedef struct {
+  int32_t n_rays;
+  int32_t n_segments;
+  int32_t max_segments;        /* max over rays of row_ptr[r+1]-row_ptr[r]  */
+  const int32_t* row_ptr;      /* n_rays + 1 */
+  const int32_t* cell;         /* 0-based cell of each CellSegment           */
+  const double* length;        /* CellSegment distance, m                    */
+  const double* cos_theta;     /* cosd(zenith) per ray                       */
+  const double* remaining_rock;/* remaining_rock_distance per ray, m         */
+  const double* exit_z;        /* surface_exit_z per ray, m                  */
+  const uint8_t* valid;        /* per ray; NULL = all valid                  */
+} dp_directional_paths;
+
+/* Vector{EnergySample} (src/Tomography.jl:203-207). */
+typedef struct {
+  int32_t n;
+  const double* energy;
+  const double* weight;
+  const double* charge;
+} dp_energy_samples;
+
+/* Replaces: compute_directional_flux_csda (src/Tomography.jl:667-726) for
+ * every bin and, when jac != NULL, directional_flux_and_grad_csda /
+ * assemble_forward_and_jacobian (:736-820) through the segment pullbacks of
+ * src/Tomography_rules.jl:63-106 (exact derivatives of the executed Heun
+ * sub-stepping).  water_fraction[n_cells] and shallow[n_cells] (0/1) are per
+ * cell.  flux[r] is NaN for an invalid path or cos_theta <= 0.  jac has one
+ * slot per path segment: d flux[r] / d w[cell] sits at the cell's first
+ * occurrence on the path (the column order of _path_layout, :640-659), later
+ * occurrences of the same cell hold 0.  The MC-calibrated corrections
+ * (CsdaCorrection, CsdaMaterialCorrection) are the reference defaults: off. */
+int dp_csda_w_flux_jacobian(dp_ctx ctx, dp_tables tables, const dp_material_config* mat,
+                            const dp_site_config* site, const dp_directional_paths* paths,
+                            int32_t n_cells, const double* water_fraction,
+                            const uint8_t* shallow, const dp_energy_samples* samples,
+                            double* flux, double* jac);
+int dp_csda_w_flux_jacobian_dev(dp_ctx ctx, dp_tables tables, const dp_material_config* mat,
+                                const dp_site_config* site, const dp_directional_paths* paths,
+                                int32_t n_cells, const double* water_fraction,
+                                const uint8_t* shallow, const dp_energy_samples* samples,
+                                double* flux, double* jac);
+
 /* Last kernel duration in ms, timed with HIP events on the context's stream
  * around the dominant kernel of the previous call (0 if none). */
 double dp_last_kernel_ms(dp_ctx ctx);

@@ -384,3 +384,163 @@ int orc_mc_flat(int M, int K, const double* data, const double* scal, double mas
 }
 
 }  // extern "C"
+
+// ======================================== w-parameterised tomography CSDA
+namespace {
+MatCfg make_matcfg(const int32_t* mi, const double* md) {
+  return MatCfg{mi[0], mi[1], mi[2], mi[3], md[0], md[1], md[2], md[3]};
+}
+// Tape entry of one cell segment: energy partials and log-weight partials.
+struct WSeg { double ew, ee, lw, le; };
+}  // namespace
+extern "C" {
+
+// One cell segment with the (w, E_in) dual walk of _cell_segment_partials
+// (src/Tomography_rules.jl:33-48).  out = E, G, ok, dE/dw, dE/dE_in, dG/dw,
+// dG/dE_in, with G the weight factor (unit incoming weight).
+void orc_w_cell_segment(int M, int K, const double* data, const double* scal, double mass,
+                        double ctau, const int32_t* mi, const double* md, double w, int shallow,
+                        double distance, double energy, double* out) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  MatCfg m = make_matcfg(mi, md);
+  Dual2 e(energy, 0.0, 1.0), g(1.0);
+  bool ok;
+  cell_segment(T, Dual2(w, 1.0, 0.0), shallow != 0, m, distance, e, g, ok);
+  out[0] = e.v; out[1] = g.v; out[2] = ok; out[3] = e.a; out[4] = e.b; out[5] = g.a; out[6] = g.b;
+}
+
+// One atmosphere segment (propagate_csda_air_segment) with its energy tangent.
+// out = E, G, ok, dE/dE_in, dG/dE_in.
+void orc_w_air_segment(int M, int K, const double* data, const double* scal, double mass,
+                       double ctau, int air, const double* site2, double start_z, double cos_t,
+                       double distance, double energy, double* out) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Site site{site2[0], site2[1]};
+  Dual e(energy, 1.0), g(1.0);
+  bool ok;
+  air_segment(T, air, site, start_z, cos_t, distance, e, g, ok);
+  out[0] = e.v; out[1] = g.v; out[2] = ok; out[3] = e.d; out[4] = g.d;
+}
+
+// compute_directional_flux_csda (src/Tomography.jl:667-726) for many paths.
+// Invalid paths and cos_t <= 0 give NaN.
+void orc_w_flux(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                const int32_t* mi, const double* md, const double* site2, int n_rays,
+                const int32_t* row_ptr, const int32_t* cells, const double* lens,
+                const double* remaining, const double* exit_z, const double* cos_t,
+                const uint8_t* valid, const double* w, const uint8_t* shallow, int n_nodes,
+                const double* ne, const double* nw, const double* nq, double* flux, int threads) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  MatCfg m = make_matcfg(mi, md);
+  Site site{site2[0], site2[1]};
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    std::vector<double> wl;
+    std::vector<uint8_t> sl;
+    std::vector<int> idx;
+#pragma omp for schedule(dynamic, 1)
+    for (int r = 0; r < n_rays; ++r) {
+      if (!valid[r] || !(cos_t[r] > 0.0)) { flux[r] = NAN; continue; }
+      const int b = row_ptr[r], ns = row_ptr[r + 1] - b;
+      wl.resize(ns); sl.resize(ns); idx.resize(ns);
+      for (int s = 0; s < ns; ++s) { wl[s] = w[cells[b + s]]; sl[s] = shallow[cells[b + s]]; idx[s] = s; }
+      flux[r] = directional_flux_w<double>(T, m, site, idx.data(), lens + b, ns, remaining[r],
+                                           exit_z[r], cos_t[r], wl.data(), sl.data(), n_nodes, ne,
+                                           nw, nq);
+    }
+  }
+}
+
+// Forward-mode derivative of one path's flux with respect to w[cell] (the
+// full-vector AD reference of test/test_tomography.jl:89).
+double orc_w_dflux(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                   const int32_t* mi, const double* md, const double* site2, int ns,
+                   const int32_t* cells, const double* lens, double remaining, double exit_z,
+                   double cos_t, int n_cells, const double* w, const uint8_t* shallow, int cell,
+                   int n_nodes, const double* ne, const double* nw, const double* nq) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  MatCfg m = make_matcfg(mi, md);
+  Site site{site2[0], site2[1]};
+  std::vector<Dual> wd(n_cells);
+  for (int c = 0; c < n_cells; ++c) wd[c] = Dual(w[c], c == cell ? 1.0 : 0.0);
+  Dual f = directional_flux_w<Dual>(T, m, site, cells, lens, ns, remaining, exit_z, cos_t,
+                                    wd.data(), shallow, n_nodes, ne, nw, nq);
+  return f.d;
+}
+
+// directional_flux_and_grad_csda / assemble_forward_and_jacobian
+// (src/Tomography.jl:736-820) through the segment pullbacks of
+// src/Tomography_rules.jl:63-106.  jac has one slot per path segment: the
+// derivative with respect to that segment's cell sits at the cell's first
+// occurrence on the path, later occurrences hold 0.
+void orc_w_jacobian(int M, int K, const double* data, const double* scal, double mass,
+                    double ctau, const int32_t* mi, const double* md, const double* site2,
+                    int n_rays, const int32_t* row_ptr, const int32_t* cells, const double* lens,
+                    const double* remaining, const double* exit_z, const double* cos_t,
+                    const uint8_t* valid, const double* w, const uint8_t* shallow, int n_nodes,
+                    const double* ne, const double* nw, const double* nq, double* flux,
+                    double* jac, int threads) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  MatCfg m = make_matcfg(mi, md);
+  Site site{site2[0], site2[1]};
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    std::vector<WSeg> tape;
+#pragma omp for schedule(dynamic, 1)
+    for (int r = 0; r < n_rays; ++r) {
+      const int b = row_ptr[r], ns = row_ptr[r + 1] - b;
+      for (int s = 0; s < ns; ++s) jac[b + s] = 0.0;
+      if (!valid[r] || !(cos_t[r] > 0.0)) { flux[r] = NAN; continue; }
+      const double ct = cos_t[r];
+      tape.resize(ns);
+      double facc = 0.0;
+      for (int n = 0; n < n_nodes; ++n) {
+        double E = ne[n], W = 1.0;
+        bool ok = true;
+        for (int s = 0; s < ns && ok; ++s) {
+          const int c = cells[b + s];
+          Dual2 e(E, 0.0, 1.0), g(W);
+          cell_segment(T, Dual2(w[c], 1.0, 0.0), shallow[c] != 0, m, lens[b + s], e, g, ok);
+          tape[s] = WSeg{e.a, e.b, g.a / g.v, g.b / g.v};
+          E = e.v; W = g.v;
+        }
+        double rc = 1.0, rl = 0.0;  // remaining-rock partials
+        if (ok && remaining[r] > 1e-8) {
+          Dual e(E, 1.0), g(W);
+          uniform_segment(T, m.rock, m.rho_rock, remaining[r], e, g, ok);
+          rc = e.d; rl = g.d / g.v; E = e.v; W = g.v;
+        }
+        double ac = 1.0, al = 0.0;  // atmosphere partials
+        if (ok) {
+          const double air_d = std::max(0.0, site.primary_altitude - exit_z[r]) / std::max(ct, 1e-3);
+          Dual e(E, 1.0), g(W);
+          air_segment(T, m.air, site, exit_z[r], ct, air_d, e, g, ok);
+          ac = e.d; al = g.d / g.v; E = e.v; W = g.v;
+        }
+        if (!ok) continue;
+        Dual fl = flux_gccly(ct, Dual(E, 1.0), nq[n]);
+        const double f = 2.0 * nw[n] * (W * fl.v);
+        facc += f;
+        if (f == 0.0) continue;
+        double Tad = fl.d / fl.v;        // d log(flux)/dE at the surface
+        Tad = al + Tad * ac;             // ... at the rock exit
+        if (remaining[r] > 1e-8) Tad = rl + Tad * rc;
+        for (int s = ns - 1; s >= 0; --s) {
+          const WSeg& t = tape[s];
+          jac[b + s] += f * (t.lw + Tad * t.ew);
+          Tad = t.le + Tad * t.ee;
+        }
+      }
+      flux[r] = facc / n_nodes;
+      for (int s = 0; s < ns; ++s) jac[b + s] /= n_nodes;
+      // merge repeated cells into their first occurrence (_path_layout)
+      for (int s = 1; s < ns; ++s)
+        for (int t = 0; t < s; ++t)
+          if (cells[b + t] == cells[b + s]) { jac[b + t] += jac[b + s]; jac[b + s] = 0.0; break; }
+    }
+  }
+}
+
+}  // extern "C"

@@ -260,3 +260,94 @@ def mc_flat(ph, rock_mats, rock_fracs, rock_rho, thick, air, elevation, emin, em
     if rc != 0:
         raise RuntimeError(f"oracle MC error {rc} at event {bad.value}")
     return (out[0], out[1], ev) if per_event else (out[0], out[1])
+
+
+# ------------------------------------------------ w-parameterised tomography
+u8_p = C.POINTER(C.c_uint8)
+_W_COMMON = [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, i_p, d_p, d_p]
+_W_RAYS = [C.c_int, i_p, i_p, d_p, d_p, d_p, d_p, u8_p, d_p, u8_p, C.c_int, d_p, d_p, d_p]
+_SIGS["orc_w_cell_segment"] = (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, i_p,
+                                      d_p, C.c_double, C.c_int, C.c_double, C.c_double, d_p])
+_SIGS["orc_w_air_segment"] = (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, C.c_int,
+                                     d_p, C.c_double, C.c_double, C.c_double, C.c_double, d_p])
+_SIGS["orc_w_flux"] = (None, _W_COMMON + _W_RAYS + [d_p, C.c_int])
+_SIGS["orc_w_jacobian"] = (None, _W_COMMON + _W_RAYS + [d_p, d_p, C.c_int])
+_SIGS["orc_w_dflux"] = (C.c_double, _W_COMMON + [C.c_int, i_p, d_p, C.c_double, C.c_double,
+                                                 C.c_double, C.c_int, d_p, u8_p, C.c_int,
+                                                 C.c_int, d_p, d_p, d_p])
+
+
+def matcfg_arrays(rock, water, air, porous, rho_rock, rho_water, rho_porous=0.0, porous_top=0.0):
+    """MaterialConfig (src/Tomography.jl:179-188) with 0-based materials
+    (porous < 0: no shallow porous recipe)."""
+    return i32([rock, water, air, porous]), f64([rho_rock, rho_water, rho_porous, porous_top])
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def w_cell_segment(ph, mc, w, shallow, distance, energy):
+    out = np.zeros(7)
+    mi, md = mc
+    lib().orc_w_cell_segment(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), w,
+                             int(shallow), distance, energy, P(out))
+    return out
+
+
+def w_air_segment(ph, air, site, start_z, cos_t, distance, energy):
+    out = np.zeros(5)
+    s = f64(site)
+    lib().orc_w_air_segment(*tabargs(ph), ph.mass, ph.ctau, air, P(s), start_z, cos_t, distance,
+                            energy, P(out))
+    return out
+
+
+def _wrays(paths):
+    rp, cells, lens = i32(paths["row_ptr"]), i32(paths["cells"]), f64(paths["lengths"])
+    n = len(rp) - 1
+    if len(cells) == 0:
+        cells, lens = np.zeros(1, np.int32), np.zeros(1)
+    return (n, P(rp, C.c_int32), P(cells, C.c_int32), P(lens), P(f64(paths["remaining"])),
+            P(f64(paths["exit_z"])), P(f64(paths["cos_theta"])),
+            P(_u8(paths["valid"]), C.c_uint8)), (rp, cells, lens)
+
+
+def w_flux(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0):
+    mi, md = mc
+    s = f64(site)
+    args, keep = _wrays(paths)
+    w, sh = f64(w), _u8(shallow)
+    ne, nw, nq = f64(ne), f64(nw), f64(nq)
+    flux = np.zeros(args[0])
+    lib().orc_w_flux(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s), *args, P(w),
+                     P(sh, C.c_uint8), len(ne), P(ne), P(nw), P(nq), P(flux), threads)
+    return flux
+
+
+def w_jacobian(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0):
+    """(flux[n_rays], jac[n_segments]) with repeated cells merged into their
+    first occurrence."""
+    mi, md = mc
+    s = f64(site)
+    args, keep = _wrays(paths)
+    w, sh = f64(w), _u8(shallow)
+    ne, nw, nq = f64(ne), f64(nw), f64(nq)
+    flux = np.zeros(args[0])
+    jac = np.zeros(max(len(paths["cells"]), 1))
+    lib().orc_w_jacobian(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s), *args,
+                         P(w), P(sh, C.c_uint8), len(ne), P(ne), P(nw), P(nq), P(flux), P(jac),
+                         threads)
+    return flux, jac[:len(paths["cells"])]
+
+
+def w_dflux(ph, mc, site, cells, lens, remaining, exit_z, cos_t, w, shallow, cell, ne, nw, nq):
+    """Forward-mode d flux / d w[cell] of one path."""
+    mi, md = mc
+    s = f64(site)
+    cells, lens = i32(cells), f64(lens)
+    w, sh = f64(w), _u8(shallow)
+    ne, nw, nq = f64(ne), f64(nw), f64(nq)
+    return lib().orc_w_dflux(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s),
+                             len(cells), P(cells, C.c_int32), P(lens), remaining, exit_z, cos_t,
+                             len(w), P(w), P(sh, C.c_uint8), cell, len(ne), P(ne), P(nw), P(nq))
