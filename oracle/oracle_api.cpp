@@ -544,3 +544,112 @@ void orc_w_jacobian(int M, int K, const double* data, const double* scal, double
 }
 
 }  // extern "C"
+
+// ============================== mixed kernel / EHS with scripted draws (pins)
+namespace {
+MixedKernel kernel_from(int n, const double* e, const double* stop, const double* sigma,
+                        const double* range, const double* hazard, const double* proper, int n_el,
+                        const double* Z, const double* A, const double* I, const double* f,
+                        double onset, double cutoff, double mass, double ctau) {
+  MixedKernel c;
+  c.mass = mass; c.ctau = ctau; c.cutoff = cutoff; c.onset = onset;
+  c.e.assign(e, e + n); c.stop.assign(stop, stop + n); c.sigma.assign(sigma, sigma + n);
+  c.range.assign(range, range + n); c.hazard.assign(hazard, hazard + n);
+  c.proper.assign(proper, proper + n);
+  for (int i = 0; i < n_el; ++i) c.el.push_back({Z[i], A[i], I[i], f[i]});
+  return c;
+}
+Mixture mixture_from(const Tables& T, int n_mix, const int32_t* mats, const double* fracs) {
+  return make_mixture(T, std::vector<int>(mats, mats + n_mix),
+                      std::vector<double>(fracs, fracs + n_mix));
+}
+}  // namespace
+extern "C" {
+
+// transport_mixed_step (src/MixedTransport.jl:144-200) on an explicit kernel.
+// st[0..10] = E, w, t, pos[3], dir[3], grammage, distance (in and out).
+// Returns 0, -2 (DomainError), -3 (other error); *used = draws consumed.
+int orc_mixed_step(int n, const double* e, const double* stop, const double* sigma,
+                   const double* range, const double* hazard, const double* proper, int n_el,
+                   const double* Z, const double* A, const double* I, const double* f,
+                   double onset, double cutoff, double mass, double ctau, double* st,
+                   double density, double distance, double energy_limit, const double* script,
+                   int n_script, uint64_t seed, int* used) {
+  MixedKernel c = kernel_from(n, e, stop, sigma, range, hazard, proper, n_el, Z, A, I, f, onset,
+                              cutoff, mass, ctau);
+  Medium md{nullptr, &c};
+  McSetup S{};
+  McState s;
+  s.E = st[0]; s.w = st[1]; s.t = st[2];
+  for (int a = 0; a < 3; ++a) { s.pos[a] = st[3 + a]; s.dir[a] = st[6 + a]; }
+  s.X = st[9]; s.L = st[10];
+  Philox4x32 rng(seed, 0);
+  if (script) { rng.script = script; rng.script_n = n_script; }
+  int rc = 0;
+  try {
+    mixed_step(S, md, s, density, distance, energy_limit, rng);
+  } catch (const DomainErr&) {
+    rc = -2;
+  } catch (const std::exception&) {
+    rc = -3;
+  }
+  st[0] = s.E; st[1] = s.w; st[2] = s.t;
+  for (int a = 0; a < 3; ++a) st[3 + a] = s.pos[a];
+  st[9] = s.X; st[10] = s.L;
+  if (used) *used = rng.script_pos;
+  return rc;
+}
+
+// reverse_del_event (src/MixedTransport.jl:111-142), details = false.
+int orc_reverse_del(int n, const double* e, const double* stop, const double* sigma,
+                    const double* range, const double* hazard, const double* proper, int n_el,
+                    const double* Z, const double* A, const double* I, const double* f,
+                    double onset, double cutoff, double mass, double ctau, double k,
+                    const double* script, int n_script, double* out) {
+  MixedKernel c = kernel_from(n, e, stop, sigma, range, hazard, proper, n_el, Z, A, I, f, onset,
+                              cutoff, mass, ctau);
+  Philox4x32 rng(0, 0);
+  rng.script = script; rng.script_n = n_script;
+  try {
+    DelOut o = reverse_del_event(c, k, rng, false);
+    out[0] = o.k; out[1] = o.w;
+  } catch (const std::exception&) {
+    return -3;
+  }
+  return rng.script_pos;
+}
+
+// sample_ehs_event, backward (src/Straggling.jl:373-434, :1284-1334).
+// out = accepted, X_ehs, k_ehs.  Returns the number of draws consumed.
+int orc_ehs_sample(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                   int n_mix, const int32_t* mats, const double* fracs, double ki, double kf,
+                   double dX, double Xtot, double ratio, double er, const double* script,
+                   int n_script, double* out) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Mixture m = mixture_from(T, n_mix, mats, fracs);
+  Philox4x32 rng(0, 0);
+  rng.script = script; rng.script_n = n_script;
+  double X = 0, k = 0;
+  bool acc = sample_ehs(T, m, ki, kf, dX, Xtot, ratio, er, rng, X, k);
+  out[0] = acc; out[1] = acc ? X : 0.0; out[2] = acc ? k : 0.0;
+  return rng.script_pos;
+}
+
+// property_range (MIXED or CSDA) and compute_ehs_mean_free_path on a
+// material or runtime mixture.
+double orc_mix_range(int M, int K, const double* data, const double* scal, double mass,
+                     double ctau, int n_mix, const int32_t* mats, const double* fracs, int mixed,
+                     double k) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Mixture m = mixture_from(T, n_mix, mats, fracs);
+  return mix_range(T, mixed != 0, m, k);
+}
+double orc_ehs_path(int M, int K, const double* data, const double* scal, double mass,
+                    double ctau, int n_mix, const int32_t* mats, const double* fracs, double k,
+                    double er) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Mixture m = mixture_from(T, n_mix, mats, fracs);
+  return ehs_path(T, m, k, er);
+}
+
+}  // extern "C"

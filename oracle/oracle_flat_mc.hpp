@@ -17,6 +17,7 @@ struct Medium {
 struct McState {
   double E, w, t;
   double pos[3], dir[3];
+  double X = 0.0, L = 0.0;  // grammage and distance accumulated by mixed steps
 };
 
 struct McSetup {
@@ -163,6 +164,8 @@ inline void mixed_step(const McSetup& S, const Medium& md, McState& st, double d
   if (weight > 0 && k >= emax && remaining > tiny && (energy_limit < 0 || k < energy_limit))
     throw DomainErr("Mixed transport exhausted energy tables before segment completion");
   const double dd = travelled / density;
+  st.X += travelled;
+  st.L += dd;
   st.E = k;
   st.t += proper;
   st.w = weight;

@@ -8,6 +8,7 @@
 // each block yields four 32-bit words, consumed in order; a uniform double on
 // [0, 1) takes two words w0, w1: ((w0 << 32 | w1) >> 11) * 2^-53.
 #pragma once
+#include <stdexcept>
 #include <cstdint>
 #include <cmath>
 
@@ -45,7 +46,17 @@ struct Philox4x32 {
     }
     return buf[pos++];
   }
+  // Scripted draws (tests only): when set, uniform() and exponential() return
+  // the listed values in order, like the scripted RNGs of the reference tests
+  // (test/test_mixed_kernel.jl:29-31,66, test/test_ehs_thinning.jl:4-12).
+  const double* script = nullptr;
+  int script_n = 0, script_pos = 0;
+  double scripted() {
+    if (script_pos >= script_n) throw std::runtime_error("scripted draws exhausted");
+    return script[script_pos++];
+  }
   double uniform() {  // [0, 1)
+    if (script) return scripted();
     uint64_t a = next_u32();
     uint64_t b = next_u32();
     return (double)(((a << 32) | b) >> 11) * 0x1.0p-53;
@@ -53,7 +64,7 @@ struct Philox4x32 {
   double uniform_open() {  // (0, 1]
     return 1.0 - uniform();
   }
-  double exponential() { return -std::log(uniform_open()); }
+  double exponential() { return script ? scripted() : -std::log(uniform_open()); }
 };
 
 }  // namespace orc

@@ -3,6 +3,7 @@
 The oracle is the checker only; the product path never loads it.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -351,3 +352,86 @@ def w_dflux(ph, mc, site, cells, lens, remaining, exit_z, cos_t, w, shallow, cel
     return lib().orc_w_dflux(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s),
                              len(cells), P(cells, C.c_int32), P(lens), remaining, exit_z, cos_t,
                              len(w), P(w), P(sh, C.c_uint8), cell, len(ne), P(ne), P(nw), P(nq))
+
+
+# ----------------------------------------- mixed kernel / EHS scripted pins
+_K_ARGS = [C.c_int, d_p, d_p, d_p, d_p, d_p, d_p, C.c_int, d_p, d_p, d_p, d_p, C.c_double,
+           C.c_double, C.c_double, C.c_double]
+_SIGS["orc_mixed_step"] = (C.c_int, _K_ARGS + [d_p, C.c_double, C.c_double, C.c_double, d_p,
+                                               C.c_int, C.c_uint64, C.POINTER(C.c_int)])
+_SIGS["orc_reverse_del"] = (C.c_int, _K_ARGS + [C.c_double, d_p, C.c_int, d_p])
+_SIGS["orc_ehs_sample"] = (C.c_int, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, C.c_int,
+                                     i_p, d_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, d_p, C.c_int, d_p])
+_SIGS["orc_mix_range"] = (C.c_double, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double,
+                                       C.c_int, i_p, d_p, C.c_int, C.c_double])
+_SIGS["orc_ehs_path"] = (C.c_double, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double,
+                                      C.c_int, i_p, d_p, C.c_double, C.c_double])
+
+MUON_MASS = 0.10565839
+MUON_C_TAU = 658.654
+
+
+class Kernel:
+    """MixedKernel fields (src/MixedTransport.jl:15-29) as arrays."""
+
+    def __init__(self, energies, stopping, sigma, range_, hazard, proper, elements=(),
+                 onset=math.inf, cutoff=0.05, mass=MUON_MASS,
+                 ctau=MUON_C_TAU):
+        self.arrs = [f64(a) for a in (energies, stopping, sigma, range_, hazard, proper)]
+        el = list(elements)
+        self.Z = f64([e[0] for e in el] or [0.0])
+        self.A = f64([e[1] for e in el] or [1.0])
+        self.I = f64([e[2] for e in el] or [0.0])
+        self.f = f64([e[3] for e in el] or [0.0])
+        self.n_el = len(el)
+        self.onset, self.cutoff, self.mass, self.ctau = onset, cutoff, mass, ctau
+
+    def args(self):
+        return ([len(self.arrs[0])] + [P(a) for a in self.arrs] +
+                [self.n_el, P(self.Z), P(self.A), P(self.I), P(self.f), self.onset, self.cutoff,
+                 self.mass, self.ctau])
+
+
+def mixed_step(kern, state, density, distance, energy_limit=-1.0, script=None, seed=0):
+    """state: dict E, w, t, pos, dir, X, L.  Returns (rc, new state, draws used)."""
+    st = f64([state["E"], state.get("w", 1.0), state.get("t", 0.0), *state.get("pos", (0, 0, 0)),
+              *state.get("dir", (0, 0, 1)), state.get("X", 0.0), state.get("L", 0.0)])
+    sc = None if script is None else f64(script)
+    used = C.c_int(0)
+    rc = lib().orc_mixed_step(*kern.args(), P(st), density, distance, energy_limit, P(sc),
+                              0 if sc is None else len(sc), seed, C.byref(used))
+    out = dict(E=st[0], w=st[1], t=st[2], pos=st[3:6].copy(), dir=st[6:9].copy(), X=st[9],
+               L=st[10])
+    return rc, out, used.value
+
+
+def reverse_del(kern, k, script):
+    sc = f64(script)
+    out = np.zeros(2)
+    used = lib().orc_reverse_del(*kern.args(), k, P(sc), len(sc), P(out))
+    return out[0], out[1], used
+
+
+def _mix(mats, fracs):
+    return i32(mats), f64(fracs)
+
+
+def ehs_sample(ph, mats, fracs, ki, kf, dX, Xtot, ratio, script, er=0.05):
+    m, f = _mix(mats, fracs)
+    sc = f64(script)
+    out = np.zeros(3)
+    used = lib().orc_ehs_sample(*tabargs(ph), ph.mass, ph.ctau, len(m), P(m, C.c_int32), P(f), ki,
+                                kf, dX, Xtot, ratio, er, P(sc), len(sc), P(out))
+    return bool(out[0]), out[1], out[2], used
+
+
+def mix_range(ph, mats, fracs, k, mixed=True):
+    m, f = _mix(mats, fracs)
+    return lib().orc_mix_range(*tabargs(ph), ph.mass, ph.ctau, len(m), P(m, C.c_int32), P(f),
+                               int(mixed), k)
+
+
+def ehs_path(ph, mats, fracs, k, er=0.05):
+    m, f = _mix(mats, fracs)
+    return lib().orc_ehs_path(*tabargs(ph), ph.mass, ph.ctau, len(m), P(m, C.c_int32), P(f), k, er)
