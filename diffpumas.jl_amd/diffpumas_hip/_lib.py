@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libdiffpumas_hip.so")
+# DIFFPUMAS_HIP_LIB selects another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("DIFFPUMAS_HIP_LIB") or os.path.join(os.path.dirname(_HERE),
+                                                                "libdiffpumas_hip.so")
 
 DP_OK = 0
 DP_E_ARGUMENT = -1

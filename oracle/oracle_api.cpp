@@ -653,3 +653,132 @@ double orc_ehs_path(int M, int K, const double* data, const double* scal, double
 }
 
 }  // extern "C"
+
+// ============================================ fixed-path MC over many paths
+extern "C" {
+
+// compute_directional_flux_mc (src/Tomography.jl:979-1041) for every path.
+// Cell c uses runtime mixture media[cell_medium[c]] at cell_density[c].
+// Event s of path r draws from Philox keyed (seed, r * n_samples + s).
+// Returns 0, -2 (DomainError) or -3 (other error) with *err_event the first
+// failing event.
+int orc_mc_paths(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                 const int32_t* n_el, const double* eZ, const double* eA, const double* eI,
+                 const double* ef, double cutoff, double elastic_ratio, int n_media,
+                 const int32_t* media_n, const int32_t* media_mats, const double* media_fracs,
+                 const int32_t* cell_medium, const double* cell_density, int rock, int air,
+                 double rho_rock, double det_elev, double palt, int n_rays,
+                 const int32_t* row_ptr, const int32_t* cells, const double* lens,
+                 const double* remaining, const double* exit_z, const double* cos_t,
+                 const uint8_t* valid, int n_samples, const double* se, const double* sw,
+                 const double* sq, unsigned long long seed, int straggling, double thr_low,
+                 int threads, double* flux, double* sigma, double* w_events,
+                 long long* err_event) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  std::vector<std::vector<ElementRec>> rec(M);
+  int o = 0;
+  for (int m = 0; m < M; ++m)
+    for (int i = 0; i < n_el[m]; ++i, ++o) rec[m].push_back({eZ[o], eA[o], eI[o], ef[o]});
+  std::vector<Mixture> mixes(n_media + 2);
+  std::vector<MixedKernel> kerns(n_media + 2);
+  for (int j = 0; j < n_media; ++j) {
+    const int n = media_n[j];
+    mixes[j] = make_mixture(T, std::vector<int>(media_mats + 4 * j, media_mats + 4 * j + n),
+                            std::vector<double>(media_fracs + 4 * j, media_fracs + 4 * j + n));
+  }
+  mixes[n_media] = make_mixture(T, {rock}, {1.0});
+  mixes[n_media + 1] = make_mixture(T, {air}, {1.0});
+  for (int j = 0; j < n_media + 2; ++j)
+    if (straggling) kerns[j] = prepare_mixed_kernel(T, mixes[j], rec, cutoff);
+  std::vector<Medium> media(n_media + 2);
+  for (int j = 0; j < n_media + 2; ++j) media[j] = {&mixes[j], &kerns[j]};
+  int n_cells = 0;
+  for (int s = 0; s < row_ptr[n_rays]; ++s) n_cells = std::max(n_cells, cells[s] + 1);
+  std::vector<Medium> cm(n_cells);
+  for (int c = 0; c < n_cells; ++c) cm[c] = media[cell_medium[c]];
+  McSetup S{};
+  S.T = &T; S.cutoff = cutoff; S.elastic_ratio = elastic_ratio;
+  double emax = 1e9;
+  for (int m = 0; m < M; ++m) emax = std::min(emax, T.arr(m, T_ENERGY)[K - 1]);
+  PathMc P{&S, cm.data(), cell_density, &media[n_media], &media[n_media + 1], rho_rock, det_elev,
+           palt, thr_low, emax, emax, straggling != 0};
+  if (threads > 0) omp_set_num_threads(threads);
+  int status = 0;
+  long long first_bad = -1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 0; r < n_rays; ++r) {
+    if ((valid && !valid[r]) || !(cos_t[r] > 0.0)) {
+      flux[r] = sigma[r] = NAN;
+      continue;
+    }
+    const double st = std::sqrt(std::max(0.0, 1.0 - cos_t[r] * cos_t[r]));
+    const double dir[3] = {st, 0.0, cos_t[r]};
+    const int b = row_ptr[r], ns = row_ptr[r + 1] - b;
+    double s1 = 0, s2 = 0;
+    for (int s = 0; s < n_samples; ++s) {
+      const long long e = (long long)r * n_samples + s;
+      Philox4x32 rng(seed, (uint64_t)e);
+      double wi = 0.0;
+      try {
+        wi = path_event(P, cells + b, lens + b, ns, remaining[r], exit_z[r], cos_t[r], dir, se[s],
+                        sq[s], sw[s], rng);
+      } catch (const DomainErr&) {
+#pragma omp critical
+        { if (status == 0 || e < first_bad) { status = -2; first_bad = e; } }
+      } catch (const std::exception&) {
+#pragma omp critical
+        { if (status == 0 || e < first_bad) { status = -3; first_bad = e; } }
+      }
+      if (w_events) w_events[e] = wi;
+      s1 += wi;
+      s2 += wi * wi;
+    }
+    const double n = std::max(n_samples, 1);
+    const double f = s1 / n;
+    flux[r] = f;
+    sigma[r] = std::sqrt(std::max(0.0, (s2 / n - f * f) / std::max(1.0, n - 1.0)));
+  }
+  if (err_event) *err_event = first_bad;
+  return status;
+}
+
+}  // extern "C"
+
+extern "C" {
+// propagate_mc_uniform_segment on one material (src/Tomography.jl:876-920):
+// st = E, w, t, L, X in and out.  Returns 0, -2 or -3.
+int orc_mc_uniform_segment(int M, int K, const double* data, const double* scal, double mass,
+                           double ctau, const int32_t* n_el, const double* eZ, const double* eA,
+                           const double* eI, const double* ef, double cutoff, int material,
+                           double density, double distance, int straggling, double thr_low,
+                           unsigned long long seed, double* st) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  std::vector<std::vector<ElementRec>> rec(M);
+  int o = 0;
+  for (int m = 0; m < M; ++m)
+    for (int i = 0; i < n_el[m]; ++i, ++o) rec[m].push_back({eZ[o], eA[o], eI[o], ef[o]});
+  Mixture mx = make_mixture(T, {material}, {1.0});
+  MixedKernel k = straggling ? prepare_mixed_kernel(T, mx, rec, cutoff) : MixedKernel{};
+  Medium md{&mx, &k};
+  McSetup S{};
+  S.T = &T; S.cutoff = cutoff;
+  McState s;
+  s.E = st[0]; s.w = st[1]; s.t = st[2]; s.L = st[3]; s.X = st[4];
+  s.pos[0] = s.pos[1] = s.pos[2] = 0.0;
+  s.dir[0] = s.dir[1] = 0.0; s.dir[2] = -1.0;
+  double emax = 1e9;
+  for (int m = 0; m < M; ++m) emax = std::min(emax, T.arr(m, T_ENERGY)[K - 1]);
+  Philox4x32 rng(seed, 0);
+  int rc = 0;
+  try {
+    mc_path_segment(S, md, s, density, false, 0.0, 1.0, distance, straggling != 0, thr_low, emax,
+                    rng);
+  } catch (const DomainErr&) {
+    rc = -2;
+  } catch (const std::exception&) {
+    rc = -3;
+  }
+  st[0] = s.E; st[1] = s.w; st[2] = s.t; st[3] = s.L; st[4] = s.X;
+  return rc;
+}
+}  // extern "C"

@@ -105,6 +105,8 @@ inline void straggled_step(const McSetup& S, const Medium& md, McState& st, doub
   }
   const double dt = step * mass / 2 * (1.0 / std::sqrt(ki * (ki + 2 * mass)) + 1.0 / std::sqrt(ckf * (ckf + 2 * mass)));
   for (int a = 0; a < 3; ++a) st.pos[a] -= step * st.dir[a];
+  st.L += step;
+  st.X += dX;
   const double wf = mix_stop(T, true, mix, ckf) / mix_stop(T, true, mix, ki);
   st.w = st.w * wf * std::exp(-dt / T.ctau) * dw;
   st.t += dt;
@@ -183,6 +185,8 @@ inline void csda_mix_step(const McSetup& S, const Medium& md, McState& st, doubl
   const double beta = std::sqrt(std::max(1.0 - 1.0 / (ga * ga), 1e-12));
   const double dt = ds / (beta * ga);
   for (int a = 0; a < 3; ++a) st.pos[a] -= ds * st.dir[a];
+  st.L += ds;
+  st.X += gs;
   const double wf = mix_stop(T, false, mix, kf) / mix_stop(T, false, mix, ki);
   st.w = st.w * wf * std::exp(-dt / T.ctau);
   st.t += dt;
@@ -259,6 +263,90 @@ inline double flat_event(const McSetup& S, double energy, double cos_t, double s
     return st.w * flux_gccly(cf, st.E, charge);
   }
   return 0.0;
+}
+
+// ------------------------------------------------ fixed-path (tomography) MC
+// _mc_segment_step, src/Tomography.jl:873-874
+inline double mc_segment_step(double remaining, double max_step) {
+  return std::min(remaining, std::max(1e-6, std::min(remaining + 1e-7, max_step)));
+}
+
+// propagate_mc_uniform_segment / propagate_mc_air_segment (src/Tomography.jl:
+// 876-969) with scattering off.  air: density from the altitude of the
+// current position.  Throws on a step that makes no spatial progress.
+inline void mc_path_segment(const McSetup& S, const Medium& md, McState& st, double density,
+                            bool air, double det_elev, double cos_t, double distance,
+                            bool straggling, double thr_low, double energy_limit,
+                            Philox4x32& rng) {
+  const Tables& T = *S.T;
+  double remaining = distance;
+  while (remaining > 1e-8 && st.E < energy_limit - EPS64) {
+    if (st.w <= 0.0 || !std::isfinite(st.w)) break;
+    const double cur = st.E < thr_low - EPS64 ? std::min(thr_low, energy_limit) : energy_limit;
+    double dens = density, max_step;
+    const double Xi = mix_range(T, false, *md.mix, st.E);
+    if (air) {
+      dens = air_density(det_elev + st.pos[2]);
+      max_step = 0.01 * Xi / std::max(dens, 1e-8);
+      max_step = std::min(max_step, 0.05 * 12e3 / std::max(cos_t, 0.05));
+    } else {
+      max_step = 0.01 * Xi / std::max(dens, 1e-6);
+    }
+    const double step = mc_segment_step(remaining, max_step);
+    const double oldL = st.L, oldE = st.E;
+    if (st.E < thr_low - EPS64) {
+      if (straggling) straggled_step(S, md, st, dens, step, false, cur, rng);
+      else csda_mix_step(S, md, st, dens, step);
+    } else {
+      if (straggling) mixed_step(S, md, st, dens, step, cur, rng);
+      else csda_mix_step(S, md, st, dens, step);
+    }
+    const double actual = st.L - oldL;
+    if (st.w <= 0.0 || !std::isfinite(st.w)) break;
+    if (actual <= 0 && oldE < cur && cur < energy_limit && st.E >= cur) continue;
+    if (!(actual > 0)) {
+      if (st.E >= energy_limit) break;
+      throw std::runtime_error("MC segment made no spatial progress");
+    }
+    remaining = std::max(0.0, remaining - actual);
+  }
+}
+
+// One event of compute_directional_flux_mc (src/Tomography.jl:979-1041):
+// cells, dry-rock remainder, atmosphere; returns 2 w_sample flux_single.
+struct PathMc {
+  const McSetup* S;
+  const Medium* cell_media;   // per cell
+  const double* cell_density; // per cell
+  const Medium* rock;
+  const Medium* air;
+  double rho_rock, det_elev, palt, thr_low, energy_limit, source_max;
+  bool straggling;
+};
+inline double path_event(const PathMc& P, const int32_t* cells, const double* lens, int ns,
+                         double remaining, double exit_z, double zenith_cos, const double dir[3],
+                         double energy, double charge, double weight, Philox4x32& rng) {
+  McState st;
+  st.E = energy; st.w = 1.0; st.t = 0.0;
+  for (int a = 0; a < 3; ++a) { st.pos[a] = 0.0; st.dir[a] = -dir[a]; }
+  for (int s = 0; s < ns; ++s) {
+    const int c = cells[s];
+    mc_path_segment(*P.S, P.cell_media[c], st, P.cell_density[c], false, P.det_elev, zenith_cos,
+                    lens[s], P.straggling, P.thr_low, P.energy_limit, rng);
+    if (st.w <= 0.0 || !std::isfinite(st.w)) break;
+  }
+  if (st.w > 0.0 && std::isfinite(st.w) && remaining > 1e-8)
+    mc_path_segment(*P.S, *P.rock, st, P.rho_rock, false, P.det_elev, zenith_cos, remaining,
+                    P.straggling, P.thr_low, P.energy_limit, rng);
+  if (st.w > 0.0 && std::isfinite(st.w)) {
+    const double air_d = std::max(0.0, P.palt - exit_z) / std::max(zenith_cos, 1e-3);
+    mc_path_segment(*P.S, *P.air, st, 0.0, true, P.det_elev, zenith_cos, air_d, P.straggling,
+                    P.thr_low, P.energy_limit, rng);
+  }
+  double f = 0.0;
+  if (st.w > 0.0 && std::isfinite(st.w) && st.E < P.source_max)
+    f = st.w * flux_gccly(zenith_cos, st.E, charge);
+  return 2.0 * weight * f;
 }
 
 }  // namespace orc

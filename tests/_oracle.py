@@ -435,3 +435,58 @@ def mix_range(ph, mats, fracs, k, mixed=True):
 def ehs_path(ph, mats, fracs, k, er=0.05):
     m, f = _mix(mats, fracs)
     return lib().orc_ehs_path(*tabargs(ph), ph.mass, ph.ctau, len(m), P(m, C.c_int32), P(f), k, er)
+
+
+# ------------------------------------------------------------ fixed-path MC
+_SIGS["orc_mc_paths"] = (C.c_int, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, i_p, d_p,
+                                   d_p, d_p, d_p, C.c_double, C.c_double, C.c_int, i_p, i_p, d_p,
+                                   i_p, d_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, i_p, i_p, d_p, d_p, d_p, d_p, u8_p,
+                                   C.c_int, d_p, d_p, d_p, C.c_ulonglong, C.c_int, C.c_double,
+                                   C.c_int, d_p, d_p, d_p, C.POINTER(C.c_longlong)])
+_SIGS["orc_mc_uniform_segment"] = (C.c_int, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double,
+                                             i_p, d_p, d_p, d_p, d_p, C.c_double, C.c_int,
+                                             C.c_double, C.c_double, C.c_int, C.c_double,
+                                             C.c_ulonglong, d_p])
+
+
+def mc_paths(ph, media, cell_medium, cell_density, rock, air, rho_rock, site, paths, samples,
+             seed, straggling=True, thr_low=100.0, threads=0, per_event=False, cutoff=0.05,
+             elastic_ratio=0.05):
+    """compute_directional_flux_mc for every path: (flux, sigma[, events]).
+    media: [(mats, fracs)], cell_medium/cell_density per cell; samples
+    (energy, weight, charge) arrays."""
+    n_el, Z, A, I, f = recipes_arrays(ph)
+    mn = i32([len(m) for m, _ in media])
+    mm = np.zeros((len(media), 4), np.int32)
+    mf = np.zeros((len(media), 4))
+    for j, (m, fr) in enumerate(media):
+        mm[j, :len(m)] = m
+        mf[j, :len(fr)] = fr
+    cmed, cden = i32(cell_medium), f64(cell_density)
+    args, keep = _wrays(paths)
+    se, sw, sq = (f64(a) for a in samples)
+    n = args[0]
+    flux, sigma = np.zeros(n), np.zeros(n)
+    ev = np.zeros(n * len(se)) if per_event else None
+    bad = C.c_longlong(-1)
+    rc = lib().orc_mc_paths(*tabargs(ph), ph.mass, ph.ctau, P(n_el, C.c_int32), P(Z), P(A), P(I),
+                            P(f), cutoff, elastic_ratio, len(media), P(mn, C.c_int32),
+                            P(mm, C.c_int32), P(mf), P(cmed, C.c_int32), P(cden), rock, air,
+                            rho_rock, site[0], site[1], *args[:6], args[6], args[7], len(se),
+                            P(se), P(sw), P(sq), seed, int(straggling), thr_low, threads,
+                            P(flux), P(sigma), P(ev), C.byref(bad))
+    if rc != 0:
+        raise RuntimeError(f"oracle path MC error {rc} at event {bad.value}")
+    return (flux, sigma, ev) if per_event else (flux, sigma)
+
+
+def mc_uniform_segment(ph, material, density, distance, state, straggling=True, thr_low=100.0,
+                       seed=0, cutoff=0.05):
+    n_el, Z, A, I, f = recipes_arrays(ph)
+    st = f64([state.get("E"), state.get("w", 1.0), state.get("t", 0.0), state.get("L", 0.0),
+              state.get("X", 0.0)])
+    rc = lib().orc_mc_uniform_segment(*tabargs(ph), ph.mass, ph.ctau, P(n_el, C.c_int32), P(Z),
+                                      P(A), P(I), P(f), cutoff, material, density, distance,
+                                      int(straggling), thr_low, seed, P(st))
+    return rc, dict(E=st[0], w=st[1], t=st[2], L=st[3], X=st[4])
