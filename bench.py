@@ -17,6 +17,12 @@ Legs
             Philox4x32-10 keyed by (seed, event).  Unit: events/s.  This is
             the headline value.
 
+  wmodel  : water-fraction CSDA flux + sparse Jacobian (tomography forward
+            model) over 4096 bins of a 64x64x32 grid, 32 energy nodes.
+            Unit: rays*voxels/s.
+  pathmc  : fixed-path straggled MC over 256 bins x 1e4 events of a lognormal
+            density field with a cavity.  Unit: events/s (kernel time).
+
 Inputs are resident in HBM before the timed region.  Timing: W warm-up
 steps, then K steps between device synchronisations; kernel time of the
 dominant kernel from HIP events on the stream it runs on.
@@ -203,6 +209,205 @@ def cpu_adjoint_baseline(physics, problem, budget_s=12.0):
         n *= 2
 
 
+def tomo_workload(dp, n_dirs_per_det=1024, seed=31):
+    """64x64x32 voxels of 10 m, 4 detectors 20 m below the block, directions
+    with elevation U[15, 75] deg and azimuth U[0, 360); flat surface 50 m above
+    the block.  Returns (edges, DirectionalPaths)."""
+    import numpy as np
+    nx, ny, nz, size = 64, 64, 32, 10.0
+    xe = np.linspace(0.0, nx * size, nx + 1)
+    ye = np.linspace(0.0, ny * size, ny + 1)
+    ze = np.linspace(0.0, nz * size, nz + 1)
+    rng = np.random.default_rng(seed)
+    dets = [(160.0, 160.0, -20.0), (480.0, 160.0, -20.0), (160.0, 480.0, -20.0),
+            (480.0, 480.0, -20.0)]
+    zen, az, org = [], [], []
+    for d in dets:
+        zen.append(90.0 - rng.uniform(15.0, 75.0, n_dirs_per_det))
+        az.append(rng.uniform(0.0, 360.0, n_dirs_per_det))
+        org.append(np.tile(d, (n_dirs_per_det, 1)))
+    zen, az, org = np.concatenate(zen), np.concatenate(az), np.concatenate(org)
+    height = nz * size + 50.0 - org[:, 2]
+    cz = np.cos(np.radians(zen))
+    surface = height / cz
+    paths = dp.trace_voxel_paths(xe, ye, ze, zen, az, surface, height, detectors=org)
+    return (nx, ny, nz), paths
+
+
+def lognormal_cavity(shape, seed=5):
+    """rho = 2650 exp(0.1 N(0,1) - 0.005) with an ellipsoidal 1800 kg/m3 cavity."""
+    import numpy as np
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    rho = 2650.0 * np.exp(0.1 * rng.standard_normal(nx * ny * nz) - 0.005)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    inside = (((i - 40) / 8.0) ** 2 + ((j - 28) / 6.0) ** 2 + ((k - 14) / 5.0) ** 2) <= 1.0
+    rho[inside.ravel()] = 1800.0
+    return rho
+
+
+def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32):
+    """w-model flux + sparse Jacobian over all tomography bins, device-resident."""
+    import ctypes as C
+    import numpy as np
+    shape, paths = tomo_workload(dp)
+    n_cells = shape[0] * shape[1] * shape[2]
+    w = np.random.default_rng(9).uniform(0.0, 0.3, n_cells)
+    s = dp.csda_energy_quadrature(n_nodes, 1.0, 1e4)
+    m = dp.MaterialConfig(physics.get_material_index("StandardRock"),
+                          physics.get_material_index("Water"),
+                          physics.get_material_index("Air"), -1, 2650.0, 1000.0)
+    dev = torch.device("cuda:0")
+    T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+    t_rp, t_cell, t_len = T(paths.row_ptr, torch.int32), T(paths.cells, torch.int32), T(paths.lengths, torch.float64)
+    t_cos, t_rem, t_ez = T(paths.cos_theta, torch.float64), T(paths.remaining_rock, torch.float64), T(paths.exit_z, torch.float64)
+    t_val = T(paths.valid, torch.uint8)
+    t_w, t_sh = T(w, torch.float64), torch.zeros(n_cells, dtype=torch.uint8, device=dev)
+    t_e, t_wt, t_q = T(s.energy, torch.float64), T(s.weight, torch.float64), T(s.charge, torch.float64)
+    flux = torch.zeros(paths.n_rays, dtype=torch.float64, device=dev)
+    jac = torch.zeros(max(paths.n_segments, 1), dtype=torch.float64, device=dev)
+    dptr = lambda t, ty: C.cast(t.data_ptr(), ty)
+    cp = L.DirectionalPaths(paths.n_rays, paths.n_segments, paths.max_segments(),
+                            dptr(t_rp, L.i32_p), dptr(t_cell, L.i32_p), dptr(t_len, L.dbl_p),
+                            dptr(t_cos, L.dbl_p), dptr(t_rem, L.dbl_p), dptr(t_ez, L.dbl_p),
+                            dptr(t_val, C.POINTER(C.c_uint8)))
+    es = L.EnergySamples(len(s.energy), dptr(t_e, L.dbl_p), dptr(t_wt, L.dbl_p), dptr(t_q, L.dbl_p))
+    mc = m.c()
+    site = L.SiteConfig(1000.0, 3e4)
+    h = physics.device_handle(ctx)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream.cuda_stream)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        L.check(ctx.lib.dp_csda_w_flux_jacobian_dev(ctx.h, h, C.byref(mc), C.byref(site),
+                                                    C.byref(cp), n_cells, t_w.data_ptr(),
+                                                    t_sh.data_ptr(), C.byref(es),
+                                                    flux.data_ptr(), jac.data_ptr()))
+        if ev is not None:
+            ev[1].record(stream)
+
+    log(f"wmodel: rays={paths.n_rays} segments={paths.n_segments} max_seg={paths.max_segments()}")
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(evs[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    ctx.set_stream(None)
+    ms_step = (t1 - t0) * 1e3 / steps
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    units = paths.n_segments
+    alg = 29.0 * units + 8.0 * paths.n_rays
+    achieved = alg / (kern_ms * 1e-3) / 1e9
+    return {"value": units / (ms_step * 1e-3), "unit": "rays*voxels/s", "ms_per_step": ms_step,
+            "kernel_ms": kern_ms, "rays": paths.n_rays, "segments": units, "nodes": n_nodes,
+            "ray_nodes_per_s": paths.n_rays * n_nodes / (ms_step * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+            "_problem": (paths, w, s, m)}
+
+
+def run_pathmc(dp, physics, ctx, n_rays=256, n_events=10000):
+    """Fixed-path straggled MC over the lognormal-cavity density field: n_rays
+    bins of the tomography workload, n_events per bin, one launch."""
+    import numpy as np
+    shape, paths_all = tomo_workload(dp)
+    idx = np.linspace(0, paths_all.n_rays - 1, n_rays).astype(int)
+    rp = [0]
+    cells, lens = [], []
+    for r in idx:
+        a, b = paths_all.row_ptr[r], paths_all.row_ptr[r + 1]
+        cells.append(paths_all.cells[a:b]); lens.append(paths_all.lengths[a:b])
+        rp.append(rp[-1] + (b - a))
+    paths = dp.DirectionalPaths(rp, np.concatenate(cells), np.concatenate(lens),
+                                paths_all.zenith[idx], paths_all.azimuth[idx],
+                                paths_all.remaining_rock[idx], paths_all.exit_z[idx])
+    rho = lognormal_cavity(shape)
+    s = dp.sample_energy_set(n_events, 1.0, 1e4, 17)
+    m = dp.MaterialConfig(physics.get_material_index("StandardRock"),
+                          physics.get_material_index("Water"),
+                          physics.get_material_index("Air"), -1, 2650.0, 1000.0)
+    site = dp.SiteConfig(1000.0, 3e4)
+    args = (physics, m, site, paths, [((m.rock_material,), (1.0,))],
+            np.zeros(len(rho), np.int32), rho, s, 1)
+    dp.compute_directional_flux_mc(*args, ctx=ctx)
+    t0 = time.perf_counter()
+    f, sg = dp.compute_directional_flux_mc(*args, ctx=ctx)
+    t1 = time.perf_counter()
+    kern = ctx.last_kernel_ms()
+    ev = n_rays * n_events
+    return {"value": ev / (kern * 1e-3), "unit": "events/s", "kernel_ms": kern,
+            "wall_ms_incl_staging": (t1 - t0) * 1e3, "rays": n_rays, "events_per_ray": n_events,
+            "segments": paths.n_segments, "mean_flux": float(np.nanmean(f)),
+            "roofline": {"bound": "hbm", "achieved": (16.0 * ev + 12.0 * paths.n_segments) /
+                         (kern * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": (16.0 * ev + 12.0 * paths.n_segments) / (kern * 1e-3) / 1e9 /
+                         HBM_PEAK_GBS, "traffic": None},
+            "_problem": (paths, rho, s, m, site)}
+
+
+def cpu_wmodel_baseline(physics, problem, budget_s=10.0):
+    """The oracle's w-model flux + reverse Jacobian on a bounded sample of bins."""
+    import numpy as np
+    import _oracle as O
+    paths, w, s, m = problem
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    mc = O.matcfg_arrays(m.rock_material, m.water_material, m.air_material, m.porous_material,
+                         m.rock_density, m.water_density)
+    n = 8
+    while True:
+        idx = np.arange(0, paths.n_rays, max(1, paths.n_rays // n))[:n]
+        rp = [0]
+        cells, lens = [], []
+        for r in idx:
+            a, b = paths.row_ptr[r], paths.row_ptr[r + 1]
+            cells.append(paths.cells[a:b]); lens.append(paths.lengths[a:b])
+            rp.append(rp[-1] + (b - a))
+        sub = dict(row_ptr=np.array(rp, np.int32), cells=np.concatenate(cells),
+                   lengths=np.concatenate(lens), remaining=paths.remaining_rock[idx],
+                   exit_z=paths.exit_z[idx], cos_theta=paths.cos_theta[idx],
+                   valid=paths.valid[idx])
+        t0 = time.perf_counter()
+        O.w_jacobian(physics, mc, (1000.0, 3e4), sub, w, np.zeros(len(w), np.uint8), s.energy,
+                     s.weight, s.charge, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= paths.n_rays:
+            return {"value": rp[-1] / dt, "unit": "rays*voxels/s", "cores": threads,
+                    "kind": "port", "sample": f"{len(idx)} of {paths.n_rays} bins, "
+                                              f"{len(s.energy)} nodes, {dt:.2f} s"}
+        n *= 2
+
+
+def cpu_pathmc_baseline(physics, problem, budget_s=10.0):
+    import numpy as np
+    import _oracle as O
+    paths, rho, s, m, site = problem
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = 256
+    while True:
+        t0 = time.perf_counter()
+        sub = dict(row_ptr=paths.row_ptr, cells=paths.cells,
+                   lengths=paths.lengths, remaining=paths.remaining_rock, exit_z=paths.exit_z,
+                   cos_theta=paths.cos_theta, valid=paths.valid)
+        O.mc_paths(physics, [([m.rock_material], [1.0])], np.zeros(len(rho), np.int32), rho,
+                   m.rock_material, m.air_material, m.rock_density,
+                   (site.detector_elevation, site.primary_altitude), sub,
+                   (s.energy[:n], s.weight[:n], s.charge[:n]), 1, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= len(s.energy):
+            return {"value": paths.n_rays * n / dt, "unit": "events/s", "cores": threads,
+                    "kind": "port", "sample": f"{paths.n_rays} bins x {n} events, {dt:.2f} s"}
+        n *= 2
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,8 +437,16 @@ def main():
     log(f"adjoint: {adj['value']:.4g} rays*voxels/s, {adj['ms_per_step']:.3f} ms/step, "
         f"kernel {adj['kernel_ms']:.3f} ms")
     problem = adj.pop("_problem")
+    wm = run_wmodel(dp, L, physics, ctx, torch, a.steps, a.warmup)
+    log(f"wmodel: {wm['value']:.4g} rays*voxels/s, {wm['ms_per_step']:.3f} ms/step")
+    wprob = wm.pop("_problem")
+    pm = run_pathmc(dp, physics, ctx)
+    log(f"pathmc: {pm['value']:.4g} events/s, kernel {pm['kernel_ms']:.3f} ms")
+    pprob = pm.pop("_problem")
     cpu = None if a.no_cpu else cpu_mc_baseline(physics)
     cpu_adj = None if a.no_cpu else cpu_adjoint_baseline(physics, problem)
+    wm["cpu_baseline"] = None if a.no_cpu else cpu_wmodel_baseline(physics, wprob)
+    pm["cpu_baseline"] = None if a.no_cpu else cpu_pathmc_baseline(physics, pprob)
     out = {
         "metric": "straggled backward-MC events/s (+ adjoint rays*voxels/s), 1 GPU, % HBM roofline",
         "value": mc["value"], "unit": "events/s", "n_gpus": 1, "steps": a.steps,
@@ -253,6 +466,11 @@ def main():
                     "rays": adj["rays"], "segments": adj["segments"], "nodes": adj["nodes"],
                     "ms_per_step": adj["ms_per_step"], "kernel_ms": adj["kernel_ms"],
                     "roofline": adj["roofline"], "cpu_baseline": cpu_adj},
+        "wmodel": dict(wm, workload="water-fraction CSDA flux + sparse Jacobian, 64x64x32 "
+                                    "voxels of 10 m, w U[0, 0.3], 4 detectors x 1024 "
+                                    "directions, 32 nodes 1-1e4 GeV"),
+        "pathmc": dict(pm, workload="fixed-path straggled MC, 64x64x32 lognormal density "
+                                    "field with a 1800 kg/m3 cavity, 256 bins x 1e4 events"),
     }
     if a.calib_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)

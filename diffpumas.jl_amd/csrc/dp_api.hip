@@ -135,6 +135,11 @@ int dp_upload_tables(dp_ctx c, const dp_table_desc* d, dp_tables* out) {
   }
   for (int m = 0; m < M; ++m)
     t->last_energy.push_back(d->data[((size_t)m * DP_T_NARRAYS + DP_T_ENERGY) * K + K - 1]);
+  // one energy grid for every material (energies and log energies identical)
+  t->shared_grid = true;
+  for (int m = 1; m < M; ++m)
+    if (std::memcmp(d->data, d->data + (size_t)m * DP_T_NARRAYS * K, 2 * (size_t)K * sizeof(double)))
+      t->shared_grid = false;
   size_t nb = (size_t)M * DP_T_NARRAYS * K * sizeof(double);
   if (hipMalloc(&t->data, nb) != hipSuccess || hipMalloc(&t->scalars, (size_t)M * 4 * sizeof(double)) != hipSuccess) {
     if (t->data) (void)hipFree(t->data);

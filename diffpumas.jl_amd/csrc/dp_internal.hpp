@@ -47,6 +47,7 @@ struct Prepared {
 
 struct dp_tables_s {
   int32_t n_materials = 0, n_knots = 0, pow2 = 0;
+  bool shared_grid = false;    // all materials on one energy grid
   double mass = 0, ctau = 0, cutoff = 0.05, elastic_ratio = 0.05;
   double* data = nullptr;      // device: M * NARR * K
   double* scalars = nullptr;   // device: M * 4

@@ -13,6 +13,8 @@
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
 #include "dp_transport.hpp"
+#include "dp_transport_fast.hpp"
+#include "dp_media.hpp"
 #include <algorithm>
 #include <cmath>
 
@@ -217,7 +219,7 @@ __device__ double flat_event(const StochArgs& a, double energy, double charge, P
                              int& err) {
   const DevTables& T = a.T;
   McState st;
-  st.E = energy; st.w = 1.0; st.t = 0.0;
+  st.E = energy; st.w = 1.0; st.t = 0.0; st.L = 0.0;
   st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
   st.dir[0] = -a.sin_t; st.dir[1] = 0.0; st.dir[2] = -a.cos_t;
   const double palt = a.palt;
@@ -281,6 +283,80 @@ __device__ double flat_event(const StochArgs& a, double energy, double charge, P
   return 0.0;
 }
 
+// flat_event on tables with one shared energy grid: each step locates the
+// event's energy once and hands it to the step functions.
+__device__ double flat_event_f(const StochArgs& a, double energy, double charge, Philox& rng,
+                               int& err) {
+  const DevTables& T = a.T;
+  McState st;
+  st.E = energy; st.w = 1.0; st.t = 0.0; st.L = 0.0;
+  st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
+  st.dir[0] = -a.sin_t; st.dir[1] = 0.0; st.dir[2] = -a.cos_t;
+  const double palt = a.palt;
+  int g = T.K / 2;  // interval guess of st.E
+  int outer = 0;
+  for (; outer < 64; ++outer) {
+    if (!(st.E < a.thr - EPS64)) break;
+    double z = st.pos[2];
+    if (z < 0 || z >= palt - EPS64) break;
+    if (st.w <= 0 || !isfinite(st.w)) break;
+    const bool below = st.E < a.sw - EPS64;
+    const bool use_str = below && a.straggling;
+    const bool use_mixed = !below && a.straggling;
+    const bool use_scat = below && a.scattering;
+    const double limit = below ? fmin(a.sw, a.thr) : a.thr;
+    for (int inner = 0; inner < 50000; ++inner) {
+      z = st.pos[2];
+      if (z < 0 || z >= palt - EPS64) break;
+      if (st.E >= limit - EPS64) break;
+      if (st.w <= 0 || !isfinite(st.w)) break;
+      const double uz = -st.dir[2];
+      const double porous_z = a.has_porous ? a.thick - a.porous_thick : a.thick;
+      const Medium* md;
+      double density, stb;
+      const bool in_air = !(z < a.thick);
+      if (!in_air) {
+        if (a.has_porous && z >= porous_z) { md = &a.porous; density = a.porous_rho; }
+        else { md = &a.rock; density = a.rock_rho; }
+        if (uz > EPS64) stb = (((a.has_porous && z < porous_z) ? porous_z : a.thick) - z) / uz;
+        else if (uz < -EPS64) stb = (((a.has_porous && z >= porous_z) ? porous_z : 0.0) - z) / uz;
+        else stb = 1000.0;
+      } else {
+        md = &a.air;
+        density = 1.205 * exp(-z / 12e3);
+        if (uz > EPS64) stb = (palt - z) / uz;
+        else if (uz < -EPS64) stb = (a.thick - z) / uz;
+        else stb = 1000.0;
+      }
+      // _backward_geometric_step_limit, src/Geometry.jl:463-476
+      const Loc L = loc_e(T, st.E, log(st.E), g);
+      const double Xi = mix_range_L(T, md->mix, false, L);
+      double lim = 0.01 * Xi / fmax(density, 1e-6);
+      double lb0 = 0.0;
+      if (use_scat) {
+        lb0 = mix_transport_path_L(T, md->mix, L);
+        if (lb0 > 0 && density > 0) lim = fmin(lim, 0.01 * lb0 / density);
+      }
+      if (in_air) lim = fmin(lim, 0.05 * 12e3 / fmax(fabs(uz), 0.05));
+      double step = fmin(stb + 1e-7, lim);
+      step = fmin(fmax(step, 1e-7), 1000.0);
+      g = L.i;
+      if (use_str) straggled_step_f(T, *md, st, L, lb0, density, step, use_scat, limit, rng, err, &g);
+      else if (use_mixed) mixed_step_f(T, *md, st, L, density, step, limit, rng, err);
+      else csda_mix_step_f(T, *md, st, L, density, step, &g);
+      if (err) return 0.0;
+    }
+  }
+  if (outer == 64) { err |= ERR_CAP; return 0.0; }
+  if (st.pos[2] >= palt - 1.0 && st.E < a.thr) {
+    const double cf = fmin(fmax(-st.dir[2], 0.0), 1.0);
+    if (cf <= 0) return 0.0;
+    return st.w * flux_gccly(cf, st.E, charge);
+  }
+  return 0.0;
+}
+
+template <bool FAST>
 __global__ __launch_bounds__(128, 4) void mc_flat_stochastic_kernel(StochArgs a) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += stride) {
@@ -294,7 +370,7 @@ __global__ __launch_bounds__(128, 4) void mc_flat_stochastic_kernel(StochArgs a)
     const double q = rng.uniform() > 0.5 ? 1.0 : -1.0;
     w *= 2.0;
     int err = 0;
-    const double f = flat_event(a, k, q, rng, err);
+    const double f = FAST ? flat_event_f(a, k, q, rng, err) : flat_event(a, k, q, rng, err);
     if (err) {
       atomicOr(a.err, err);
       atomicMin(a.bad_event, (unsigned long long)e);
@@ -305,6 +381,9 @@ __global__ __launch_bounds__(128, 4) void mc_flat_stochastic_kernel(StochArgs a)
   }
 }
 
+}  // namespace
+
+namespace dph {
 // ---------------------------------------------------- host preparation
 // Mixture tables (src/Physics.jl:1049-1103) and MixedKernel snapshots
 // (src/MixedTransport.jl:55-97) for a medium, built once per physics object
@@ -467,7 +546,8 @@ int prepare_kernel(dp_tables_s* t, const std::vector<int>& mats, const std::vect
   return DP_OK;
 }
 
-int make_medium(dp_tables_s* t, std::vector<int> mats, std::vector<double> fr, Medium& md) {
+int make_medium(dp_tables_s* t, std::vector<int> mats, std::vector<double> fr, Medium& md,
+                bool with_kernel) {
   double tot = 0;
   for (double f : fr) tot += f;
   if (!(tot > 0)) return dph::fail(DP_E_ARGUMENT, "mixture fractions must sum to a positive value");
@@ -483,6 +563,8 @@ int make_medium(dp_tables_s* t, std::vector<int> mats, std::vector<double> fr, M
     const int K = t->n_knots;
     md.mix.e = b; md.mix.le = b + K; md.mix.crange = b + 2 * K; md.mix.mrange = b + 3 * K;
   }
+  md.kern = DevKernel{};
+  if (!with_kernel) return DP_OK;
   dph::Prepared* k;
   int rc = prepare_kernel(t, mats, fr, &k);
   if (rc) return rc;
@@ -494,6 +576,11 @@ int make_medium(dp_tables_s* t, std::vector<int> mats, std::vector<double> fr, M
   md.kern.onset = k->onset; md.kern.cutoff = k->cutoff;
   return DP_OK;
 }
+
+}  // namespace dph
+
+namespace {
+using dph::make_medium;
 
 __global__ void reset_bad(unsigned long long* b) { if (threadIdx.x == 0) *b = ~0ull; }
 
@@ -653,14 +740,17 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     s.thr = thr; s.sw = cfg->mode_switch;
     s.straggling = cfg->straggling != 0; s.scattering = cfg->scattering != 0;
     s.n = n; s.seed = cfg->seed; s.out_w = out_w; s.err = flags; s.bad_event = bad;
+    const bool fast = t->shared_grid;
     int occ = 0;
-    DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel, 128, 0));
+    if (fast) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<true>, 128, 0));
+    else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<false>, 128, 0));
     if (occ < 1) occ = 1;
     int64_t want = (n + 127) / 128;
     int64_t grid = (int64_t)c->n_cu * occ * 4;
     if (grid > want) grid = want;
     (void)hipEventRecord(c->ev0, c->stream);
-    hipLaunchKernelGGL(mc_flat_stochastic_kernel, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
+    if (fast) hipLaunchKernelGGL(mc_flat_stochastic_kernel<true>, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
+    else hipLaunchKernelGGL(mc_flat_stochastic_kernel<false>, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
     (void)hipEventRecord(c->ev1, c->stream);
     DP_HIP(hipGetLastError());
   }

@@ -447,6 +447,7 @@ __device__ __noinline__ static DelOut reverse_del_event(const DevKernel& c, doub
 struct McState {
   double E, w, t;
   double pos[3], dir[3];
+  double L;  // distance travelled (State.distance)
 };
 
 // transport_backward_step_full, mode = :straggled, src/Geometry.jl:148-357
@@ -530,6 +531,7 @@ __device__ __noinline__ static void straggled_step(const DevTables& T, const Med
   const double dt = step * mass / 2 *
                     (1.0 / sqrt(ki * (ki + 2 * mass)) + 1.0 / sqrt(ckf * (ckf + 2 * mass)));
   for (int a = 0; a < 3; ++a) st.pos[a] -= step * st.dir[a];
+  st.L += step;
   const double wf = mix_sum(T, x, A_MIXED_STOP, ckf, log(ckf)) / mix_sum(T, x, A_MIXED_STOP, ki, lki);
   st.w = st.w * wf * exp(-dt / T.ctau) * dw;
   st.t += dt;
@@ -598,6 +600,7 @@ __device__ __noinline__ static void mixed_step(const DevTables& T, const Medium&
     return;
   }
   const double dd = travelled / density;
+  st.L += dd;
   st.E = k;
   st.t += proper;
   st.w = weight;
@@ -615,6 +618,7 @@ __device__ void csda_mix_step(const DevTables& T, const Medium& md, McState& st,
   const double beta = sqrt(fmax(1.0 - 1.0 / (ga * ga), 1e-12));
   const double dt = ds / (beta * ga);
   for (int a = 0; a < 3; ++a) st.pos[a] -= ds * st.dir[a];
+  st.L += ds;
   const double wf = mix_sum(T, x, A_CSDA_STOP, kf, log(kf)) / mix_sum(T, x, A_CSDA_STOP, ki, lki);
   st.w = st.w * wf * exp(-dt / T.ctau);
   st.t += dt;

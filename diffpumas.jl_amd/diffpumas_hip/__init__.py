@@ -15,7 +15,9 @@ from .tomography import (DirectionalPaths, EnergySamples, MaterialConfig, SiteCo
                          assemble_forward_and_jacobian, cell_density,
                          compute_directional_flux_csda, csda_energy_quadrature,
                          direction_vector, directional_flux_and_grad_csda,
-                         flux_and_segment_jacobian, sample_energy_set, trace_voxel_paths)
+                         flux_and_segment_jacobian, sample_energy_set, trace_voxel_paths,
+                         mc_cell_mixture_and_density, build_cell_properties_for_mc,
+                         compute_directional_flux_mc)
 
 __all__ = [
     "ArgumentError", "CapacityError", "Context", "DomainError", "HipError", "TransportError",
@@ -27,4 +29,5 @@ __all__ = [
     "assemble_forward_and_jacobian", "cell_density", "compute_directional_flux_csda",
     "csda_energy_quadrature", "direction_vector", "directional_flux_and_grad_csda",
     "flux_and_segment_jacobian", "sample_energy_set", "trace_voxel_paths",
+    "mc_cell_mixture_and_density", "build_cell_properties_for_mc", "compute_directional_flux_mc",
 ]

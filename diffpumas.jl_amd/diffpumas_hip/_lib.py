@@ -142,6 +142,11 @@ class EnergySamples(C.Structure):
     _fields_ = [("n", C.c_int32), ("energy", dbl_p), ("weight", dbl_p), ("charge", dbl_p)]
 
 
+class Mixture(C.Structure):
+    _fields_ = [("n_components", C.c_int32), ("components", C.c_int32 * 4),
+                ("fractions", C.c_double * 4)]
+
+
 # Exported symbols and their argument types.
 _SIGS = {
     "dp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -174,6 +179,11 @@ _SIGS = {
     "dp_mc_flat_backward_events": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),
                                              C.POINTER(McConfig), C.POINTER(McResult), dbl_p]),
     "dp_last_kernel_ms": (C.c_double, [C.c_void_p]),
+    "dp_mc_fixed_paths": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(MaterialConfig),
+                                    C.POINTER(SiteConfig), C.POINTER(DirectionalPaths), C.c_int32,
+                                    C.c_int32, C.POINTER(Mixture), i32_p, dbl_p,
+                                    C.POINTER(EnergySamples), C.c_uint64, C.c_int32, C.c_double,
+                                    dbl_p, dbl_p, dbl_p]),
     "dp_csda_w_flux_jacobian": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(MaterialConfig),
                                           C.POINTER(SiteConfig), C.POINTER(DirectionalPaths),
                                           C.c_int32, dbl_p, C.POINTER(C.c_uint8),

@@ -158,13 +158,15 @@ class DirectionalPaths:
 
 
 def trace_voxel_paths(x_edges, y_edges, z_edges, zeniths, azimuths, surface_distance, exit_z,
-                      ctx=None):
-    """trace_path(::VoxelVolume) (examples/lvd_tomography.jl:738-779) from a
-    detector at the origin for many (zenith, azimuth) bins; surface_distance
-    and exit_z come from the topography (ray_surface_exit_distance)."""
+                      detectors=None, ctx=None):
+    """trace_path(::VoxelVolume) (examples/lvd_tomography.jl:738-779) for many
+    (zenith, azimuth) bins; surface_distance and exit_z come from the
+    topography (ray_surface_exit_distance).  detectors[n, 3] places each bin's
+    detector in the frame of the edges (default: the origin, as the reference's
+    volume frame is centred on the detector)."""
     zen, az = f64(zeniths), f64(azimuths)
     dirs = np.array([direction_vector(z, a) for z, a in zip(zen, az)]).reshape(-1, 3)
-    origins = np.zeros_like(dirs)
+    origins = np.zeros_like(dirs) if detectors is None else f64(detectors).reshape(-1, 3)
     rp, cells, lens, traveled, remaining = trace_edge_paths(x_edges, y_edges, z_edges, origins,
                                                             dirs, surface_distance, ctx=ctx)
     return DirectionalPaths(rp, cells, lens, zen, az, remaining, exit_z,
@@ -237,3 +239,74 @@ def assemble_forward_and_jacobian(physics, shallow_flags, matcfg, site, paths, w
     keep = np.isfinite(jac) & (jac != 0.0)
     J = sp.csr_matrix((jac[keep], (rows[keep], paths.cells[keep])), shape=(paths.n_rays, n_cells))
     return flux, J
+
+
+# ------------------------------------------------------------ fixed-path MC
+def mc_cell_mixture_and_density(w, shallow, matcfg):
+    """mc_cell_mixture_and_density (src/Tomography.jl:826-845):
+    ((materials, fractions), density) of one cell."""
+    if shallow and matcfg.porous_material >= 0 and matcfg.porous_top_thickness > 0.0:
+        pf = rf = 0.5 * (1.0 - w)
+        mix = ((matcfg.porous_material, matcfg.rock_material, matcfg.water_material), (pf, rf, w))
+        return mix, 1.0 / (pf / matcfg.porous_density + rf / matcfg.rock_density +
+                           w / matcfg.water_density)
+    if w <= 1e-12:
+        return ((matcfg.rock_material,), (1.0,)), matcfg.rock_density
+    mix = ((matcfg.rock_material, matcfg.water_material), (1.0 - w, w))
+    return mix, 1.0 / ((1.0 - w) / matcfg.rock_density + w / matcfg.water_density)
+
+
+def build_cell_properties_for_mc(shallow_flags, water_fractions, matcfg):
+    """build_cell_properties_for_mc (src/Tomography.jl:847-861) in indexed form:
+    (media [(materials, fractions)], cell_medium[n_cells], cell_density[n_cells])
+    with one medium per distinct (shallow, w)."""
+    media, index = [], {}
+    cm = np.zeros(len(water_fractions), np.int32)
+    cd = np.zeros(len(water_fractions))
+    for i, (sh, w) in enumerate(zip(shallow_flags, water_fractions)):
+        key = (bool(sh), float(w))
+        if key not in index:
+            mix, rho = mc_cell_mixture_and_density(float(w), bool(sh), matcfg)
+            index[key] = (len(media), rho)
+            media.append(mix)
+        cm[i], cd[i] = index[key]
+    return media, cm, cd
+
+
+def compute_directional_flux_mc(physics, matcfg, site, paths, media, cell_medium, cell_density,
+                                samples, seed, straggling=True, energy_threshold_low=100.0,
+                                scattering=False, return_events=False, ctx=None):
+    """compute_directional_flux_mc (src/Tomography.jl:979-1041) for every path
+    in one launch: (flux[n_rays], sigma[n_rays]).  Event s of path r draws
+    from Philox keyed (seed, r * n_samples + s)."""
+    if scattering:
+        raise ArgumentError("Fixed RayPath MC supports energy-loss fluctuations only; use "
+                            "Geometry for scattered boundary retracing")
+    ctx = ctx or _lib.default_context()
+    h = physics.device_handle(ctx)
+    mix = (_lib.Mixture * len(media))()
+    for j, (mats, fr) in enumerate(media):
+        if len(mats) > 4:
+            raise ArgumentError("at most 4 components per medium")
+        mix[j].n_components = len(mats)
+        for k, (m, f) in enumerate(zip(mats, fr)):
+            mix[j].components[k] = int(m)
+            mix[j].fractions[k] = float(f)
+    cm = i32(cell_medium)
+    cd = f64(cell_density)
+    ne, nw, nq = f64(samples.energy), f64(samples.weight), f64(samples.charge)
+    es = _lib.EnergySamples(len(ne), ptr(ne), ptr(nw), ptr(nq))
+    mc = matcfg.c()
+    sc = _lib.SiteConfig(float(site.detector_elevation), float(site.primary_altitude))
+    cp = paths.c_paths()
+    flux = np.zeros(paths.n_rays)
+    sigma = np.zeros(paths.n_rays)
+    ev = np.zeros(paths.n_rays * len(ne)) if return_events else None
+    _lib.check(ctx.lib.dp_mc_fixed_paths(ctx.h, h, C.byref(mc), C.byref(sc), C.byref(cp), len(cm),
+                                         len(media), mix, ptr(cm, C.c_int32), ptr(cd),
+                                         C.byref(es), int(seed) & (2**64 - 1), int(bool(straggling)),
+                                         float(energy_threshold_low), ptr(flux), ptr(sigma),
+                                         ptr(ev)))
+    if return_events:
+        return flux, sigma, ev.reshape(paths.n_rays, len(ne))
+    return flux, sigma

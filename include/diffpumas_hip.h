@@ -326,6 +326,35 @@ int dp_csda_w_flux_jacobian_dev(dp_ctx ctx, dp_tables tables, const dp_material_
                                 const uint8_t* shallow, const dp_energy_samples* samples,
                                 double* flux, double* jac);
 
+/* ------------------------------------- fixed-path (tomography) backward MC
+ * MaterialMixture (src/Types.jl:357-391): up to 4 components, mass
+ * fractions (normalised by the library).                                   */
+typedef struct {
+  int32_t n_components;
+  int32_t components[4];
+  double fractions[4];
+} dp_mixture;
+
+/* Replaces: compute_directional_flux_mc (src/Tomography.jl:979-1041) for
+ * every path in one persistent launch, with propagate_mc_uniform_segment /
+ * propagate_mc_air_segment (:876-969).  Cell c is transported as
+ * media[cell_medium[c]] at cell_density[c] (build_cell_properties_for_mc,
+ * :826-861, or one rock medium with per-cell densities); the dry remainder
+ * uses mat->rock_material at mat->rock_density and the atmosphere
+ * mat->air_material at the altitude of the event (site).  Samples must carry
+ * charges +/-1.  Event s of path r draws from Philox keyed (seed,
+ * r * samples->n + s).  flux[r], sigma[r] (sigma from (E[w2]-E[w]^2)/(n-1))
+ * are NaN for an invalid path or cos_theta <= 0.  event_weights, if not
+ * NULL, receives the n_rays * samples->n contributions 2 w_s flux_s.
+ * straggling = 0 runs the deterministic CSDA steps.                        */
+int dp_mc_fixed_paths(dp_ctx ctx, dp_tables tables, const dp_material_config* mat,
+                      const dp_site_config* site, const dp_directional_paths* paths,
+                      int32_t n_cells, int32_t n_media, const dp_mixture* media,
+                      const int32_t* cell_medium, const double* cell_density,
+                      const dp_energy_samples* samples, uint64_t seed, int32_t straggling,
+                      double energy_threshold_low, double* flux, double* sigma,
+                      double* event_weights);
+
 /* Last kernel duration in ms, timed with HIP events on the context's stream
  * around the dominant kernel of the previous call (0 if none). */
 double dp_last_kernel_ms(dp_ctx ctx);
