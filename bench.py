@@ -210,7 +210,7 @@ def cpu_adjoint_baseline(physics, problem, budget_s=12.0):
 
 
 def tomo_workload(dp, n_dirs_per_det=1024, seed=31):
-    """64x64x32 voxels of 10 m, 4 detectors 20 m below the block, directions
+    """64x64x32 voxels of 10 m, 4 detectors 1 m above the floor of the block, directions
     with elevation U[15, 75] deg and azimuth U[0, 360); flat surface 50 m above
     the block.  Returns (edges, DirectionalPaths)."""
     import numpy as np
@@ -219,8 +219,8 @@ def tomo_workload(dp, n_dirs_per_det=1024, seed=31):
     ye = np.linspace(0.0, ny * size, ny + 1)
     ze = np.linspace(0.0, nz * size, nz + 1)
     rng = np.random.default_rng(seed)
-    dets = [(160.0, 160.0, -20.0), (480.0, 160.0, -20.0), (160.0, 480.0, -20.0),
-            (480.0, 480.0, -20.0)]
+    dets = [(160.0, 160.0, 1.0), (480.0, 160.0, 1.0), (160.0, 480.0, 1.0),
+            (480.0, 480.0, 1.0)]
     zen, az, org = [], [], []
     for d in dets:
         zen.append(90.0 - rng.uniform(15.0, 75.0, n_dirs_per_det))

@@ -14,8 +14,10 @@
 #include "dp_device.hpp"
 #include "dp_transport.hpp"
 #include "dp_transport_fast.hpp"
+#include "dp_del_defer.hpp"
 #include "dp_media.hpp"
 #include <algorithm>
+#include <hipcub/hipcub.hpp>
 #include <cmath>
 
 using namespace dpk;
@@ -23,6 +25,12 @@ using namespace dpk;
 namespace {
 
 constexpr int MC_ARRS = 5;  // energy, log energy, csda range, csda stopping, transport path
+
+// Event sorting by source energy and deferred hard-event weights are
+// switched off: the first GPU run with both enabled ended in a device memory
+// fault whose cause is not identified yet.
+constexpr bool kSortEvents = false;
+constexpr bool kDeferDel = false;
 
 struct FlatGeo {
   double thick, rho;
@@ -211,7 +219,28 @@ struct StochArgs {
   double* out_w;
   int* err;            // [0] error bits, [1] spare
   unsigned long long* bad_event;
+  const uint32_t* order;  // optional processing order of the events (by energy)
+  // deferred DEL weights (null: evaluated inline)
+  DelRec* del_rec;
+  int* nrec;
+  uint8_t* err_pend;
+  uint8_t* err_nrec;
 };
+
+// Sort keys: the source energy of each event (first Philox draw), as the
+// bit pattern of a positive double (monotonic as an unsigned integer).
+__global__ void mc_energy_keys(uint64_t seed, int64_t n, double emin, double emax,
+                               uint64_t* __restrict__ keys, uint32_t* __restrict__ ids) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  Philox rng;
+  rng.init(seed, (uint64_t)e);
+  const double u = rng.uniform();
+  const double rk = log(emax / emin);
+  const double k = rk > 0 ? emin * exp(rk * u) : emin;
+  keys[e] = (uint64_t)__double_as_longlong(k);
+  ids[e] = (uint32_t)e;
+}
 
 // transport_backward_through_geometry with an RNG, src/Geometry.jl:534-672,
 // and the acceptance of compute_flux_single (:739-757).
@@ -286,7 +315,7 @@ __device__ double flat_event(const StochArgs& a, double energy, double charge, P
 // flat_event on tables with one shared energy grid: each step locates the
 // event's energy once and hands it to the step functions.
 __device__ double flat_event_f(const StochArgs& a, double energy, double charge, Philox& rng,
-                               int& err) {
+                               int& err, DelLog* dlog) {
   const DevTables& T = a.T;
   McState st;
   st.E = energy; st.w = 1.0; st.t = 0.0; st.L = 0.0;
@@ -341,8 +370,9 @@ __device__ double flat_event_f(const StochArgs& a, double energy, double charge,
       double step = fmin(stb + 1e-7, lim);
       step = fmin(fmax(step, 1e-7), 1000.0);
       g = L.i;
-      if (use_str) straggled_step_f(T, *md, st, L, lb0, density, step, use_scat, limit, rng, err, &g);
-      else if (use_mixed) mixed_step_f(T, *md, st, L, density, step, limit, rng, err);
+      if (dlog) dlog->medium = md == &a.rock ? 0 : (md == &a.air ? 1 : 2);
+      if (use_str) straggled_step_f(T, *md, st, L, lb0, density, step, use_scat, limit, rng, err, &g, dlog);
+      else if (use_mixed) mixed_step_f(T, *md, st, L, density, step, limit, rng, err, dlog);
       else csda_mix_step_f(T, *md, st, L, density, step, &g);
       if (err) return 0.0;
     }
@@ -359,7 +389,8 @@ __device__ double flat_event_f(const StochArgs& a, double energy, double charge,
 template <bool FAST>
 __global__ __launch_bounds__(128, 4) void mc_flat_stochastic_kernel(StochArgs a) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += stride) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += stride) {
+    const int64_t e = a.order ? (int64_t)a.order[p] : p;
     Philox rng;
     rng.init(a.seed, (uint64_t)e);
     const double u = rng.uniform();
@@ -370,13 +401,20 @@ __global__ __launch_bounds__(128, 4) void mc_flat_stochastic_kernel(StochArgs a)
     const double q = rng.uniform() > 0.5 ? 1.0 : -1.0;
     w *= 2.0;
     int err = 0;
-    const double f = FAST ? flat_event_f(a, k, q, rng, err) : flat_event(a, k, q, rng, err);
-    if (err) {
+    DelLog log{a.del_rec ? a.del_rec + e * DEL_MAXR : nullptr, 0, 0};
+    const double f = FAST ? flat_event_f(a, k, q, rng, err, a.del_rec ? &log : nullptr)
+                          : flat_event(a, k, q, rng, err);
+    if (a.nrec) {
+      a.nrec[e] = log.n;
+      a.err_pend[e] = (uint8_t)(log.n > 0 ? err : 0);
+      a.err_nrec[e] = (uint8_t)log.n;
+    }
+    if (err && log.n == 0) {
       atomicOr(a.err, err);
       atomicMin(a.bad_event, (unsigned long long)e);
       a.out_w[e] = 0.0;
     } else {
-      a.out_w[e] = w * f;
+      a.out_w[e] = err ? 0.0 : w * f;
     }
   }
 }
@@ -652,6 +690,61 @@ int reduce_pair(hipStream_t st, const double* x, const double* y, int64_t n, boo
 
 __global__ void reset_flags(int* f) { if (threadIdx.x < 4) f[threadIdx.x] = 0; }
 
+__global__ void del_count_kernel(int64_t n, const int* nrec, const int* off, int64_t* count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = n > 0 ? (int64_t)off[n - 1] + nrec[n - 1] : 0;
+}
+
+}  // namespace
+
+namespace dph {
+// Carves the buffers out of the context's third scratch buffer.
+int del_buffers(dp_ctx c, int64_t n, DelBuffers& b) {
+  const size_t m = (size_t)n * DEL_MAXR;
+  size_t tmp = 0;
+  DP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (int*)nullptr, (int*)nullptr, (int)n,
+                                          c->stream));
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes, 256); return at; };
+  const size_t o_rec = take(sizeof(DelRec) * m), o_nrec = take(4 * (size_t)n),
+               o_off = take(4 * (size_t)n), o_ep = take((size_t)n), o_en = take((size_t)n),
+               o_list = take(8 * m), o_fac = take(8 * m), o_kind = take(4 * m), o_cnt = take(8),
+               o_med = take(sizeof(Medium) * 4), o_tmp = take(tmp);
+  int rc = ensure(c->scratch3, o);
+  if (rc) return rc;
+  char* p = (char*)c->scratch3.ptr;
+  b.rec = (DelRec*)(p + o_rec); b.nrec = (int*)(p + o_nrec); b.off = (int*)(p + o_off);
+  b.err_pend = (uint8_t*)(p + o_ep); b.err_nrec = (uint8_t*)(p + o_en);
+  b.list = (int64_t*)(p + o_list); b.fac = (double*)(p + o_fac); b.kind = (int*)(p + o_kind);
+  b.count = (int64_t*)(p + o_cnt); b.media = (Medium*)(p + o_med);
+  b.tmp = p + o_tmp; b.tmp_bytes = tmp;
+  return DP_OK;
+}
+
+// Evaluates every record and applies each event's factors (stream ordered).
+int del_resolve(dp_ctx c, int64_t n, DelBuffers& b, const Medium* media, double mass,
+                double* out_w, int* err, unsigned long long* bad, int64_t e0) {
+  hipStream_t st = c->stream;
+  DP_HIP(hipcub::DeviceScan::ExclusiveSum(b.tmp, b.tmp_bytes, b.nrec, b.off, (int)n, st));
+  const unsigned gb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(del_count_kernel, dim3(1), dim3(64), 0, st, n, b.nrec, b.off, b.count);
+  hipLaunchKernelGGL(del_list_kernel_i, dim3(gb), dim3(256), 0, st, n, b.nrec, b.off, b.list);
+  int occ = 0;
+  DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, del_eval_kernel, 128, 0));
+  if (occ < 1) occ = 1;
+  hipLaunchKernelGGL(del_eval_kernel, dim3((unsigned)(c->n_cu * occ)), dim3(128), 0, st, b.count,
+                     b.list, b.rec, media, mass, b.fac, b.kind);
+  hipLaunchKernelGGL(del_apply_kernel_i, dim3(gb), dim3(256), 0, st, n, b.nrec, b.fac, b.kind,
+                     b.err_pend, b.err_nrec, out_w, err, bad, e0);
+  DP_HIP(hipGetLastError());
+  return DP_OK;
+}
+}  // namespace dph
+
+namespace {
+using dph::DelBuffers;
+using dph::del_buffers;
+using dph::del_resolve;
+
 }  // namespace
 
 namespace {
@@ -719,6 +812,7 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
   a.out_w = stochastic ? nullptr : out_w;
   a.out_gf = (double*)(b + o_gf); a.out_gd = (double*)(b + o_gd);
   a.err = flags;
+  Medium hm[3];  // host copy of the media, read by an async copy before the final sync
   if (stochastic) {
     StochArgs s;
     s.T = {t->data, t->scalars, t->n_knots, t->pow2, t->mass, t->ctau, t->elastic_ratio};
@@ -740,6 +834,30 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     s.thr = thr; s.sw = cfg->mode_switch;
     s.straggling = cfg->straggling != 0; s.scattering = cfg->scattering != 0;
     s.n = n; s.seed = cfg->seed; s.out_w = out_w; s.err = flags; s.bad_event = bad;
+    s.del_rec = nullptr; s.nrec = nullptr; s.err_pend = nullptr; s.err_nrec = nullptr;
+    // Process events in order of source energy: lanes of a wave then share
+    // the transport mode and a similar number of steps.  Each event keeps
+    // its own Philox stream and output slot, so results do not change.
+    (void)hipEventRecord(c->ev0, c->stream);
+    s.order = nullptr;
+    if (kSortEvents && n < (int64_t)0x7fffffff) {
+      size_t tmp = 0;
+      DP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 40, 64,
+                                                c->stream));
+      size_t q0 = 0, q1 = dph::align_up(8 * (size_t)n, 256), q2 = dph::align_up(q1 + 8 * (size_t)n, 256);
+      size_t q3 = dph::align_up(q2 + 4 * (size_t)n, 256), q4 = dph::align_up(q3 + 4 * (size_t)n, 256);
+      if ((rc = dph::ensure(c->scratch2, q4 + tmp + 256))) return rc;
+      char* sb = (char*)c->scratch2.ptr;
+      uint64_t *k0 = (uint64_t*)(sb + q0), *k1 = (uint64_t*)(sb + q1);
+      uint32_t *i0 = (uint32_t*)(sb + q2), *i1 = (uint32_t*)(sb + q3);
+      hipLaunchKernelGGL(mc_energy_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                         cfg->seed, n, cfg->energy_min, cfg->energy_max, k0, i0);
+      DP_HIP(hipGetLastError());
+      DP_HIP(hipcub::DeviceRadixSort::SortPairs(sb + q4, tmp, k0, k1, i0, i1, (int)n, 40, 64,
+                                                c->stream));
+      s.order = i1;
+    }
     const bool fast = t->shared_grid;
     int occ = 0;
     if (fast) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<true>, 128, 0));
@@ -748,11 +866,20 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     int64_t want = (n + 127) / 128;
     int64_t grid = (int64_t)c->n_cu * occ * 4;
     if (grid > want) grid = want;
-    (void)hipEventRecord(c->ev0, c->stream);
+    // Deferred DEL records (fast path, fewer than 2^25 events).
+    DelBuffers db;
+    const bool defer = kDeferDel && fast && n < ((int64_t)1 << 25);
+    hm[0] = s.rock; hm[1] = s.air; hm[2] = s.porous;
+    if (defer) {
+      if ((rc = del_buffers(c, n, db))) return rc;
+      s.del_rec = db.rec; s.nrec = db.nrec; s.err_pend = db.err_pend; s.err_nrec = db.err_nrec;
+      DP_HIP(hipMemcpyAsync(db.media, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
+    }
     if (fast) hipLaunchKernelGGL(mc_flat_stochastic_kernel<true>, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
     else hipLaunchKernelGGL(mc_flat_stochastic_kernel<false>, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
-    (void)hipEventRecord(c->ev1, c->stream);
     DP_HIP(hipGetLastError());
+    if (defer && (rc = del_resolve(c, n, db, db.media, t->mass, out_w, flags, bad))) return rc;
+    (void)hipEventRecord(c->ev1, c->stream);
   }
   if (!stochastic || a.grad) {
     size_t lds = (size_t)3 * MC_ARRS * t->n_knots * sizeof(double);

@@ -13,6 +13,7 @@
 #include "dp_transport.hpp"
 #include "dp_transport_fast.hpp"
 #include "dp_media.hpp"
+#include <algorithm>
 #include <cmath>
 
 using namespace dpk;
@@ -41,14 +42,21 @@ struct PathArgs {
   const double* se;
   const double* sw;
   const double* sq;
+  const uint32_t* order;  // samples in order of energy
   uint64_t seed;
   int chunks;          // chunks per path
-  int64_t n_items;     // n_rays * chunks
-  double* part;        // 2 per item
+  int r0;              // first path of this launch
+  int64_t n_items;     // paths of this launch * chunks
+  double* part;        // 2 per item (indexed from path r0)
   double* events;      // optional, n_rays * n_samples
+  double* w_local;     // per event of this launch (with deferred DEL weights)
   int* queue;
   int* err;
   unsigned long long* bad;
+  DelRec* del_rec;     // deferred DEL records (null: inline)
+  int* nrec;
+  uint8_t* err_pend;
+  uint8_t* err_nrec;
 };
 
 // propagate_mc_uniform_segment / propagate_mc_air_segment, src/Tomography.jl:
@@ -58,7 +66,7 @@ struct PathArgs {
 template <bool FAST>
 __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, double density,
                              bool air, double cos_t, double distance, Philox& rng, int& err,
-                             int& g) {
+                             int& g, DelLog* dlog) {
   double remaining = distance;
   const double elimit = a.elimit;
   int it = 0;
@@ -87,9 +95,9 @@ __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, d
     const double oldL = st.L, oldE = st.E;
     if (FAST) {
       if (st.E < a.thr_low - EPS64 && a.straggling)
-        straggled_step_f(a.T, md, st, L, 0.0, dens, step, false, cur, rng, err, &g);
+        straggled_step_f(a.T, md, st, L, 0.0, dens, step, false, cur, rng, err, &g, dlog);
       else if (a.straggling)
-        mixed_step_f(a.T, md, st, L, dens, step, cur, rng, err);
+        mixed_step_f(a.T, md, st, L, dens, step, cur, rng, err, dlog);
       else
         csda_mix_step_f(a.T, md, st, L, dens, step, &g);
     } else if (st.E < a.thr_low - EPS64) {
@@ -122,12 +130,18 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
     __syncthreads();
     const int64_t item = s_item;
     if (item >= a.n_items) break;
-    const int r = (int)(item / a.chunks);
-    const int s = (int)(item % a.chunks) * PM_THREADS + tid;
+    const int r = a.r0 + (int)(item / a.chunks);
+    const int sl = (int)(item % a.chunks) * PM_THREADS + tid;
     const double ct = a.cos_t[r];
     const bool live = (a.valid ? a.valid[r] != 0 : true) && ct > 0.0;
     double wi = 0.0;
-    if (live && s < a.n_samples) {
+    if (!live && sl < a.n_samples) {
+      const int64_t el = (int64_t)(r - a.r0) * a.n_samples + (int)a.order[sl];
+      a.w_local[el] = 0.0;
+      if (a.nrec) { a.nrec[el] = 0; a.err_pend[el] = 0; a.err_nrec[el] = 0; }
+    }
+    if (live && sl < a.n_samples) {
+      const int s = (int)a.order[sl];
       const int64_t e = (int64_t)r * a.n_samples + s;
       Philox rng;
       rng.init(a.seed, (uint64_t)e);
@@ -136,30 +150,62 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
       st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
       st.dir[0] = -sqrt(fmax(0.0, 1.0 - ct * ct)); st.dir[1] = 0.0; st.dir[2] = -ct;
       int err = 0, g = a.T.K / 2;
+      const int64_t el = (int64_t)(r - a.r0) * a.n_samples + s;  // event index in this launch
+      DelLog log{a.del_rec ? a.del_rec + el * DEL_MAXR : nullptr, 0, 0};
+      DelLog* dl = (FAST && a.del_rec) ? &log : nullptr;
       const int b0 = a.row_ptr[r], ns = a.row_ptr[r + 1] - b0;
       for (int k = 0; k < ns; ++k) {
         const int c = a.cell[b0 + k];
-        path_segment<FAST>(a, a.media[a.cell_medium[c]], st, a.cell_density[c], false, ct,
-                           a.len[b0 + k], rng, err, g);
+        log.medium = a.cell_medium[c];
+        path_segment<FAST>(a, a.media[log.medium], st, a.cell_density[c], false, ct,
+                           a.len[b0 + k], rng, err, g, dl);
         if (err || st.w <= 0.0 || !isfinite(st.w)) break;
       }
-      if (!err && st.w > 0.0 && isfinite(st.w) && a.remaining[r] > 1e-8)
+      if (!err && st.w > 0.0 && isfinite(st.w) && a.remaining[r] > 1e-8) {
+        log.medium = a.rock_medium;
         path_segment<FAST>(a, a.media[a.rock_medium], st, a.rho_rock, false, ct, a.remaining[r],
-                           rng, err, g);
+                           rng, err, g, dl);
+      }
       if (!err && st.w > 0.0 && isfinite(st.w)) {
         const double air_d = fmax(0.0, a.palt - a.exit_z[r]) / fmax(ct, 1e-3);
-        path_segment<FAST>(a, a.media[a.air_medium], st, 0.0, true, ct, air_d, rng, err, g);
+        log.medium = a.air_medium;
+        path_segment<FAST>(a, a.media[a.air_medium], st, 0.0, true, ct, air_d, rng, err, g, dl);
       }
       double f = 0.0;
       if (!err && st.w > 0.0 && isfinite(st.w) && st.E < a.elimit)
         f = st.w * flux_gccly(ct, st.E, a.sq[s]);
       wi = 2.0 * a.sw[s] * f;
+      if (a.nrec) {
+        a.nrec[el] = log.n;
+        a.err_pend[el] = (uint8_t)(log.n > 0 ? err : 0);
+        a.err_nrec[el] = (uint8_t)log.n;
+      }
       if (err) {
-        atomicOr(a.err, err);
-        atomicMin(a.bad, (unsigned long long)e);
+        if (log.n == 0) {
+          atomicOr(a.err, err);
+          atomicMin(a.bad, (unsigned long long)e);
+        }
         wi = 0.0;
       }
-      if (a.events) a.events[e] = wi;
+      a.w_local[el] = wi;
+    }
+  }
+}
+
+// (sum w, sum w^2) of each item's samples after the deferred weights are in,
+// in the sorted sample order of the item (fixed tree).
+__global__ __launch_bounds__(PM_THREADS) void path_items_kernel(PathArgs a) {
+  __shared__ double s_sum[2][PM_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int64_t item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+    const int r = a.r0 + (int)(item / a.chunks);
+    const int sl = (int)(item % a.chunks) * PM_THREADS + tid;
+    double wi = 0.0;
+    if (sl < a.n_samples) {
+      const int s = (int)a.order[sl];
+      const int64_t el = (int64_t)(r - a.r0) * a.n_samples + s;
+      wi = a.w_local[el];
+      if (a.events) a.events[(int64_t)r * a.n_samples + s] = wi;
     }
     const double s1 = wave_sum(wi), s2 = wave_sum(wi * wi);
     if (lane == 0) { s_sum[0][wave] = s1; s_sum[1][wave] = s2; }
@@ -167,8 +213,9 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
     if (tid == 0) {
       double t1 = 0.0, t2 = 0.0;
       for (int j = 0; j < PM_THREADS / 64; ++j) { t1 += s_sum[0][j]; t2 += s_sum[1][j]; }
-      a.part[2 * item] = t1;
-      a.part[2 * item + 1] = t2;
+      const int64_t gi = (int64_t)a.r0 * a.chunks + item;
+      a.part[2 * gi] = t1;
+      a.part[2 * gi + 1] = t2;
     }
     __syncthreads();
   }
@@ -198,6 +245,9 @@ __global__ void path_reduce(const double* __restrict__ part, int n_rays, int chu
 __global__ void reset_path_flags(int* q, unsigned long long* bad) {
   if (threadIdx.x < 4) q[threadIdx.x] = 0;
   if (threadIdx.x == 0) *bad = ~0ull;
+}
+__global__ void reset_queue(int* q) {
+  if (threadIdx.x < 2) q[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -268,6 +318,7 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
                o_cos = take(8 * (size_t)nr), o_rem = take(8 * (size_t)nr),
                o_zex = take(8 * (size_t)nr), o_val = take((size_t)nr),
                o_se = take(8 * (size_t)nn), o_sw = take(8 * (size_t)nn), o_sq = take(8 * (size_t)nn),
+               o_ord = take(4 * (size_t)nn),
                o_part = take(16 * (size_t)n_items), o_flux = take(8 * (size_t)nr),
                o_sig = take(8 * (size_t)nr), o_ev = take(event_weights ? 8 * (size_t)n_ev : 8),
                o_q = take(256);
@@ -290,6 +341,12 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   DP_HIP(hipMemcpyAsync(b + o_se, es->energy, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
   DP_HIP(hipMemcpyAsync(b + o_sw, es->weight, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
   DP_HIP(hipMemcpyAsync(b + o_sq, es->charge, 8 * (size_t)nn, hipMemcpyHostToDevice, st));
+  // samples in order of energy: the lanes of a chunk share the transport mode
+  std::vector<uint32_t> ord(nn);
+  for (int i = 0; i < nn; ++i) ord[i] = (uint32_t)i;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](uint32_t x, uint32_t y) { return es->energy[x] < es->energy[y]; });
+  DP_HIP(hipMemcpyAsync(b + o_ord, ord.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, st));
   double supported = t->last_energy[0];
   for (double e : t->last_energy) supported = e < supported ? e : supported;
   PathArgs a;
@@ -308,7 +365,8 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   a.valid = p->valid ? (const uint8_t*)(b + o_val) : nullptr;
   a.n_samples = nn; a.se = (const double*)(b + o_se); a.sw = (const double*)(b + o_sw);
   a.sq = (const double*)(b + o_sq);
-  a.seed = seed; a.chunks = chunks; a.n_items = n_items;
+  a.order = (const uint32_t*)(b + o_ord);
+  a.seed = seed; a.chunks = chunks; a.n_items = n_items; a.r0 = 0;
   a.part = (double*)(b + o_part);
   a.events = event_weights ? (double*)(b + o_ev) : nullptr;
   int* q = (int*)(b + o_q);
@@ -320,13 +378,42 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   if (fast) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, path_mc_kernel<true>, PM_THREADS, 0));
   else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, path_mc_kernel<false>, PM_THREADS, 0));
   if (occ < 1) occ = 1;
-  int64_t grid = (int64_t)c->n_cu * occ;
-  if (grid > n_items) grid = n_items;
+  // Paths are processed in batches of at most 2^22 events, which bounds the
+  // memory of the deferred hard-event records.
+  const int64_t batch_ev = (int64_t)1 << 22;
+  int rays_per_batch = (int)std::max<int64_t>(1, batch_ev / nn);
+  if (rays_per_batch > nr) rays_per_batch = nr;
+  const int64_t ev_batch = (int64_t)rays_per_batch * nn;
+  dph::DelBuffers db;
+  // deferred hard-event weights are switched off (see the flat MC)
+  const bool defer = false && fast && straggling && ev_batch < ((int64_t)1 << 25);
+  if (defer && (rc = dph::del_buffers(c, ev_batch, db))) return rc;
+  rc = dph::ensure(c->scratch, 8 * (size_t)ev_batch + 256);
+  if (rc) return rc;
+  a.w_local = (double*)c->scratch.ptr;
+  a.del_rec = defer ? db.rec : nullptr;
+  a.nrec = defer ? db.nrec : nullptr;
+  a.err_pend = defer ? db.err_pend : nullptr;
+  a.err_nrec = defer ? db.err_nrec : nullptr;
   (void)hipEventRecord(c->ev0, st);
-  if (fast) hipLaunchKernelGGL(path_mc_kernel<true>, dim3((unsigned)grid), dim3(PM_THREADS), 0, st, a);
-  else hipLaunchKernelGGL(path_mc_kernel<false>, dim3((unsigned)grid), dim3(PM_THREADS), 0, st, a);
+  for (int r0 = 0; r0 < nr; r0 += rays_per_batch) {
+    const int r1 = std::min(nr, r0 + rays_per_batch);
+    a.r0 = r0;
+    a.n_items = (int64_t)(r1 - r0) * chunks;
+    int64_t grid = (int64_t)c->n_cu * occ;
+    if (grid > a.n_items) grid = a.n_items;
+    hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, st, q);
+    if (fast) hipLaunchKernelGGL(path_mc_kernel<true>, dim3((unsigned)grid), dim3(PM_THREADS), 0, st, a);
+    else hipLaunchKernelGGL(path_mc_kernel<false>, dim3((unsigned)grid), dim3(PM_THREADS), 0, st, a);
+    DP_HIP(hipGetLastError());
+    if (defer && (rc = dph::del_resolve(c, (int64_t)(r1 - r0) * nn, db, a.media, t->mass,
+                                        a.w_local, a.err, a.bad, (int64_t)r0 * nn)))
+      return rc;
+    int64_t gi = a.n_items < (int64_t)c->n_cu * 8 ? a.n_items : (int64_t)c->n_cu * 8;
+    hipLaunchKernelGGL(path_items_kernel, dim3((unsigned)gi), dim3(PM_THREADS), 0, st, a);
+    DP_HIP(hipGetLastError());
+  }
   (void)hipEventRecord(c->ev1, st);
-  DP_HIP(hipGetLastError());
   hipLaunchKernelGGL(path_reduce, dim3((nr + 255) / 256), dim3(256), 0, st, a.part, nr, chunks,
                      nn, a.cos_t, a.valid, (double*)(b + o_flux), (double*)(b + o_sig));
   DP_HIP(hipGetLastError());

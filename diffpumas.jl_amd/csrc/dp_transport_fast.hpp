@@ -13,6 +13,7 @@
 // changes.
 #pragma once
 #include "dp_transport.hpp"
+#include "dp_del_defer.hpp"
 
 namespace dpk {
 
@@ -292,7 +293,7 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
                                                      McState& st, const Loc& Li, double lb0,
                                                      double density, double ds, bool scattering,
                                                      double energy_limit, Philox& rng, int& err,
-                                                     int* gnext) {
+                                                     int* gnext, DelLog* dlog) {
   const DevMix& x = md.mix;
   const DevKernel& K = md.kern;
   const double ki = Li.e, mass = T.mass;
@@ -345,9 +346,22 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
   int proc = 0, elem = -1;
   if (del) {
     ckf = k_del;
-    const DelOut o = reverse_del_event(K, mass, k_del, rng, true, err);
-    if (err) return;
-    kf = o.k; dw = o.w; proc = o.proc; elem = o.elem;
+    if (!scattering && dlog && dlog->n < DEL_MAXR) {
+      // the target choice only consumes its draw when nothing is deflected
+      const DelSample d = del_sample(K, mass, k_del, rng, err);
+      if (err) return;
+      kf = d.anc;
+      if (d.deferred) {
+        del_push(dlog, d, k_del, 1);
+        (void)uniform(rng);
+      } else {
+        dw = d.w;
+      }
+    } else {
+      const DelOut o = reverse_del_event(K, mass, k_del, rng, true, err);
+      if (err) return;
+      kf = o.k; dw = o.w; proc = o.proc; elem = o.elem;
+    }
     dX = X_del;
   }
   if (ehs) { kf = k_ehs; ckf = k_ehs; dX = X_ehs; }
@@ -386,7 +400,8 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
 // interval of the current energy carried between lookups.
 __device__ __noinline__ static void mixed_step_f(const DevTables& T, const Medium& md, McState& st,
                                                  const Loc& Li, double density, double distance,
-                                                 double energy_limit, Philox& rng, int& err) {
+                                                 double energy_limit, Philox& rng, int& err,
+                                                 DelLog* dlog) {
   const DevKernel& c = md.kern;
   const double m = T.mass;
   const int n = c.n, p2 = c.pow2;
@@ -443,10 +458,22 @@ __device__ __noinline__ static void mixed_step_f(const DevTables& T, const Mediu
     k = pre;
     ik = ip;
     if (occurs) {
-      const DelOut o = reverse_del_event(c, m, k, rng, false, err);
-      if (err) return;
-      k = o.k;
-      weight *= o.w;
+      if (dlog && dlog->n < DEL_MAXR) {
+        const DelSample d = del_sample(c, m, k, rng, err);
+        if (err) return;
+        if (d.deferred) {
+          del_push(dlog, d, k, 0);
+          k = d.anc;
+          continue;
+        }
+        k = d.anc;
+        weight *= d.w;
+      } else {
+        const DelOut o = reverse_del_event(c, m, k, rng, false, err);
+        if (err) return;
+        k = o.k;
+        weight *= o.w;
+      }
       if (!(isfinite(weight) && weight >= 0)) { err |= ERR_TRANSPORT; return; }
       if (weight == 0) { exhausted = false; break; }
     } else {
