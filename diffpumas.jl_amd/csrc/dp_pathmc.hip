@@ -123,8 +123,7 @@ __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, d
 template <bool FAST>
 __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
   __shared__ int64_t s_item;
-  __shared__ double s_sum[2][PM_THREADS / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x;
   for (;;) {
     if (tid == 0) s_item = atomicAdd((unsigned long long*)a.queue, 1ull);
     __syncthreads();
@@ -189,6 +188,7 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
       }
       a.w_local[el] = wi;
     }
+    __syncthreads();  // every thread has read s_item before thread 0 takes the next item
   }
 }
 
