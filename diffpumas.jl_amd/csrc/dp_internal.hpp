@@ -72,6 +72,8 @@ struct dp_ctx_s {
   dph::DevBuf scratch3;      // third scratch (deferred hard-event records)
   dph::DevBuf counter;       // small: work-queue counters
   dph::DevBuf staging;       // host-API staging of inputs/outputs
+  dp_csda_correction corr{};            // w-model corrections (off)
+  dp_csda_material_correction mcorr{};
 };
 
 namespace dph {

@@ -18,6 +18,15 @@
 // term, which is the cotangent algebra of the rrules written for a
 // multiplicative weight.  Per-segment sums over samples use a fixed wave/LDS
 // tree, so the Jacobian is deterministic and independent of launch shape.
+//
+// With the MC-calibrated corrections on (CORR), each sample also accumulates
+// V = sum Omega(E_mid) dX and H = sum (S_csda - S_mixed)(E_mid) dX over the
+// rock column (_directional_flux_core, src/Tomography.jl:585-637); the tape
+// gains the total derivatives of each segment's V and H terms in (w, E_in),
+// and the reverse sweep adds A_V dV + A_H dH, A_V and A_H the derivatives of
+// log csda_var_correction (:384-391) in V and H.  The geometric and material
+// multipliers (:394-412, :560-580) are per bin; the material one adds
+// M'(q) dq/dw_s times the bin's sum.
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
 #include <cstring>
@@ -42,6 +51,7 @@ struct WTab {
   const double *wE, *wLE, *wS;  // water
   const double *pE, *pLE, *pS;  // porous (aliases rock when unused)
   const double *aE, *aLE, *aS;  // air
+  const double *rM, *rO, *wM, *wO, *pM, *pO;  // MIXED stopping power, straggling (CORR)
   int K, pow2, shared;          // shared: rock/water/porous energy grids identical
   double mass, ctau;
 };
@@ -89,6 +99,81 @@ template <class R>
 __device__ __forceinline__ R cell_density(const WMat& m, R w, bool porous) {
   if (porous) return 1.0 / (0.5 * (1.0 - w) / m.rho_p + 0.5 * (1.0 - w) / m.rho_r + w / m.rho_w);
   return 1.0 / ((1.0 - w) / m.rho_r + w / m.rho_w);
+}
+
+// cell_straggling and cell_hardloss (src/Tomography.jl:329-352) at one
+// energy: om = Omega(w), hl = sum of fractions x (S_csda - S_mixed).
+template <class R>
+__device__ __forceinline__ void cell_strag_hard(const WTab& T, R w, bool porous, R e, R& om,
+                                                R& hl) {
+  const int K = T.K;
+  R le = log(e);
+  R orr, ow, op, hr, hw, hp;
+  if (T.shared) {
+    if (val(e) <= T.rE[0] || val(e) >= T.rE[K - 1]) {
+      const int j = val(e) <= T.rE[0] ? 0 : K - 1;
+      orr = R(T.rO[j]); ow = R(T.wO[j]); op = R(T.pO[j]);
+      hr = R(T.rS[j]) - R(T.rM[j]); hw = R(T.wS[j]) - R(T.wM[j]); hp = R(T.pS[j]) - R(T.pM[j]);
+    } else {
+      const int i = bracket(T.rLE, K, T.pow2, val(le));
+      R t = (le - T.rLE[i]) / (T.rLE[i + 1] - T.rLE[i]);
+      orr = lerp_at(T.rO, i, t); ow = lerp_at(T.wO, i, t);
+      hr = lerp_at(T.rS, i, t) - lerp_at(T.rM, i, t);
+      hw = lerp_at(T.wS, i, t) - lerp_at(T.wM, i, t);
+      if (porous) {
+        op = lerp_at(T.pO, i, t);
+        hp = lerp_at(T.pS, i, t) - lerp_at(T.pM, i, t);
+      } else {
+        op = R(0.0); hp = R(0.0);
+      }
+    }
+  } else {
+    orr = interp_r(T.rE, T.rLE, T.rO, K, T.pow2, e, le);
+    ow = interp_r(T.wE, T.wLE, T.wO, K, T.pow2, e, le);
+    hr = interp_r(T.rE, T.rLE, T.rS, K, T.pow2, e, le) - interp_r(T.rE, T.rLE, T.rM, K, T.pow2, e, le);
+    hw = interp_r(T.wE, T.wLE, T.wS, K, T.pow2, e, le) - interp_r(T.wE, T.wLE, T.wM, K, T.pow2, e, le);
+    if (porous) {
+      op = interp_r(T.pE, T.pLE, T.pO, K, T.pow2, e, le);
+      hp = interp_r(T.pE, T.pLE, T.pS, K, T.pow2, e, le) - interp_r(T.pE, T.pLE, T.pM, K, T.pow2, e, le);
+    } else {
+      op = R(0.0); hp = R(0.0);
+    }
+  }
+  if (porous) {
+    om = 0.5 * (1.0 - w) * op + 0.5 * (1.0 - w) * orr + w * ow;
+    hl = 0.5 * (1.0 - w) * hp + 0.5 * (1.0 - w) * hr + w * hw;
+  } else {
+    om = (1.0 - w) * orr + w * ow;
+    hl = (1.0 - w) * hr + w * hw;
+  }
+}
+
+// property_straggling and rock_hardloss (src/Tomography.jl:318-321) of the
+// dry-rock remainder.
+template <class R>
+__device__ __forceinline__ void rock_strag_hard(const WTab& T, R e, R& om, R& hl) {
+  R le = log(e);
+  om = interp_r(T.rE, T.rLE, T.rO, T.K, T.pow2, e, le);
+  hl = interp_r(T.rE, T.rLE, T.rS, T.K, T.pow2, e, le) - interp_r(T.rE, T.rLE, T.rM, T.K, T.pow2, e, le);
+}
+
+// CsdaCorrection / CsdaMaterialCorrection coefficients.
+struct WCorr {
+  int enabled, geom, mat;  // geom: some residual coefficient is non-zero
+  double ks, kh, ra, rb, rc, rd, lin, quad;
+};
+
+// log csda_var_correction (src/Tomography.jl:384-391) with the spectral
+// index of local_spectral_index (:367-372) at charge +1.
+template <class R>
+__device__ __forceinline__ R log_var_corr(R Es, double V, double H, double ct, const WCorr& C) {
+  R e = cmax(Es, 1.0);
+  const double eta = 1.0e-2;
+  R fp = flux_gccly_r(ct, e * (1.0 + eta), 1.0);
+  R fm = flux_gccly_r(ct, e * (1.0 - eta), 1.0);
+  R b = -(log(fp) - log(fm)) / (2 * eta);
+  R cs = 0.5 * b * (b + 1.0);
+  return C.ks * cs * V / (e * e) - C.kh * (H >= 0.0 ? H : 0.0) / e;
 }
 
 // compute_decay_weight_from_path, src/Geometry.jl:1083-1107 (the beta floor
@@ -206,6 +291,7 @@ struct WArgs {
   int K, pow2, shared;
   int rock, water, air, porous;
   WMat mat;
+  WCorr corr;
   double det_elev, palt, mass, ctau;
   int n_rays, max_seg, n_cells;
   const int32_t* row_ptr;
@@ -228,16 +314,24 @@ struct WArgs {
   int* err;
 };
 
-template <bool JAC>
+// Doubles of LDS table staging: (E, log E, S_csda) for rock, water, porous,
+// air, plus (S_mixed, Omega) for rock, water, porous with the corrections.
+__host__ __device__ constexpr int w_tab_cols(bool corr) { return corr ? 18 : 12; }
+// Tape doubles per segment and lane.
+__host__ __device__ constexpr int w_tape(bool corr) { return corr ? 8 : 4; }
+
+template <bool JAC, bool CORR>
 __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
   extern __shared__ double lds[];
   const int K = a.K;
   const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nwv = bd >> 6;
-  double* wsum = lds + 12 * K;                            // [nwv][max_seg]
+  constexpr int NT = w_tape(CORR);
+  double* wsum = lds + w_tab_cols(CORR) * K;              // [nwv][max_seg]
   double* wflux = wsum + (size_t)nwv * a.max_seg;         // [nwv]
   int* scell = (int*)(wflux + nwv);                       // [max_seg]
   __shared__ int s_ray;
+  __shared__ double s_G, s_M, s_q, s_mass;                // per-bin multipliers (CORR)
   {
     const int mats[4] = {a.rock, a.water, a.porous >= 0 ? a.porous : a.rock, a.air};
     for (int j = 0; j < 4; ++j) {
@@ -249,15 +343,29 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
         d[2 * K + i] = t[2 * K + i]; // CSDA stopping power
       }
     }
+    if (CORR) {
+      for (int j = 0; j < 3; ++j) {
+        const double* t = a.tab + (size_t)mats[j] * 19 * K;
+        double* d = lds + (12 + 2 * j) * K;
+        for (int i = tid; i < K; i += bd) {
+          d[i] = t[5 * K + i];       // MIXED stopping power
+          d[K + i] = t[7 * K + i];   // straggling
+        }
+      }
+    }
   }
   WTab T;
   T.rE = lds; T.rLE = lds + K; T.rS = lds + 2 * K;
   T.wE = lds + 3 * K; T.wLE = lds + 4 * K; T.wS = lds + 5 * K;
   T.pE = lds + 6 * K; T.pLE = lds + 7 * K; T.pS = lds + 8 * K;
   T.aE = lds + 9 * K; T.aLE = lds + 10 * K; T.aS = lds + 11 * K;
+  T.rM = lds + 12 * K; T.rO = lds + 13 * K;
+  T.wM = lds + 14 * K; T.wO = lds + 15 * K;
+  T.pM = lds + 16 * K; T.pO = lds + 17 * K;
   T.K = K; T.pow2 = a.pow2; T.shared = a.shared; T.mass = a.mass; T.ctau = a.ctau;
   const WMat M = a.mat;
-  double* sc = JAC ? a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd : nullptr;
+  const WCorr C = a.corr;
+  double* sc = JAC ? a.scratch + (size_t)blockIdx.x * a.max_seg * NT * bd : nullptr;
 
   for (;;) {
     if (tid == 0) s_ray = atomicAdd(a.queue, 1);
@@ -292,12 +400,39 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
     const double zex = a.exit_z[r];
     const double air_d = fmax(0.0, a.palt - zex) / fmax(ct, 1e-3);
     const double z0 = a.det_elev + zex;
+    if (CORR && tid == 0) {
+      // csda_geom_multiplier (path_rock_length) and _column_water_fraction,
+      // summed in path order
+      double L = rem, water = 0.0, mass = rem * M.rho_r;
+      bool bad = false;
+      for (int s = 0; s < ns; ++s) {
+        const double d = a.len[b0 + s];
+        L += d;
+        const int c = scell[s];
+        if (c < 0) { bad = true; continue; }
+        const double wk = a.w[c];
+        const double gx = d * cell_density(M, wk, M.porous_on && a.shallow[c] != 0);
+        water += wk * gx;
+        mass += gx;
+      }
+      double G = 1.0;
+      if (C.enabled && C.geom) {
+        const double lL = log(fmax(L, 1.0));
+        G = exp(C.ra + C.rb * lL + C.rc * lL * lL + C.rd * ct);
+      }
+      const double q = bad ? 0.0 : water / fmax(mass, 1e-30);
+      s_G = G;
+      s_q = q;
+      s_mass = mass;
+      s_M = C.mat ? exp(C.lin * q + C.quad * q * q) : 1.0;
+    }
 
     for (int n0 = 0; n0 < a.n_nodes; n0 += bd) {
       const int n = n0 + tid;
       const bool active = n < a.n_nodes;
       double E = active ? a.ne[n] : a.ne[0];
-      double W = 1.0;
+      double W = 1.0, V = 0.0, H = 0.0;
+      const bool acc = CORR && C.enabled;
       bool ok = true;
       // ------------------------------------------------ forward sweep
       for (int s = 0; s < ns && ok; ++s) {
@@ -307,27 +442,70 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
         const bool porous = M.porous_on && a.shallow[c] != 0;
         const double d = a.len[b0 + s];
         if (JAC) {
+          const D2 wd(wv, 1.0, 0.0);
           D2 e(E, 0.0, 1.0), g(W);
-          ok = cell_segment(T, M, D2(wv, 1.0, 0.0), porous, d, e, g);
-          double* p = sc + (size_t)s * 4 * bd + tid;
+          ok = cell_segment(T, M, wd, porous, d, e, g);
+          double* p = sc + (size_t)s * NT * bd + tid;
           p[0] = e.a;
           p[bd] = e.b;
           p[2 * bd] = g.a / g.v;
           p[3 * bd] = g.b / g.v;
+          if (CORR) {
+            double vw = 0.0, ve = 0.0, hw = 0.0, he = 0.0;
+            if (acc && ok) {
+              D2 er = 0.5 * (D2(E, 0.0, 1.0) + e);
+              D2 om, hl;
+              cell_strag_hard(T, wd, porous, er, om, hl);
+              D2 gx = d * cell_density(M, wd, porous);
+              D2 v = om * gx, h = hl * gx;
+              V += v.v; H += h.v;
+              vw = v.a; ve = v.b; hw = h.a; he = h.b;
+            }
+            p[4 * bd] = vw;
+            p[5 * bd] = ve;
+            p[6 * bd] = hw;
+            p[7 * bd] = he;
+          }
           E = e.v;
           W = g.v;
         } else {
+          const double e_in = E;
           ok = cell_segment(T, M, wv, porous, d, E, W);
+          if (CORR && acc && ok) {
+            const double er = 0.5 * (e_in + E);
+            double om, hl;
+            cell_strag_hard(T, wv, porous, er, om, hl);
+            const double gx = d * cell_density(M, wv, porous);
+            V += om * gx;
+            H += hl * gx;
+          }
         }
       }
-      double rc = 1.0, rl = 0.0, ac = 1.0, al = 0.0;
+      double rc = 1.0, rl = 0.0, ac = 1.0, al = 0.0, rv = 0.0, rh = 0.0;
       if (ok && rem > 1e-8) {
+        const double gx = rem * M.rho_r;
         if (JAC) {
           DualD e(E, 1.0), g(W);
           ok = rock_segment(T, M.rho_r, rem, e, g);
-          rc = e.d; rl = g.d / g.v; E = e.v; W = g.v;
+          rc = e.d; rl = g.d / g.v;
+          if (CORR && acc && ok) {
+            DualD er = 0.5 * (DualD(E, 1.0) + e);
+            DualD om, hl;
+            rock_strag_hard(T, er, om, hl);
+            V += om.v * gx; H += hl.v * gx;
+            rv = om.d * gx; rh = hl.d * gx;
+          }
+          E = e.v; W = g.v;
         } else {
+          const double e_in = E;
           ok = rock_segment(T, M.rho_r, rem, E, W);
+          if (CORR && acc && ok) {
+            const double er = 0.5 * (e_in + E);
+            double om, hl;
+            rock_strag_hard(T, er, om, hl);
+            V += om * gx;
+            H += hl * gx;
+          }
         }
       }
       if (ok) {
@@ -339,16 +517,32 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
           ok = air_segment(T, z0, ct, air_d, E, W);
         }
       }
-      double f = 0.0, Tad = 0.0;
+      double f = 0.0, Tad = 0.0, AV = 0.0, AH = 0.0;
       if (active && ok) {
+        double cv = 1.0, AE = 0.0;
+        if (CORR && acc) {
+          if (JAC) {
+            const DualD lc = log_var_corr(DualD(E, 1.0), V, H, ct, C);
+            cv = exp(lc.v);
+            AE = lc.d;
+            const double e = fmax(E, 1.0);
+            const double eta = 1.0e-2;
+            const double bi = -(log(flux_gccly_r(ct, e * (1.0 + eta), 1.0)) -
+                                log(flux_gccly_r(ct, e * (1.0 - eta), 1.0))) / (2 * eta);
+            AV = C.ks * (0.5 * bi * (bi + 1.0)) / (e * e);
+            AH = H >= 0.0 ? -C.kh / e : 0.0;
+          } else {
+            cv = exp(log_var_corr(E, V, H, ct, C));
+          }
+        }
         if (JAC) {
           Dd phi = flux_gccly_d(ct, E, a.nq[n]);
-          f = 2.0 * a.nw[n] * (W * phi.v);
-          Tad = phi.d / phi.v;
+          f = 2.0 * a.nw[n] * (W * cv * phi.v);
+          Tad = phi.d / phi.v + AE;
           Tad = al + Tad * ac;
-          if (rem > 1e-8) Tad = rl + Tad * rc;
+          if (rem > 1e-8) Tad = rl + Tad * rc + AV * rv + AH * rh;
         } else {
-          f = 2.0 * a.nw[n] * (W * flux_gccly(ct, E, a.nq[n]));
+          f = 2.0 * a.nw[n] * (W * cv * flux_gccly(ct, E, a.nq[n]));
         }
       }
       const double fs = wave_sum(f);
@@ -356,11 +550,16 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
       if (JAC) {
         // ---------------------------------------------- reverse sweep
         for (int s = ns - 1; s >= 0; --s) {
-          const double* p = sc + (size_t)s * 4 * bd + tid;
+          const double* p = sc + (size_t)s * NT * bd + tid;
           double v = 0.0;
           if (f != 0.0) {
-            v = f * (p[2 * bd] + Tad * p[0]);
-            Tad = p[3 * bd] + Tad * p[bd];
+            if (CORR) {
+              v = f * (p[2 * bd] + Tad * p[0] + AV * p[4 * bd] + AH * p[6 * bd]);
+              Tad = p[3 * bd] + Tad * p[bd] + AV * p[5 * bd] + AH * p[7 * bd];
+            } else {
+              v = f * (p[2 * bd] + Tad * p[0]);
+              Tad = p[3 * bd] + Tad * p[bd];
+            }
           }
           const double vs = wave_sum(v);
           if (lane == 0) wsum[wave * a.max_seg + s] += vs;
@@ -368,12 +567,30 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
       }
     }
     __syncthreads();
+    double fsum = 0.0;
+    for (int j = 0; j < nwv; ++j) fsum += wflux[j];
     if (JAC) {
       // total per segment, then repeated cells merged into the first occurrence
       for (int s = tid; s < ns; s += bd) {
         double v = 0.0;
         for (int j = 0; j < nwv; ++j) v += wsum[j * a.max_seg + s];
-        wsum[s] = v / a.n_nodes;  // row 0 holds the totals; each s is read then written by one thread
+        if (CORR) {
+          v = s_G * s_M * (v / a.n_nodes);
+          const int c = scell[s];
+          if (C.mat && c >= 0) {
+            // d q / d w_s of the column water fraction, times M'(q)
+            const double wk = a.w[c];
+            const DualD gx = a.len[b0 + s] * cell_density(M, DualD(wk, 1.0),
+                                                          M.porous_on && a.shallow[c] != 0);
+            const double dwater = gx.v + wk * gx.d;
+            const double dq = s_mass >= 1e-30 ? (dwater - s_q * gx.d) / s_mass : dwater / 1e-30;
+            const double dM = s_M * (C.lin + 2.0 * C.quad * s_q);
+            v += s_G * dM * dq * (fsum / a.n_nodes);
+          }
+          wsum[s] = v;
+        } else {
+          wsum[s] = v / a.n_nodes;  // row 0 holds the totals; each s is read then written by one thread
+        }
       }
       __syncthreads();
       for (int s = tid; s < ns; s += bd) {
@@ -390,11 +607,7 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
         a.jac[b0 + s] = v;
       }
     }
-    if (tid == 0) {
-      double fsum = 0.0;
-      for (int j = 0; j < nwv; ++j) fsum += wflux[j];
-      a.flux[r] = fsum / a.n_nodes;
-    }
+    if (tid == 0) a.flux[r] = CORR ? s_G * s_M * fsum / a.n_nodes : fsum / a.n_nodes;
     __syncthreads();
   }
 }
@@ -404,8 +617,8 @@ __global__ void reset_counters_w(int* q, int n) {
   if (i < n) q[i] = 0;
 }
 
-size_t w_lds_bytes(int K, int nwv, int max_seg) {
-  return ((size_t)12 * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
+size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr) {
+  return ((size_t)w_tab_cols(corr) * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
          (size_t)max_seg * sizeof(int);
 }
 
@@ -445,24 +658,33 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
   const int bd = es->n >= 256 ? 256 : ((es->n + 63) / 64) * 64;
   const int nwv = bd / 64;
   const int max_seg = p->max_segments > 0 ? p->max_segments : 1;
-  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg);
+  const bool J = jac != nullptr;
+  const bool CR = c->corr.enabled || c->mcorr.enabled;
+  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg, CR);
   if (lds > 160 * 1024)
     return dph::fail(DP_E_CAPACITY, "dp_csda_w_flux_jacobian: max_segments " +
                                         std::to_string(max_seg) + " exceeds the LDS budget");
-  const bool J = jac != nullptr;
+  auto kern = J ? (CR ? wcsda_kernel<true, true> : wcsda_kernel<true, false>)
+                : (CR ? wcsda_kernel<false, true> : wcsda_kernel<false, false>);
   int occ = 0;
-  if (J) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcsda_kernel<true>, bd, lds));
-  else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcsda_kernel<false>, bd, lds));
+  DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, bd, lds));
   if (occ < 1) occ = 1;
   int grid = c->n_cu * occ;
   if (grid > p->n_rays) grid = p->n_rays;
   WArgs a;
   a.scratch = nullptr;
   if (J) {
-    int rc = dph::ensure(c->scratch, (size_t)grid * max_seg * 4 * bd * sizeof(double));
+    int rc = dph::ensure(c->scratch, (size_t)grid * max_seg * w_tape(CR) * bd * sizeof(double));
     if (rc) return rc;
     a.scratch = (double*)c->scratch.ptr;
   }
+  const dp_csda_correction& k = c->corr;
+  a.corr.enabled = k.enabled != 0;
+  a.corr.geom = k.resid_a != 0.0 || k.resid_b != 0.0 || k.resid_c != 0.0 || k.resid_d != 0.0;
+  a.corr.mat = c->mcorr.enabled != 0;
+  a.corr.ks = k.kappa_strag; a.corr.kh = k.kappa_hard;
+  a.corr.ra = k.resid_a; a.corr.rb = k.resid_b; a.corr.rc = k.resid_c; a.corr.rd = k.resid_d;
+  a.corr.lin = c->mcorr.linear; a.corr.quad = c->mcorr.quadratic;
   const int porous = m->porous_material;
   a.tab = t->data; a.K = t->n_knots; a.pow2 = t->pow2;
   a.rock = m->rock_material; a.water = m->water_material; a.air = m->air_material;
@@ -481,8 +703,7 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
   a.queue = qerr; a.err = qerr + 1;
   hipLaunchKernelGGL(reset_counters_w, dim3(1), dim3(64), 0, c->stream, qerr, 2);
   (void)hipEventRecord(c->ev0, c->stream);
-  if (J) hipLaunchKernelGGL(wcsda_kernel<true>, dim3(grid), dim3(bd), lds, c->stream, a);
-  else hipLaunchKernelGGL(wcsda_kernel<false>, dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(bd), lds, c->stream, a);
   (void)hipEventRecord(c->ev1, c->stream);
   DP_HIP(hipGetLastError());
   return DP_OK;
@@ -491,6 +712,20 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
 }  // namespace
 
 extern "C" {
+
+int dp_set_csda_correction(dp_ctx c, const dp_csda_correction* k) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_set_csda_correction: null context");
+  c->corr = k ? *k : dp_csda_correction{};
+  return DP_OK;
+}
+
+int dp_set_csda_material_correction(dp_ctx c, const dp_csda_material_correction* k) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_set_csda_material_correction: null context");
+  if (k && !(std::isfinite(k->linear) && std::isfinite(k->quadratic)))
+    return dph::fail(DP_E_ARGUMENT, "dp_set_csda_material_correction: nonfinite material coefficients");
+  c->mcorr = k ? *k : dp_csda_material_correction{};
+  return DP_OK;
+}
 
 int dp_csda_w_flux_jacobian_dev(dp_ctx c, dp_tables t, const dp_material_config* m,
                                 const dp_site_config* site, const dp_directional_paths* p,

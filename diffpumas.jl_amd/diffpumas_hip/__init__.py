@@ -17,7 +17,9 @@ from .tomography import (DirectionalPaths, EnergySamples, MaterialConfig, SiteCo
                          direction_vector, directional_flux_and_grad_csda,
                          flux_and_segment_jacobian, sample_energy_set, trace_voxel_paths,
                          mc_cell_mixture_and_density, build_cell_properties_for_mc,
-                         compute_directional_flux_mc)
+                         compute_directional_flux_mc, CsdaCorrection, CsdaMaterialCorrection,
+                         set_csda_correction, get_csda_correction,
+                         set_csda_material_correction, get_csda_material_correction)
 
 __all__ = [
     "ArgumentError", "CapacityError", "Context", "DomainError", "HipError", "TransportError",
@@ -28,6 +30,8 @@ __all__ = [
     "run_backward_mc", "DirectionalPaths", "EnergySamples", "MaterialConfig", "SiteConfig",
     "assemble_forward_and_jacobian", "cell_density", "compute_directional_flux_csda",
     "csda_energy_quadrature", "direction_vector", "directional_flux_and_grad_csda",
+    "CsdaCorrection", "CsdaMaterialCorrection", "set_csda_correction", "get_csda_correction",
+    "set_csda_material_correction", "get_csda_material_correction",
     "flux_and_segment_jacobian", "sample_energy_set", "trace_voxel_paths",
     "mc_cell_mixture_and_density", "build_cell_properties_for_mc", "compute_directional_flux_mc",
 ]

@@ -138,6 +138,16 @@ class DirectionalPaths(C.Structure):
                 ("remaining_rock", dbl_p), ("exit_z", dbl_p), ("valid", C.POINTER(C.c_uint8))]
 
 
+class CsdaCorrection(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("kappa_strag", C.c_double), ("kappa_hard", C.c_double),
+                ("resid_a", C.c_double), ("resid_b", C.c_double), ("resid_c", C.c_double),
+                ("resid_d", C.c_double)]
+
+
+class CsdaMaterialCorrection(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("linear", C.c_double), ("quadratic", C.c_double)]
+
+
 class EnergySamples(C.Structure):
     _fields_ = [("n", C.c_int32), ("energy", dbl_p), ("weight", dbl_p), ("charge", dbl_p)]
 
@@ -184,6 +194,8 @@ _SIGS = {
                                     C.c_int32, C.POINTER(Mixture), i32_p, dbl_p,
                                     C.POINTER(EnergySamples), C.c_uint64, C.c_int32, C.c_double,
                                     dbl_p, dbl_p, dbl_p]),
+    "dp_set_csda_correction": (C.c_int, [C.c_void_p, C.POINTER(CsdaCorrection)]),
+    "dp_set_csda_material_correction": (C.c_int, [C.c_void_p, C.POINTER(CsdaMaterialCorrection)]),
     "dp_csda_w_flux_jacobian": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(MaterialConfig),
                                           C.POINTER(SiteConfig), C.POINTER(DirectionalPaths),
                                           C.c_int32, dbl_p, C.POINTER(C.c_uint8),

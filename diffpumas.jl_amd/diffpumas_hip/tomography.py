@@ -5,7 +5,8 @@ Host mirror of the Tomography module's forward model (src/Tomography.jl:
 179-820).  Cells and materials are 0-based.  Every flux and Jacobian value
 comes from the HIP kernel behind dp_csda_w_flux_jacobian; there is no CPU
 path.  The MC-calibrated corrections (CsdaCorrection, CsdaMaterialCorrection)
-stay at the reference default, disabled.
+are module state as in the reference (global Refs, disabled by default) and
+are handed to the library with every call.
 """
 import ctypes as C
 import math
@@ -54,6 +55,68 @@ class EnergySamples:
 
     def __len__(self):
         return len(self.energy)
+
+
+@dataclass(frozen=True)
+class CsdaCorrection:
+    """CsdaCorrection (src/Tomography.jl:112-137)."""
+    enabled: bool
+    kappa_strag: float
+    kappa_hard: float
+    resid_a: float
+    resid_b: float
+    resid_c: float = 0.0
+    resid_d: float = 0.0
+
+
+@dataclass(frozen=True)
+class CsdaMaterialCorrection:
+    """CsdaMaterialCorrection (src/Tomography.jl:164-166)."""
+    enabled: bool
+    linear: float
+    quadratic: float
+
+
+_CSDA_CORRECTION = CsdaCorrection(False, 0.0, 0.0, 0.0, 0.0)            # :139
+_CSDA_MATERIAL_CORRECTION = CsdaMaterialCorrection(False, 0.0, 0.0)     # :167
+
+
+def set_csda_correction(enabled=True, kappa_strag=0.0, kappa_hard=0.0, resid_a=0.0,
+                        resid_b=0.0, resid_c=0.0, resid_d=0.0):
+    """set_csda_correction! (src/Tomography.jl:149-155)."""
+    global _CSDA_CORRECTION
+    _CSDA_CORRECTION = CsdaCorrection(bool(enabled), float(kappa_strag), float(kappa_hard),
+                                      float(resid_a), float(resid_b), float(resid_c),
+                                      float(resid_d))
+    return _CSDA_CORRECTION
+
+
+def get_csda_correction():
+    return _CSDA_CORRECTION
+
+
+def set_csda_material_correction(enabled=True, linear=0.0, quadratic=0.0):
+    """set_csda_material_correction! (src/Tomography.jl:168-172); non-finite
+    coefficients raise ArgumentError."""
+    global _CSDA_MATERIAL_CORRECTION
+    if not (math.isfinite(linear) and math.isfinite(quadratic)):
+        raise ArgumentError("nonfinite material coefficients")
+    _CSDA_MATERIAL_CORRECTION = CsdaMaterialCorrection(bool(enabled), float(linear),
+                                                       float(quadratic))
+    return _CSDA_MATERIAL_CORRECTION
+
+
+def get_csda_material_correction():
+    return _CSDA_MATERIAL_CORRECTION
+
+
+def _push_corrections(ctx):
+    c, m = _CSDA_CORRECTION, _CSDA_MATERIAL_CORRECTION
+    cc = _lib.CsdaCorrection(int(c.enabled), c.kappa_strag, c.kappa_hard, c.resid_a, c.resid_b,
+                             c.resid_c, c.resid_d)
+    mm = _lib.CsdaMaterialCorrection(int(m.enabled), m.linear, m.quadratic)
+    _lib.check(ctx.lib.dp_set_csda_correction(ctx.h, C.byref(cc)))
+    _lib.check(ctx.lib.dp_set_csda_material_correction(ctx.h, C.byref(mm)))
 
 
 def csda_energy_quadrature(n, energy_min, energy_max):
@@ -187,6 +250,7 @@ def _run(physics, shallow_flags, matcfg, site, paths, water_fractions, samples, 
     cp = paths.c_paths()
     flux = np.zeros(paths.n_rays)
     jac = np.zeros(max(paths.n_segments, 1)) if jacobian else None
+    _push_corrections(ctx)
     _lib.check(ctx.lib.dp_csda_w_flux_jacobian(ctx.h, h, C.byref(mc), C.byref(sc), C.byref(cp),
                                                len(w), ptr(w), ptr(sh, C.c_uint8), C.byref(es),
                                                ptr(flux), ptr(jac)))

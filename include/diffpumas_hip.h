@@ -297,6 +297,36 @@ typedef struct {
   const uint8_t* valid;        /* per ray; NULL = all valid                  */
 } dp_directional_paths;
 
+/* CsdaCorrection (src/Tomography.jl:112-137).  Per energy node the flux is
+ * multiplied by exp(kappa_strag b(b+1)/2 V/E^2 - kappa_hard max(H,0)/E)
+ * (csda_var_correction, :384-391), E the surface energy floored at 1 GeV,
+ * b the local spectral index of the surface flux (:367-372), V and H the
+ * straggling and hard-loss integrals along the rock column; each bin by
+ * exp(resid_a + resid_b logL + resid_c logL^2 + resid_d cos theta)
+ * (csda_geom_multiplier, :405-412), L the rock path length in m.          */
+typedef struct {
+  int32_t enabled;
+  double kappa_strag;
+  double kappa_hard;
+  double resid_a, resid_b, resid_c, resid_d;
+} dp_csda_correction;
+
+/* CsdaMaterialCorrection (src/Tomography.jl:164-166): each bin's flux is
+ * multiplied by exp(linear q + quadratic q^2), q the water mass fraction of
+ * the whole rock column (_column_water_fraction, :560-574).               */
+typedef struct {
+  int32_t enabled;
+  double linear;
+  double quadratic;
+} dp_csda_material_correction;
+
+/* Replaces: set_csda_correction! (src/Tomography.jl:149-155) and
+ * set_csda_material_correction! (:168-172).  The context keeps the values
+ * for its later w-model calls; NULL switches the correction off.
+ * Non-finite material coefficients return DP_E_ARGUMENT (ArgumentError). */
+int dp_set_csda_correction(dp_ctx ctx, const dp_csda_correction* corr);
+int dp_set_csda_material_correction(dp_ctx ctx, const dp_csda_material_correction* corr);
+
 /* Vector{EnergySample} (src/Tomography.jl:203-207). */
 typedef struct {
   int32_t n;
@@ -313,8 +343,10 @@ typedef struct {
  * cell.  flux[r] is NaN for an invalid path or cos_theta <= 0.  jac has one
  * slot per path segment: d flux[r] / d w[cell] sits at the cell's first
  * occurrence on the path (the column order of _path_layout, :640-659), later
- * occurrences of the same cell hold 0.  The MC-calibrated corrections
- * (CsdaCorrection, CsdaMaterialCorrection) are the reference defaults: off. */
+ * occurrences of the same cell hold 0.  The MC-calibrated corrections are
+ * those last set on the context (dp_set_csda_correction,
+ * dp_set_csda_material_correction); both are off by default, as in the
+ * reference (:139, :167), and their derivatives enter jac when on. */
 int dp_csda_w_flux_jacobian(dp_ctx ctx, dp_tables tables, const dp_material_config* mat,
                             const dp_site_config* site, const dp_directional_paths* paths,
                             int32_t n_cells, const double* water_fraction,

@@ -391,7 +391,19 @@ MatCfg make_matcfg(const int32_t* mi, const double* md) {
   return MatCfg{mi[0], mi[1], mi[2], mi[3], md[0], md[1], md[2], md[3]};
 }
 // Tape entry of one cell segment: energy partials and log-weight partials.
-struct WSeg { double ew, ee, lw, le; };
+struct WSeg { double ew, ee, lw, le, vw, ve, hw, he; };
+// corr[10] = enabled, kappa_strag, kappa_hard, resid_a..d, material enabled,
+// linear, quadratic; NULL = both corrections off.
+Corr make_corr(const double* c) {
+  Corr k;
+  if (!c) return k;
+  k.enabled = c[0] != 0.0;
+  k.kappa_strag = c[1]; k.kappa_hard = c[2];
+  k.resid_a = c[3]; k.resid_b = c[4]; k.resid_c = c[5]; k.resid_d = c[6];
+  k.mat_enabled = c[7] != 0.0;
+  k.linear = c[8]; k.quadratic = c[9];
+  return k;
+}
 }  // namespace
 extern "C" {
 
@@ -429,10 +441,12 @@ void orc_w_flux(int M, int K, const double* data, const double* scal, double mas
                 const int32_t* row_ptr, const int32_t* cells, const double* lens,
                 const double* remaining, const double* exit_z, const double* cos_t,
                 const uint8_t* valid, const double* w, const uint8_t* shallow, int n_nodes,
-                const double* ne, const double* nw, const double* nq, double* flux, int threads) {
+                const double* ne, const double* nw, const double* nq, const double* corr10,
+                double* flux, int threads) {
   Tables T = make_tables(M, K, data, scal, mass, ctau);
   MatCfg m = make_matcfg(mi, md);
   Site site{site2[0], site2[1]};
+  const Corr corr = make_corr(corr10);
   if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel
   {
@@ -447,7 +461,7 @@ void orc_w_flux(int M, int K, const double* data, const double* scal, double mas
       for (int s = 0; s < ns; ++s) { wl[s] = w[cells[b + s]]; sl[s] = shallow[cells[b + s]]; idx[s] = s; }
       flux[r] = directional_flux_w<double>(T, m, site, idx.data(), lens + b, ns, remaining[r],
                                            exit_z[r], cos_t[r], wl.data(), sl.data(), n_nodes, ne,
-                                           nw, nq);
+                                           nw, nq, corr);
     }
   }
 }
@@ -458,14 +472,15 @@ double orc_w_dflux(int M, int K, const double* data, const double* scal, double 
                    const int32_t* mi, const double* md, const double* site2, int ns,
                    const int32_t* cells, const double* lens, double remaining, double exit_z,
                    double cos_t, int n_cells, const double* w, const uint8_t* shallow, int cell,
-                   int n_nodes, const double* ne, const double* nw, const double* nq) {
+                   int n_nodes, const double* ne, const double* nw, const double* nq,
+                   const double* corr10) {
   Tables T = make_tables(M, K, data, scal, mass, ctau);
   MatCfg m = make_matcfg(mi, md);
   Site site{site2[0], site2[1]};
   std::vector<Dual> wd(n_cells);
   for (int c = 0; c < n_cells; ++c) wd[c] = Dual(w[c], c == cell ? 1.0 : 0.0);
   Dual f = directional_flux_w<Dual>(T, m, site, cells, lens, ns, remaining, exit_z, cos_t,
-                                    wd.data(), shallow, n_nodes, ne, nw, nq);
+                                    wd.data(), shallow, n_nodes, ne, nw, nq, make_corr(corr10));
   return f.d;
 }
 
@@ -474,16 +489,23 @@ double orc_w_dflux(int M, int K, const double* data, const double* scal, double 
 // src/Tomography_rules.jl:63-106.  jac has one slot per path segment: the
 // derivative with respect to that segment's cell sits at the cell's first
 // occurrence on the path, later occurrences hold 0.
+//
+// With the corrections on, log flux_n gains log corr(E_s, V, H): per segment
+// the tape also holds dv/dw, dv/dE_in, dh/dw, dh/dE_in of the segment's V and H
+// terms (total derivatives through E_out), and the reverse sweep adds
+// A_V dv + A_H dh with A_V = d log corr / dV, A_H = d log corr / dH.  The
+// material multiplier adds M'(q) dq/dw_s times the node sum.
 void orc_w_jacobian(int M, int K, const double* data, const double* scal, double mass,
                     double ctau, const int32_t* mi, const double* md, const double* site2,
                     int n_rays, const int32_t* row_ptr, const int32_t* cells, const double* lens,
                     const double* remaining, const double* exit_z, const double* cos_t,
                     const uint8_t* valid, const double* w, const uint8_t* shallow, int n_nodes,
-                    const double* ne, const double* nw, const double* nq, double* flux,
-                    double* jac, int threads) {
+                    const double* ne, const double* nw, const double* nq, const double* corr10,
+                    double* flux, double* jac, int threads) {
   Tables T = make_tables(M, K, data, scal, mass, ctau);
   MatCfg m = make_matcfg(mi, md);
   Site site{site2[0], site2[1]};
+  const Corr corr = make_corr(corr10);
   if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel
   {
@@ -494,23 +516,44 @@ void orc_w_jacobian(int M, int K, const double* data, const double* scal, double
       for (int s = 0; s < ns; ++s) jac[b + s] = 0.0;
       if (!valid[r] || !(cos_t[r] > 0.0)) { flux[r] = NAN; continue; }
       const double ct = cos_t[r];
+      const double rem = remaining[r];
       tape.resize(ns);
       double facc = 0.0;
       for (int n = 0; n < n_nodes; ++n) {
-        double E = ne[n], W = 1.0;
+        double E = ne[n], W = 1.0, V = 0.0, H = 0.0;
         bool ok = true;
         for (int s = 0; s < ns && ok; ++s) {
           const int c = cells[b + s];
-          Dual2 e(E, 0.0, 1.0), g(W);
-          cell_segment(T, Dual2(w[c], 1.0, 0.0), shallow[c] != 0, m, lens[b + s], e, g, ok);
-          tape[s] = WSeg{e.a, e.b, g.a / g.v, g.b / g.v};
+          const bool sh = shallow[c] != 0;
+          const Dual2 wd(w[c], 1.0, 0.0), e_in(E, 0.0, 1.0);
+          Dual2 e = e_in, g(W);
+          cell_segment(T, wd, sh, m, lens[b + s], e, g, ok);
+          WSeg t{e.a, e.b, g.a / g.v, g.b / g.v, 0.0, 0.0, 0.0, 0.0};
+          if (ok && corr.enabled) {
+            Dual2 e_rep = 0.5 * (e_in + e);
+            Dual2 gx = lens[b + s] * cell_density(wd, sh, m);
+            Dual2 v = cell_strag(T, wd, sh, m, e_rep) * gx;
+            Dual2 h = cell_hard(T, wd, sh, m, e_rep) * gx;
+            V += v.v; H += h.v;
+            t.vw = v.a; t.ve = v.b; t.hw = h.a; t.he = h.b;
+          }
+          tape[s] = t;
           E = e.v; W = g.v;
         }
-        double rc = 1.0, rl = 0.0;  // remaining-rock partials
-        if (ok && remaining[r] > 1e-8) {
-          Dual e(E, 1.0), g(W);
-          uniform_segment(T, m.rock, m.rho_rock, remaining[r], e, g, ok);
-          rc = e.d; rl = g.d / g.v; E = e.v; W = g.v;
+        double rc = 1.0, rl = 0.0, rv = 0.0, rh = 0.0;  // remaining-rock partials
+        if (ok && rem > 1e-8) {
+          const Dual e_in(E, 1.0);
+          Dual e = e_in, g(W);
+          uniform_segment(T, m.rock, m.rho_rock, rem, e, g, ok);
+          rc = e.d; rl = g.d / g.v;
+          if (ok && corr.enabled) {
+            Dual e_rep = 0.5 * (e_in + e);
+            const double gx = rem * m.rho_rock;
+            Dual v = prop_strag(T, m.rock, e_rep) * gx;
+            Dual h = hardloss(T, m.rock, e_rep) * gx;
+            V += v.v; H += h.v; rv = v.d; rh = h.d;
+          }
+          E = e.v; W = g.v;
         }
         double ac = 1.0, al = 0.0;  // atmosphere partials
         if (ok) {
@@ -521,26 +564,78 @@ void orc_w_jacobian(int M, int K, const double* data, const double* scal, double
         }
         if (!ok) continue;
         Dual fl = flux_gccly(ct, Dual(E, 1.0), nq[n]);
-        const double f = 2.0 * nw[n] * (W * fl.v);
+        double cv = 1.0, AE = 0.0, AV = 0.0, AH = 0.0;
+        if (corr.enabled) {
+          Dual cd = var_correction(Dual(E, 1.0), Dual(V), Dual(H), ct, corr);
+          cv = cd.v;
+          AE = cd.d / cd.v;
+          const double e = std::max(E, 1.0);
+          const double bi = spectral_index(e, ct);
+          AV = corr.kappa_strag * (0.5 * bi * (bi + 1.0)) / (e * e);
+          AH = H >= 0.0 ? -corr.kappa_hard / e : 0.0;
+        }
+        const double f = 2.0 * nw[n] * (W * cv * fl.v);
         facc += f;
         if (f == 0.0) continue;
-        double Tad = fl.d / fl.v;        // d log(flux)/dE at the surface
+        double Tad = fl.d / fl.v + AE;   // d log(flux_n)/dE at the surface
         Tad = al + Tad * ac;             // ... at the rock exit
-        if (remaining[r] > 1e-8) Tad = rl + Tad * rc;
+        if (rem > 1e-8) Tad = rl + Tad * rc + AV * rv + AH * rh;
         for (int s = ns - 1; s >= 0; --s) {
           const WSeg& t = tape[s];
-          jac[b + s] += f * (t.lw + Tad * t.ew);
-          Tad = t.le + Tad * t.ee;
+          jac[b + s] += f * (t.lw + Tad * t.ew + AV * t.vw + AH * t.hw);
+          Tad = t.le + Tad * t.ee + AV * t.ve + AH * t.he;
         }
       }
-      flux[r] = facc / n_nodes;
-      for (int s = 0; s < ns; ++s) jac[b + s] /= n_nodes;
+      const double G = geom_multiplier(lens + b, ns, rem, ct, corr);
+      std::vector<int> idx(ns);
+      for (int s = 0; s < ns; ++s) idx[s] = s;
+      std::vector<double> wl(ns);
+      std::vector<uint8_t> sl(ns);
+      for (int s = 0; s < ns; ++s) { wl[s] = w[cells[b + s]]; sl[s] = shallow[cells[b + s]]; }
+      const double q = column_water_fraction(m, rem, lens + b, idx.data(), ns, wl.data(), sl.data());
+      const double Mq = material_multiplier(q, corr);
+      flux[r] = G * Mq * facc / n_nodes;
+      // d q / d w_s (the column mass fraction is linear in each cell's mass)
+      double cmass = rem * m.rho_rock;
+      for (int s = 0; s < ns; ++s) cmass += lens[b + s] * cell_density(wl[s], sl[s] != 0, m);
+      const double dM = corr.mat_enabled ? Mq * (corr.linear + 2.0 * corr.quadratic * q) : 0.0;
+      for (int s = 0; s < ns; ++s) {
+        double v = G * Mq * (jac[b + s] / n_nodes);
+        if (dM != 0.0) {
+          const Dual gx = lens[b + s] * cell_density(Dual(wl[s], 1.0), sl[s] != 0, m);
+          const double dwater = gx.v + wl[s] * gx.d;
+          const double dq = cmass >= 1e-30 ? (dwater - q * gx.d) / cmass : dwater / 1e-30;
+          v += G * dM * dq * (facc / n_nodes);
+        }
+        jac[b + s] = v;
+      }
       // merge repeated cells into their first occurrence (_path_layout)
       for (int s = 1; s < ns; ++s)
         for (int t = 0; t < s; ++t)
           if (cells[b + t] == cells[b + s]) { jac[b + t] += jac[b + s]; jac[b + s] = 0.0; break; }
     }
   }
+}
+
+// One energy node on one path (walk of _directional_flux_core,
+// src/Tomography.jl:585-637): out = surface energy, weight, V, H, ok.
+void orc_w_node_walk(int M, int K, const double* data, const double* scal, double mass,
+                     double ctau, const int32_t* mi, const double* md, const double* site2,
+                     int ns, const int32_t* cells, const double* lens, double remaining,
+                     double exit_z, double cos_t, const double* w, const uint8_t* shallow,
+                     double energy, double* out) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  MatCfg m = make_matcfg(mi, md);
+  Site site{site2[0], site2[1]};
+  NodeWalk<double> o = walk_node_w<double>(T, m, site, cells, lens, ns, remaining, exit_z, cos_t,
+                                           w, shallow, energy, true);
+  out[0] = o.energy; out[1] = o.weight; out[2] = o.V; out[3] = o.H; out[4] = o.ok;
+}
+
+// local_spectral_index (src/Tomography.jl:367-372) and csda_var_correction (:384-391).
+double orc_spectral_index(double energy, double cos_t) { return spectral_index(energy, cos_t); }
+double orc_var_correction(double energy, double V, double H, double cos_t, const double* corr10) {
+  return var_correction(energy, V, H, cos_t, make_corr(corr10));
 }
 
 }  // extern "C"

@@ -445,30 +445,155 @@ inline void air_segment(const Tables& T, int air, const Site& site, double start
   }
 }
 
-// compute_directional_flux_csda with the corrections disabled (their default,
-// src/Tomography.jl:139,167): sum over energy nodes of 2 w_n flux / N.
+// ------------------------------------------- MC-calibrated CSDA corrections
+// CsdaCorrection (src/Tomography.jl:112-137) and CsdaMaterialCorrection
+// (:164-172); both are disabled by default (:139, :167).
+struct Corr {
+  bool enabled = false;
+  double kappa_strag = 0, kappa_hard = 0;
+  double resid_a = 0, resid_b = 0, resid_c = 0, resid_d = 0;
+  bool mat_enabled = false;
+  double linear = 0, quadratic = 0;
+};
+
+// property_straggling, src/Physics.jl:820-824
+template <class R> inline R prop_strag(const Tables& T, int mat, R e) {
+  return property(T, mat, T_STRAGGLING, e);
+}
+// cell_straggling, src/Tomography.jl:329-339
+template <class R, class E>
+inline R cell_strag(const Tables& T, R w, bool shallow, const MatCfg& m, E e) {
+  if (shallow_porous(shallow, m)) {
+    R sp = prop_strag(T, m.porous, R(e)), sr = prop_strag(T, m.rock, R(e)),
+      sw = prop_strag(T, m.water, R(e));
+    return 0.5 * (1.0 - w) * sp + 0.5 * (1.0 - w) * sr + w * sw;
+  }
+  R sr = prop_strag(T, m.rock, R(e)), sw = prop_strag(T, m.water, R(e));
+  return (1.0 - w) * sr + w * sw;
+}
+// dedx_CSDA - dedx_MIXED of one material (rock_hardloss, :318-321)
+template <class R> inline R hardloss(const Tables& T, int mat, R e) {
+  return prop_stop(T, false, mat, e) - prop_stop(T, true, mat, e);
+}
+// cell_hardloss, src/Tomography.jl:341-352
+template <class R, class E>
+inline R cell_hard(const Tables& T, R w, bool shallow, const MatCfg& m, E e) {
+  R hr = hardloss(T, m.rock, R(e)), hw = hardloss(T, m.water, R(e));
+  if (shallow_porous(shallow, m)) {
+    R hp = hardloss(T, m.porous, R(e));
+    return 0.5 * (1.0 - w) * hp + 0.5 * (1.0 - w) * hr + w * hw;
+  }
+  return (1.0 - w) * hr + w * hw;
+}
+// local_spectral_index, src/Tomography.jl:367-372: symmetric log-difference
+// of the surface flux at charge +1.
+template <class R> inline R spectral_index(R E, double cos_t) {
+  const double eta = 1.0e-2;
+  R fp = flux_gccly(cos_t, E * (1.0 + eta), 1.0);
+  R fm = flux_gccly(cos_t, E * (1.0 - eta), 1.0);
+  return -(log(fp) - log(fm)) / (2 * eta);
+}
+// csda_var_correction, src/Tomography.jl:384-391
+template <class R>
+inline R var_correction(R energy_surface, R V, R H, double cos_t, const Corr& c) {
+  if (!c.enabled) return R(1.0);
+  R e = cmax(energy_surface, 1.0);
+  R b = spectral_index(e, cos_t);
+  R cs = 0.5 * b * (b + 1.0);
+  return exp(c.kappa_strag * cs * V / (e * e) - c.kappa_hard * cmax(H, 0.0) / e);
+}
+// path_rock_length + csda_geom_multiplier, src/Tomography.jl:394-412
+inline double geom_multiplier(const double* lens, int nseg, double remaining_rock, double cos_t,
+                              const Corr& c) {
+  if (!(c.enabled && (c.resid_a != 0.0 || c.resid_b != 0.0 || c.resid_c != 0.0 ||
+                      c.resid_d != 0.0)))
+    return 1.0;
+  double L = remaining_rock;
+  for (int s = 0; s < nseg; ++s) L += lens[s];
+  const double lL = std::log(std::max(L, 1.0));
+  return std::exp(c.resid_a + c.resid_b * lL + c.resid_c * lL * lL + c.resid_d * cos_t);
+}
+// _column_water_fraction, src/Tomography.jl:560-570
+template <class R>
+inline R column_water_fraction(const MatCfg& m, double remaining_rock, const double* lens,
+                               const int* cells, int nseg, const R* w, const uint8_t* shallow) {
+  R water = R(0.0);
+  R mass = R(remaining_rock * m.rho_rock);
+  for (int s = 0; s < nseg; ++s) {
+    R wk = w[cells[s]];
+    R gx = lens[s] * cell_density(wk, shallow[cells[s]] != 0, m);
+    water = water + wk * gx;
+    mass = mass + gx;
+  }
+  return water / cmax(mass, 1e-30);
+}
+// _material_multiplier, src/Tomography.jl:571-574
+template <class R> inline R material_multiplier(R q, const Corr& c) {
+  return c.mat_enabled ? exp(c.linear * q + c.quadratic * q * q) : R(1.0);
+}
+
+// Per energy node: the surface energy, the weight, the accumulated soft-loss
+// variance proxy V and hard loss H of _directional_flux_core
+// (src/Tomography.jl:585-637).
+template <class R> struct NodeWalk { R energy, weight, V, H; bool ok; };
+
+template <class R>
+inline NodeWalk<R> walk_node_w(const Tables& T, const MatCfg& m, const Site& site, const int* cells,
+                               const double* lens, int nseg, double remaining_rock, double exit_z,
+                               double cos_t, const R* w, const uint8_t* shallow, double e0,
+                               bool accumulate) {
+  NodeWalk<R> o{R(e0), R(1.0), R(0.0), R(0.0), true};
+  for (int s = 0; s < nseg && o.ok; ++s) {
+    const bool sh = shallow[cells[s]] != 0;
+    R e_in = o.energy;
+    cell_segment(T, w[cells[s]], sh, m, lens[s], o.energy, o.weight, o.ok);
+    if (!o.ok || !accumulate) continue;
+    R e_rep = 0.5 * (e_in + o.energy);
+    R wseg = w[cells[s]];
+    R gx = lens[s] * cell_density(wseg, sh, m);
+    o.V = o.V + cell_strag(T, wseg, sh, m, e_rep) * gx;
+    o.H = o.H + cell_hard(T, wseg, sh, m, e_rep) * gx;
+  }
+  if (o.ok && remaining_rock > 1e-8) {
+    R e_in = o.energy;
+    uniform_segment(T, m.rock, m.rho_rock, remaining_rock, o.energy, o.weight, o.ok);
+    if (o.ok && accumulate) {
+      R e_rep = 0.5 * (e_in + o.energy);
+      const double gx = remaining_rock * m.rho_rock;
+      o.V = o.V + prop_strag(T, m.rock, e_rep) * gx;
+      o.H = o.H + hardloss(T, m.rock, e_rep) * gx;
+    }
+  }
+  if (o.ok) {
+    const double air_d = std::max(0.0, site.primary_altitude - exit_z) / std::max(cos_t, 1e-3);
+    air_segment(T, m.air, site, exit_z, cos_t, air_d, o.energy, o.weight, o.ok);
+  }
+  return o;
+}
+
+// compute_directional_flux_csda, src/Tomography.jl:667-726: sum over energy
+// nodes of 2 w_n flux / N, times the geometric and material multipliers.
 // R is the water-fraction type (double, or a dual seeded on cells).
 template <class R>
 inline R directional_flux_w(const Tables& T, const MatCfg& m, const Site& site, const int* cells,
                             const double* lens, int nseg, double remaining_rock, double exit_z,
                             double cos_t, const R* w, const uint8_t* shallow, int n_nodes,
-                            const double* ne, const double* nw, const double* nq) {
+                            const double* ne, const double* nw, const double* nq,
+                            const Corr& corr = Corr{}) {
   R sum = R(0.0);
   for (int n = 0; n < n_nodes; ++n) {
-    R energy = R(ne[n]);
-    R weight = R(1.0);
-    bool ok = true;
-    for (int s = 0; s < nseg && ok; ++s)
-      cell_segment(T, w[cells[s]], shallow[cells[s]] != 0, m, lens[s], energy, weight, ok);
-    if (ok && remaining_rock > 1e-8) uniform_segment(T, m.rock, m.rho_rock, remaining_rock, energy, weight, ok);
-    if (ok) {
-      const double air_d = std::max(0.0, site.primary_altitude - exit_z) / std::max(cos_t, 1e-3);
-      air_segment(T, m.air, site, exit_z, cos_t, air_d, energy, weight, ok);
+    NodeWalk<R> o = walk_node_w(T, m, site, cells, lens, nseg, remaining_rock, exit_z, cos_t, w,
+                                shallow, ne[n], corr.enabled);
+    R fs = R(0.0);
+    if (o.ok) {
+      R c = var_correction(o.energy, o.V, o.H, cos_t, corr);
+      fs = o.weight * c * flux_gccly(cos_t, o.energy, nq[n]);
     }
-    R fs = ok ? weight * flux_gccly(cos_t, energy, nq[n]) : R(0.0);
     sum = sum + 2.0 * nw[n] * fs;
   }
-  return sum / (double)n_nodes;
+  R q = column_water_fraction(m, remaining_rock, lens, cells, nseg, w, shallow);
+  return geom_multiplier(lens, nseg, remaining_rock, cos_t, corr) * material_multiplier(q, corr) *
+         sum / (double)n_nodes;
 }
 
 }  // namespace orc

@@ -91,6 +91,10 @@ def flux_gccly(c, k, q):
     return lib().orc_flux_gccly(c, k, q)
 
 
+def dflux_gccly_dk(c, k, q):
+    return lib().orc_dflux_gccly_dk(c, k, q)
+
+
 def flux_gaisser(c, k, q):
     return lib().orc_flux_gaisser(c, k, q)
 
@@ -271,11 +275,29 @@ _SIGS["orc_w_cell_segment"] = (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.
                                       d_p, C.c_double, C.c_int, C.c_double, C.c_double, d_p])
 _SIGS["orc_w_air_segment"] = (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, C.c_int,
                                      d_p, C.c_double, C.c_double, C.c_double, C.c_double, d_p])
-_SIGS["orc_w_flux"] = (None, _W_COMMON + _W_RAYS + [d_p, C.c_int])
-_SIGS["orc_w_jacobian"] = (None, _W_COMMON + _W_RAYS + [d_p, d_p, C.c_int])
+_SIGS["orc_w_flux"] = (None, _W_COMMON + _W_RAYS + [d_p, d_p, C.c_int])
+_SIGS["orc_w_jacobian"] = (None, _W_COMMON + _W_RAYS + [d_p, d_p, d_p, C.c_int])
 _SIGS["orc_w_dflux"] = (C.c_double, _W_COMMON + [C.c_int, i_p, d_p, C.c_double, C.c_double,
                                                  C.c_double, C.c_int, d_p, u8_p, C.c_int,
-                                                 C.c_int, d_p, d_p, d_p])
+                                                 C.c_int, d_p, d_p, d_p, d_p])
+_SIGS["orc_w_node_walk"] = (None, _W_COMMON + [C.c_int, i_p, d_p, C.c_double, C.c_double,
+                                               C.c_double, d_p, u8_p, C.c_double, d_p])
+_SIGS["orc_spectral_index"] = (C.c_double, [C.c_double, C.c_double])
+_SIGS["orc_var_correction"] = (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double, d_p])
+
+
+def corr_array(enabled=True, kappa_strag=0.0, kappa_hard=0.0, resid_a=0.0, resid_b=0.0,
+               resid_c=0.0, resid_d=0.0, material=None):
+    """CsdaCorrection (src/Tomography.jl:112-137) and, if material = (linear,
+    quadratic), an enabled CsdaMaterialCorrection (:164-172), packed as the
+    oracle's 10 doubles."""
+    lin, quad = material if material is not None else (0.0, 0.0)
+    return f64([float(enabled), kappa_strag, kappa_hard, resid_a, resid_b, resid_c, resid_d,
+                float(material is not None), lin, quad])
+
+
+def _cp(corr):
+    return None if corr is None else P(corr)
 
 
 def matcfg_arrays(rock, water, air, porous, rho_rock, rho_water, rho_porous=0.0, porous_top=0.0):
@@ -314,7 +336,7 @@ def _wrays(paths):
             P(_u8(paths["valid"]), C.c_uint8)), (rp, cells, lens)
 
 
-def w_flux(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0):
+def w_flux(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0, corr=None):
     mi, md = mc
     s = f64(site)
     args, keep = _wrays(paths)
@@ -322,11 +344,12 @@ def w_flux(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0):
     ne, nw, nq = f64(ne), f64(nw), f64(nq)
     flux = np.zeros(args[0])
     lib().orc_w_flux(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s), *args, P(w),
-                     P(sh, C.c_uint8), len(ne), P(ne), P(nw), P(nq), P(flux), threads)
+                     P(sh, C.c_uint8), len(ne), P(ne), P(nw), P(nq), _cp(corr), P(flux),
+                     threads)
     return flux
 
 
-def w_jacobian(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0):
+def w_jacobian(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0, corr=None):
     """(flux[n_rays], jac[n_segments]) with repeated cells merged into their
     first occurrence."""
     mi, md = mc
@@ -337,12 +360,13 @@ def w_jacobian(ph, mc, site, paths, w, shallow, ne, nw, nq, threads=0):
     flux = np.zeros(args[0])
     jac = np.zeros(max(len(paths["cells"]), 1))
     lib().orc_w_jacobian(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s), *args,
-                         P(w), P(sh, C.c_uint8), len(ne), P(ne), P(nw), P(nq), P(flux), P(jac),
-                         threads)
+                         P(w), P(sh, C.c_uint8), len(ne), P(ne), P(nw), P(nq), _cp(corr),
+                         P(flux), P(jac), threads)
     return flux, jac[:len(paths["cells"])]
 
 
-def w_dflux(ph, mc, site, cells, lens, remaining, exit_z, cos_t, w, shallow, cell, ne, nw, nq):
+def w_dflux(ph, mc, site, cells, lens, remaining, exit_z, cos_t, w, shallow, cell, ne, nw, nq,
+            corr=None):
     """Forward-mode d flux / d w[cell] of one path."""
     mi, md = mc
     s = f64(site)
@@ -351,7 +375,32 @@ def w_dflux(ph, mc, site, cells, lens, remaining, exit_z, cos_t, w, shallow, cel
     ne, nw, nq = f64(ne), f64(nw), f64(nq)
     return lib().orc_w_dflux(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s),
                              len(cells), P(cells, C.c_int32), P(lens), remaining, exit_z, cos_t,
-                             len(w), P(w), P(sh, C.c_uint8), cell, len(ne), P(ne), P(nw), P(nq))
+                             len(w), P(w), P(sh, C.c_uint8), cell, len(ne), P(ne), P(nw), P(nq),
+                             _cp(corr))
+
+
+def w_node_walk(ph, mc, site, cells, lens, remaining, exit_z, cos_t, w, shallow, energy):
+    """One energy node: (surface energy, weight, V, H, ok)."""
+    mi, md = mc
+    s = f64(site)
+    cells, lens = i32(cells), f64(lens)
+    ns = len(cells)
+    if ns == 0:
+        cells, lens = np.zeros(1, np.int32), np.zeros(1)
+    w, sh = f64(w), _u8(shallow)
+    out = np.zeros(5)
+    lib().orc_w_node_walk(*tabargs(ph), ph.mass, ph.ctau, P(mi, C.c_int32), P(md), P(s),
+                          ns, P(cells, C.c_int32),
+                          P(lens), remaining, exit_z, cos_t, P(w), P(sh, C.c_uint8), energy, P(out))
+    return out
+
+
+def spectral_index(energy, cos_t):
+    return lib().orc_spectral_index(energy, cos_t)
+
+
+def var_correction(energy, V, H, cos_t, corr):
+    return lib().orc_var_correction(energy, V, H, cos_t, P(corr))
 
 
 # ----------------------------------------- mixed kernel / EHS scripted pins
