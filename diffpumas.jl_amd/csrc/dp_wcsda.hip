@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
       double E = active ? a.ne[n] : a.ne[0];
       double W = 1.0, V = 0.0, H = 0.0;
       const bool acc = CORR && C.enabled;
-      bool ok = true;
+      bool ok = active;  // lanes past the last sample walk nothing and add 0
       // ------------------------------------------------ forward sweep
       for (int s = 0; s < ns && ok; ++s) {
         const int c = scell[s];
