@@ -67,7 +67,6 @@ int dp_destroy(dp_ctx c) {
   (void)hipStreamSynchronize(c->stream);
   dph::release(c->scratch);
   dph::release(c->scratch2);
-  dph::release(c->scratch3);
   dph::release(c->counter);
   dph::release(c->staging);
   if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -1,36 +1,44 @@
-// Deferred evaluation of hard-event (DEL) weights.
+// Hard-event (DEL) weights pooled per block and evaluated by all its lanes.
 //
 // reverse_del_event (src/MixedTransport.jl:111-142) draws the ancestor energy
 // from a nu^-2 proposal and weights it by the differential kernel summed over
-// every element and process of the medium (4 DCS models per element).  Where
-// the weight only multiplies the event weight - the mixed step, and the
-// straggled step without scattering, where the element/process choice only
-// consumes its draw - the transport keeps going with the ancestor energy and
-// records (ancestor, q, denominator, medium).  A separate pass then evaluates
-// every record with all lanes doing the same DCS sequence, and multiplies each
-// event's factors in order.  The draws consumed, the energies and the factors
-// are those of the inline evaluation; only the product order of one event's
-// weight changes (rounding).
+// every element and process of the medium (4 DCS models per element, about
+// 2e4 FP64 operations per element).  Evaluated where it occurs, that sum runs
+// on one lane while the rest of the wave waits.  Where the weight only
+// multiplies the event weight - the mixed step, and the straggled step
+// without scattering, where the element/process choice only consumes its
+// draw - the transport keeps going with the ancestor energy and records
+// (ancestor, q, denominator, medium) in the block's LDS pool.  When every lane
+// of the block has finished its event, each lane evaluates pooled records
+// (one record per lane and round), and each lane then multiplies its own
+// factors in the order it recorded them.  The draws consumed, the energies
+// and the factors are those of the inline evaluation; only the product order
+// of one event's weight changes (rounding).  A record that finds the pool
+// full is evaluated inline.
 //
-// A record whose factor is 0 kills the event (weight 0), as in the inline
-// path; an error raised by the transport after such a record is void, so
-// errors of events with records are resolved in the same pass.
+// A factor of 0 kills the event (weight 0), as inline.  An error raised by
+// the transport after some records is void when one of those records killed
+// the event; otherwise it stands.
 #pragma once
 #include "dp_transport.hpp"
 
 namespace dpk {
 
-constexpr int DEL_MAXR = 8;  // records kept per event; more are evaluated inline
-
-struct DelRec {
-  double anc, q, denom;
-  int medium;
-  int kind;  // 0: mixed step (a non-finite weight is an error), 1: straggled (kills)
+struct PoolRec {
+  double anc, q;
+  double denom;  // replaced by the factor once evaluated
+  int owner;     // lane (thread index in the block) that recorded it
+  int mk;        // medium << 1 | kind; kind 0: mixed step (non-finite weight is an
+                 // error), 1: straggled step (non-finite weight kills the event)
 };
+
 struct DelLog {
-  DelRec* slot;  // this event's DEL_MAXR slots
-  int n;         // records used
-  int medium;    // medium index of the current step
+  PoolRec* pool;  // the block's pool (LDS)
+  int* count;     // records reserved so far in the block (LDS)
+  int cap;        // pool capacity
+  int owner;      // this lane
+  int n;          // records this lane's event has pooled
+  int medium;     // medium index of the current step
 };
 
 struct DelSample {
@@ -65,9 +73,18 @@ __device__ __forceinline__ DelSample del_sample(const DevKernel& c, double mass,
   return d;
 }
 
-__device__ __forceinline__ void del_push(DelLog* log, const DelSample& d, double k, int kind) {
-  DelRec& r = log->slot[log->n++];
-  r.anc = d.anc; r.q = d.anc - k; r.denom = d.denom; r.medium = log->medium; r.kind = kind;
+// Reserves a pool slot; false when the pool is full (evaluate inline).
+__device__ __forceinline__ bool del_push(DelLog* log, const DelSample& d, double k, int kind) {
+  const int i = atomicAdd(log->count, 1);
+  if (i >= log->cap) return false;
+  PoolRec& r = log->pool[i];
+  r.anc = d.anc;
+  r.q = d.anc - k;
+  r.denom = d.denom;
+  r.owner = log->owner;
+  r.mk = log->medium << 1 | kind;
+  ++log->n;
+  return true;
 }
 
 // differential_kernel (src/MixedTransport.jl:107-109) in component order.
@@ -79,68 +96,39 @@ __device__ __noinline__ static double del_total(const DevKernel& c, double mass,
   return total;
 }
 
-// Records of event e: list[off[e] + j] = e * DEL_MAXR + j.
-static __global__ void del_list_kernel_i(int64_t n, const int* __restrict__ nrec,
-                                  const int* __restrict__ off, int64_t* __restrict__ list) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  for (int j = 0; j < nrec[e]; ++j) list[off[e] + j] = e * DEL_MAXR + j;
-}
-
-// factor of each record (persistent grid over the record list).
-static __global__ __launch_bounds__(128, 4) void del_eval_kernel(const int64_t* __restrict__ count,
-                                                       const int64_t* __restrict__ list,
-                                                       const DelRec* __restrict__ rec,
-                                                       const Medium* __restrict__ media,
-                                                       double mass, double* __restrict__ fac,
-                                                       int* __restrict__ kind) {
-  const int64_t total = *count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
-    const int64_t slot = list[t];
-    const DelRec r = rec[slot];
-    const DevKernel& c = media[r.medium].kern;
-    fac[slot] = del_total(c, mass, r.anc, r.q) / r.denom;
-    kind[slot] = r.kind;
+// All threads of the block, after a barrier that follows the last push:
+// factor of every pooled record.  kern(m) gives the DevKernel of medium m.
+template <class KernOf>
+__device__ __forceinline__ void pool_eval(PoolRec* pool, int m, double mass, KernOf kern) {
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    PoolRec& r = pool[j];
+    r.denom = del_total(kern(r.mk >> 1), mass, r.anc, r.q) / r.denom;
   }
 }
 
-// Apply each event's factors in order.  err_pend[e] holds an error raised
-// after err_nrec[e] records; it stands unless one of those records killed
-// the event.  e0 is the global index of event 0 (error reports).
-static __global__ void del_apply_kernel_i(int64_t n, const int* __restrict__ nrec,
-                                 const double* __restrict__ fac, const int* __restrict__ kind,
-                                 const uint8_t* __restrict__ err_pend,
-                                 const uint8_t* __restrict__ err_nrec, double* __restrict__ out_w,
-                                 int* __restrict__ err, unsigned long long* __restrict__ bad,
-                                 int64_t e0) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int m = nrec[e];
-  if (m == 0) return;
-  double w = out_w[e];
-  const int pend = err_pend[e];
-  const int upto = pend ? err_nrec[e] : m;
+// After a barrier that follows pool_eval: this lane's factors in recording
+// order applied to its event weight w.  pend is the error the transport
+// raised after the lane's nown records (0: none).  Returns the final weight
+// and sets fail to the error that stands.
+__device__ __forceinline__ double pool_apply(const PoolRec* pool, int m, int owner, int nown,
+                                             double w, int pend, int& fail) {
   bool dead = false;
-  int fail = 0;
-  for (int j = 0; j < upto; ++j) {
-    const double f = fac[e * DEL_MAXR + j];
+  fail = 0;
+  int seen = 0;
+  for (int j = 0; j < m && seen < nown; ++j) {
+    if (pool[j].owner != owner) continue;
+    ++seen;
+    const double f = pool[j].denom;
     if (f == 0.0) { dead = true; break; }
     w *= f;
     if (!(isfinite(w) && w >= 0)) {
-      if (kind[e * DEL_MAXR + j]) dead = true;
+      if (pool[j].mk & 1) dead = true;
       else fail = ERR_TRANSPORT;
       break;
     }
   }
   if (!dead && !fail && pend) fail = pend;
-  if (fail) {
-    atomicOr(err, fail);
-    atomicMin(bad, (unsigned long long)(e0 + e));
-    out_w[e] = 0.0;
-  } else {
-    out_w[e] = dead ? 0.0 : w;
-  }
+  return (fail || dead) ? 0.0 : w;
 }
 
 }  // namespace dpk

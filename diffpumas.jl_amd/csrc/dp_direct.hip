@@ -1,0 +1,185 @@
+// One-shot CSDA flux of a flat rock slab under the atmosphere and its
+// rock-density tangent for many samples: compute_flux_csda_direct and
+// compute_flux_gradient_csda (src/Geometry.jl:1126-1262).
+//
+// One sample per lane in a grid-stride loop.  The rock and air energy,
+// log-energy, stopping-power and range arrays are staged in LDS (8 x 193
+// doubles); a sample reads 40 B (density, thickness, elevation, energy,
+// charge) and writes 16 B (flux, d flux / d density), so the kernel is bound by
+// those streams and by the FP64 transcendental count (two range inversions,
+// six lookups, two decays, one flux).
+#include "dp_internal.hpp"
+#include "dp_device.hpp"
+#include <cmath>
+
+using namespace dpk;
+
+namespace {
+
+struct DirectArgs {
+  const double* tab;
+  int K, pow2;
+  int rock, air;
+  double mass, ctau, palt;
+  int64_t n;
+  const double* rho;
+  const double* thick;
+  const double* elev;
+  const double* energy;
+  const double* charge;
+  double* flux;
+  double* dflux;
+};
+
+// compute_decay_weight_from_path, src/Geometry.jl:1083-1107 (the beta floor
+// branch is chosen on the primal value).
+template <class R>
+__device__ __forceinline__ R decay_direct(double mass, double ctau, double path, R eavg) {
+  R g = 1.0 + eavg / mass;
+  R b2 = 1.0 - 1.0 / (g * g);
+  R beta = sqrt(val(b2) >= 1e-12 ? b2 : R(1e-12));
+  R L = beta * g * ctau;
+  return exp(-path / L);
+}
+
+__global__ __launch_bounds__(256) void csda_direct_kernel(DirectArgs a) {
+  extern __shared__ double lds[];  // rock, then air: E, log E, S_csda, R_csda
+  const int K = a.K;
+  {
+    const int mats[2] = {a.rock, a.air};
+    for (int j = 0; j < 2; ++j) {
+      const double* t = a.tab + (size_t)mats[j] * 19 * K;
+      double* d = lds + 4 * j * K;
+      for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        d[i] = t[i];
+        d[K + i] = t[K + i];
+        d[2 * K + i] = t[2 * K + i];
+        d[3 * K + i] = t[3 * K + i];
+      }
+    }
+  }
+  __syncthreads();
+  const double *rE = lds, *rLE = lds + K, *rS = lds + 2 * K, *rR = lds + 3 * K;
+  const double *aE = lds + 4 * K, *aLE = lds + 5 * K, *aS = lds + 6 * K, *aR = lds + 7 * K;
+  const int p2 = a.pow2;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+    const double e0 = a.energy[i], thick = a.thick[i];
+    const DualD rho(a.rho[i], 1.0);
+    const double el = a.elev[i] * (M_PI / 180.0);
+    const double cos_t = cos(M_PI / 2 - el);
+    const double cs = fmax(fabs(cos_t), 0.01);
+    double f = 0.0, df = 0.0;
+    // rock: compute_rock_grammage (:1064-1068), range inversion
+    const DualD x_rock = (thick / cs) * rho;
+    const double le0 = log(e0);
+    const double x0 = interp_r(rE, rLE, rR, K, p2, e0, le0);
+    const DualD e1 = inverse_range_r(rE, rLE, rR, K, p2, x0 + x_rock);
+    if (e1.v < 1e12 && isfinite(e1.v)) {
+      const double s0 = interp_r(rE, rLE, rS, K, p2, e0, le0);
+      const DualD le1 = log(e1);
+      const DualD s1 = interp_r(rE, rLE, rS, K, p2, e1, le1);
+      const DualD w_rock = s1 / s0;
+      const DualD d_rock = decay_direct(a.mass, a.ctau, thick / cs, (e0 + e1) / 2.0);
+      // atmosphere: compute_air_grammage (:1043-1055), range inversion
+      double x_air = (1.205 * 12e3 / cs) * (exp(-thick / 12e3) - exp(-a.palt / 12e3));
+      x_air = fmax(x_air, 0.0);
+      const DualD la1 = log(e1);
+      const DualD xa0 = interp_r(aE, aLE, aR, K, p2, e1, la1);
+      const DualD e2 = inverse_range_r(aE, aLE, aR, K, p2, xa0 + x_air);
+      if (e2.v < 1e12 && isfinite(e2.v)) {
+        const DualD sa0 = interp_r(aE, aLE, aS, K, p2, e1, la1);
+        const DualD sa1 = interp_r(aE, aLE, aS, K, p2, e2, log(e2));
+        const DualD w_air = sa1 / sa0;
+        const DualD d_air = decay_direct(a.mass, a.ctau, (a.palt - thick) / cs, (e1 + e2) / 2.0);
+        const DualD w = w_rock * d_rock * w_air * d_air;
+        const DualD fl = w * flux_gccly_r(cos_t, e2, a.charge[i]);
+        f = fl.v;
+        df = fl.d;
+      }
+    }
+    a.flux[i] = f;
+    if (a.dflux) a.dflux[i] = df;
+  }
+}
+
+int check_direct(const char* fn, dp_tables t, int32_t rock, int32_t air, int64_t n,
+                 const double* rho, const double* thick, const double* elev, const double* energy,
+                 const double* charge, const double* flux) {
+  const std::string f(fn);
+  if (!t || (n > 0 && (!rho || !thick || !elev || !energy || !charge || !flux)))
+    return dph::fail(DP_E_ARGUMENT, f + ": null argument");
+  if (n < 0) return dph::fail(DP_E_ARGUMENT, f + ": negative sample count");
+  if (rock < 0 || rock >= t->n_materials || air < 0 || air >= t->n_materials)
+    return dph::fail(DP_E_ARGUMENT, f + ": material index out of range");
+  return DP_OK;
+}
+
+int launch_direct(dp_ctx c, dp_tables t, int32_t rock, int32_t air, double palt, int64_t n,
+                  const double* rho, const double* thick, const double* elev, const double* energy,
+                  const double* charge, double* flux, double* dflux) {
+  const size_t lds = (size_t)8 * t->n_knots * sizeof(double);
+  int occ = 0;
+  DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_direct_kernel, 256, lds));
+  if (occ < 1) occ = 1;
+  int64_t grid = (int64_t)c->n_cu * occ * 2;
+  const int64_t want = (n + 255) / 256;
+  if (grid > want) grid = want;
+  if (grid < 1) grid = 1;
+  DirectArgs a;
+  a.tab = t->data; a.K = t->n_knots; a.pow2 = t->pow2;
+  a.rock = rock; a.air = air; a.mass = t->mass; a.ctau = t->ctau; a.palt = palt;
+  a.n = n; a.rho = rho; a.thick = thick; a.elev = elev; a.energy = energy; a.charge = charge;
+  a.flux = flux; a.dflux = dflux;
+  (void)hipEventRecord(c->ev0, c->stream);
+  hipLaunchKernelGGL(csda_direct_kernel, dim3((unsigned)grid), dim3(256), lds, c->stream, a);
+  (void)hipEventRecord(c->ev1, c->stream);
+  DP_HIP(hipGetLastError());
+  return DP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dp_csda_direct_dev(dp_ctx c, dp_tables t, int32_t rock, int32_t air, double palt, int64_t n,
+                       const double* rho, const double* thick, const double* elev,
+                       const double* energy, const double* charge, double* flux, double* dflux) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_csda_direct_dev: null context");
+  int rc = check_direct("dp_csda_direct_dev", t, rock, air, n, rho, thick, elev, energy, charge, flux);
+  if (rc || n == 0) return rc;
+  DP_HIP(hipSetDevice(c->device));
+  return launch_direct(c, t, rock, air, palt, n, rho, thick, elev, energy, charge, flux, dflux);
+}
+
+int dp_csda_direct(dp_ctx c, dp_tables t, int32_t rock, int32_t air, double palt, int64_t n,
+                   const double* rho, const double* thick, const double* elev,
+                   const double* energy, const double* charge, double* flux, double* dflux) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_csda_direct: null context");
+  int rc = check_direct("dp_csda_direct", t, rock, air, n, rho, thick, elev, energy, charge, flux);
+  if (rc || n == 0) return rc;
+  DP_HIP(hipSetDevice(c->device));
+  const size_t b8 = 8 * (size_t)n;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
+  const size_t o_in[5] = {take(b8), take(b8), take(b8), take(b8), take(b8)};
+  const size_t o_f = take(b8), o_d = take(b8);
+  rc = dph::ensure(c->staging, o);
+  if (rc) return rc;
+  char* b = (char*)c->staging.ptr;
+  hipStream_t st = c->stream;
+  const double* src[5] = {rho, thick, elev, energy, charge};
+  for (int k = 0; k < 5; ++k)
+    DP_HIP(hipMemcpyAsync(b + o_in[k], src[k], b8, hipMemcpyHostToDevice, st));
+  rc = launch_direct(c, t, rock, air, palt, n, (double*)(b + o_in[0]), (double*)(b + o_in[1]),
+                     (double*)(b + o_in[2]), (double*)(b + o_in[3]), (double*)(b + o_in[4]),
+                     (double*)(b + o_f), dflux ? (double*)(b + o_d) : nullptr);
+  if (rc) return rc;
+  DP_HIP(hipMemcpyAsync(flux, b + o_f, b8, hipMemcpyDeviceToHost, st));
+  if (dflux) DP_HIP(hipMemcpyAsync(dflux, b + o_d, b8, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipStreamSynchronize(st));
+  (void)hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1);
+  return DP_OK;
+}
+
+}  // extern "C"

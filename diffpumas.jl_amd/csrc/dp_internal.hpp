@@ -69,7 +69,6 @@ struct dp_ctx_s {
   float last_ms = 0.f;
   dph::DevBuf scratch;       // kernel scratch (partials, queues)
   dph::DevBuf scratch2;      // second scratch (sort buffers)
-  dph::DevBuf scratch3;      // third scratch (deferred hard-event records)
   dph::DevBuf counter;       // small: work-queue counters
   dph::DevBuf staging;       // host-API staging of inputs/outputs
   dp_csda_correction corr{};            // w-model corrections (off)

@@ -17,7 +17,6 @@
 #include "dp_del_defer.hpp"
 #include "dp_media.hpp"
 #include <algorithm>
-#include <hipcub/hipcub.hpp>
 #include <cmath>
 
 using namespace dpk;
@@ -26,11 +25,10 @@ namespace {
 
 constexpr int MC_ARRS = 5;  // energy, log energy, csda range, csda stopping, transport path
 
-// Event sorting by source energy and deferred hard-event weights are
-// switched off: the first GPU run with both enabled ended in a device memory
-// fault whose cause is not identified yet.
-constexpr bool kSortEvents = false;
-constexpr bool kDeferDel = false;
+// Stochastic MC blocks: one wave, with a pool of MC_POOL deferred hard-event
+// records evaluated by the whole wave after each round of 64 events.
+constexpr int MC_THREADS = 64;
+constexpr int MC_POOL = 192;
 
 struct FlatGeo {
   double thick, rho;
@@ -219,28 +217,8 @@ struct StochArgs {
   double* out_w;
   int* err;            // [0] error bits, [1] spare
   unsigned long long* bad_event;
-  const uint32_t* order;  // optional processing order of the events (by energy)
-  // deferred DEL weights (null: evaluated inline)
-  DelRec* del_rec;
-  int* nrec;
-  uint8_t* err_pend;
-  uint8_t* err_nrec;
 };
 
-// Sort keys: the source energy of each event (first Philox draw), as the
-// bit pattern of a positive double (monotonic as an unsigned integer).
-__global__ void mc_energy_keys(uint64_t seed, int64_t n, double emin, double emax,
-                               uint64_t* __restrict__ keys, uint32_t* __restrict__ ids) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  Philox rng;
-  rng.init(seed, (uint64_t)e);
-  const double u = rng.uniform();
-  const double rk = log(emax / emin);
-  const double k = rk > 0 ? emin * exp(rk * u) : emin;
-  keys[e] = (uint64_t)__double_as_longlong(k);
-  ids[e] = (uint32_t)e;
-}
 
 // transport_backward_through_geometry with an RNG, src/Geometry.jl:534-672,
 // and the acceptance of compute_flux_single (:739-757).
@@ -386,35 +364,58 @@ __device__ double flat_event_f(const StochArgs& a, double energy, double charge,
   return 0.0;
 }
 
+// One event per lane, rounds of MC_THREADS events per block (the loop bound is
+// uniform over the block).  On tables with a shared energy grid (FAST) the
+// deferrable hard-event weights of a round go to the block's pool and are
+// evaluated by all its lanes once every lane has finished its event.
 template <bool FAST>
-__global__ __launch_bounds__(128, 4) void mc_flat_stochastic_kernel(StochArgs a) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += stride) {
-    const int64_t e = a.order ? (int64_t)a.order[p] : p;
-    Philox rng;
-    rng.init(a.seed, (uint64_t)e);
-    const double u = rng.uniform();
-    double k, w;
-    const double rk = log(a.emax / a.emin);
-    if (rk > 0) { k = a.emin * exp(rk * u); w = k * rk; }
-    else { k = a.emin; w = 1.0; }
-    const double q = rng.uniform() > 0.5 ? 1.0 : -1.0;
-    w *= 2.0;
-    int err = 0;
-    DelLog log{a.del_rec ? a.del_rec + e * DEL_MAXR : nullptr, 0, 0};
-    const double f = FAST ? flat_event_f(a, k, q, rng, err, a.del_rec ? &log : nullptr)
-                          : flat_event(a, k, q, rng, err);
-    if (a.nrec) {
-      a.nrec[e] = log.n;
-      a.err_pend[e] = (uint8_t)(log.n > 0 ? err : 0);
-      a.err_nrec[e] = (uint8_t)log.n;
+__global__ __launch_bounds__(MC_THREADS, 4) void mc_flat_stochastic_kernel(StochArgs a) {
+  __shared__ PoolRec pool[FAST ? MC_POOL : 1];
+  __shared__ int pool_n;
+  const int tid = threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * MC_THREADS;
+  for (int64_t base = (int64_t)blockIdx.x * MC_THREADS; base < a.n; base += stride) {
+    if (FAST) {
+      if (tid == 0) pool_n = 0;
+      __syncthreads();  // the previous round's records are applied; the pool is empty
     }
-    if (err && log.n == 0) {
-      atomicOr(a.err, err);
-      atomicMin(a.bad_event, (unsigned long long)e);
-      a.out_w[e] = 0.0;
-    } else {
-      a.out_w[e] = err ? 0.0 : w * f;
+    const int64_t e = base + tid;
+    const bool active = e < a.n;
+    double v = 0.0;
+    int err = 0;
+    DelLog dlg{pool, &pool_n, MC_POOL, tid, 0, 0};
+    if (active) {
+      Philox rng;
+      rng.init(a.seed, (uint64_t)e);
+      const double u = rng.uniform();
+      double k, w;
+      const double rk = log(a.emax / a.emin);
+      if (rk > 0) { k = a.emin * exp(rk * u); w = k * rk; }
+      else { k = a.emin; w = 1.0; }
+      const double q = rng.uniform() > 0.5 ? 1.0 : -1.0;
+      w *= 2.0;
+      const double f = FAST ? flat_event_f(a, k, q, rng, err, &dlg) : flat_event(a, k, q, rng, err);
+      v = err ? 0.0 : w * f;
+    }
+    int m = 0;
+    if (FAST) {
+      __syncthreads();  // every record of the round is in the pool
+      m = min(pool_n, MC_POOL);
+      pool_eval(pool, m, a.T.mass, [&](int md) -> const DevKernel& {
+        return md == 0 ? a.rock.kern : (md == 1 ? a.air.kern : a.porous.kern);
+      });
+      __syncthreads();  // every factor is in the pool
+    }
+    if (active) {
+      int fail = err;
+      if (FAST && dlg.n > 0) v = pool_apply(pool, m, tid, dlg.n, v, err, fail);
+      if (fail) {
+        atomicOr(a.err, fail);
+        atomicMin(a.bad_event, (unsigned long long)e);
+        a.out_w[e] = 0.0;
+      } else {
+        a.out_w[e] = v;
+      }
     }
   }
 }
@@ -690,61 +691,6 @@ int reduce_pair(hipStream_t st, const double* x, const double* y, int64_t n, boo
 
 __global__ void reset_flags(int* f) { if (threadIdx.x < 4) f[threadIdx.x] = 0; }
 
-__global__ void del_count_kernel(int64_t n, const int* nrec, const int* off, int64_t* count) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *count = n > 0 ? (int64_t)off[n - 1] + nrec[n - 1] : 0;
-}
-
-}  // namespace
-
-namespace dph {
-// Carves the buffers out of the context's third scratch buffer.
-int del_buffers(dp_ctx c, int64_t n, DelBuffers& b) {
-  const size_t m = (size_t)n * DEL_MAXR;
-  size_t tmp = 0;
-  DP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (int*)nullptr, (int*)nullptr, (int)n,
-                                          c->stream));
-  size_t o = 0;
-  auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes, 256); return at; };
-  const size_t o_rec = take(sizeof(DelRec) * m), o_nrec = take(4 * (size_t)n),
-               o_off = take(4 * (size_t)n), o_ep = take((size_t)n), o_en = take((size_t)n),
-               o_list = take(8 * m), o_fac = take(8 * m), o_kind = take(4 * m), o_cnt = take(8),
-               o_med = take(sizeof(Medium) * 4), o_tmp = take(tmp);
-  int rc = ensure(c->scratch3, o);
-  if (rc) return rc;
-  char* p = (char*)c->scratch3.ptr;
-  b.rec = (DelRec*)(p + o_rec); b.nrec = (int*)(p + o_nrec); b.off = (int*)(p + o_off);
-  b.err_pend = (uint8_t*)(p + o_ep); b.err_nrec = (uint8_t*)(p + o_en);
-  b.list = (int64_t*)(p + o_list); b.fac = (double*)(p + o_fac); b.kind = (int*)(p + o_kind);
-  b.count = (int64_t*)(p + o_cnt); b.media = (Medium*)(p + o_med);
-  b.tmp = p + o_tmp; b.tmp_bytes = tmp;
-  return DP_OK;
-}
-
-// Evaluates every record and applies each event's factors (stream ordered).
-int del_resolve(dp_ctx c, int64_t n, DelBuffers& b, const Medium* media, double mass,
-                double* out_w, int* err, unsigned long long* bad, int64_t e0) {
-  hipStream_t st = c->stream;
-  DP_HIP(hipcub::DeviceScan::ExclusiveSum(b.tmp, b.tmp_bytes, b.nrec, b.off, (int)n, st));
-  const unsigned gb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(del_count_kernel, dim3(1), dim3(64), 0, st, n, b.nrec, b.off, b.count);
-  hipLaunchKernelGGL(del_list_kernel_i, dim3(gb), dim3(256), 0, st, n, b.nrec, b.off, b.list);
-  int occ = 0;
-  DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, del_eval_kernel, 128, 0));
-  if (occ < 1) occ = 1;
-  hipLaunchKernelGGL(del_eval_kernel, dim3((unsigned)(c->n_cu * occ)), dim3(128), 0, st, b.count,
-                     b.list, b.rec, media, mass, b.fac, b.kind);
-  hipLaunchKernelGGL(del_apply_kernel_i, dim3(gb), dim3(256), 0, st, n, b.nrec, b.fac, b.kind,
-                     b.err_pend, b.err_nrec, out_w, err, bad, e0);
-  DP_HIP(hipGetLastError());
-  return DP_OK;
-}
-}  // namespace dph
-
-namespace {
-using dph::DelBuffers;
-using dph::del_buffers;
-using dph::del_resolve;
-
 }  // namespace
 
 namespace {
@@ -812,7 +758,6 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
   a.out_w = stochastic ? nullptr : out_w;
   a.out_gf = (double*)(b + o_gf); a.out_gd = (double*)(b + o_gd);
   a.err = flags;
-  Medium hm[3];  // host copy of the media, read by an async copy before the final sync
   if (stochastic) {
     StochArgs s;
     s.T = {t->data, t->scalars, t->n_knots, t->pow2, t->mass, t->ctau, t->elastic_ratio};
@@ -834,51 +779,18 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     s.thr = thr; s.sw = cfg->mode_switch;
     s.straggling = cfg->straggling != 0; s.scattering = cfg->scattering != 0;
     s.n = n; s.seed = cfg->seed; s.out_w = out_w; s.err = flags; s.bad_event = bad;
-    s.del_rec = nullptr; s.nrec = nullptr; s.err_pend = nullptr; s.err_nrec = nullptr;
-    // Process events in order of source energy: lanes of a wave then share
-    // the transport mode and a similar number of steps.  Each event keeps
-    // its own Philox stream and output slot, so results do not change.
     (void)hipEventRecord(c->ev0, c->stream);
-    s.order = nullptr;
-    if (kSortEvents && n < (int64_t)0x7fffffff) {
-      size_t tmp = 0;
-      DP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 40, 64,
-                                                c->stream));
-      size_t q0 = 0, q1 = dph::align_up(8 * (size_t)n, 256), q2 = dph::align_up(q1 + 8 * (size_t)n, 256);
-      size_t q3 = dph::align_up(q2 + 4 * (size_t)n, 256), q4 = dph::align_up(q3 + 4 * (size_t)n, 256);
-      if ((rc = dph::ensure(c->scratch2, q4 + tmp + 256))) return rc;
-      char* sb = (char*)c->scratch2.ptr;
-      uint64_t *k0 = (uint64_t*)(sb + q0), *k1 = (uint64_t*)(sb + q1);
-      uint32_t *i0 = (uint32_t*)(sb + q2), *i1 = (uint32_t*)(sb + q3);
-      hipLaunchKernelGGL(mc_energy_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                         cfg->seed, n, cfg->energy_min, cfg->energy_max, k0, i0);
-      DP_HIP(hipGetLastError());
-      DP_HIP(hipcub::DeviceRadixSort::SortPairs(sb + q4, tmp, k0, k1, i0, i1, (int)n, 40, 64,
-                                                c->stream));
-      s.order = i1;
-    }
     const bool fast = t->shared_grid;
     int occ = 0;
-    if (fast) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<true>, 128, 0));
-    else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<false>, 128, 0));
+    if (fast) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<true>, MC_THREADS, 0));
+    else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<false>, MC_THREADS, 0));
     if (occ < 1) occ = 1;
-    int64_t want = (n + 127) / 128;
+    const int64_t want = (n + MC_THREADS - 1) / MC_THREADS;
     int64_t grid = (int64_t)c->n_cu * occ * 4;
     if (grid > want) grid = want;
-    // Deferred DEL records (fast path, fewer than 2^25 events).
-    DelBuffers db;
-    const bool defer = kDeferDel && fast && n < ((int64_t)1 << 25);
-    hm[0] = s.rock; hm[1] = s.air; hm[2] = s.porous;
-    if (defer) {
-      if ((rc = del_buffers(c, n, db))) return rc;
-      s.del_rec = db.rec; s.nrec = db.nrec; s.err_pend = db.err_pend; s.err_nrec = db.err_nrec;
-      DP_HIP(hipMemcpyAsync(db.media, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
-    }
-    if (fast) hipLaunchKernelGGL(mc_flat_stochastic_kernel<true>, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
-    else hipLaunchKernelGGL(mc_flat_stochastic_kernel<false>, dim3((unsigned)grid), dim3(128), 0, c->stream, s);
+    if (fast) hipLaunchKernelGGL(mc_flat_stochastic_kernel<true>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
+    else hipLaunchKernelGGL(mc_flat_stochastic_kernel<false>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
     DP_HIP(hipGetLastError());
-    if (defer && (rc = del_resolve(c, n, db, db.media, t->mass, out_w, flags, bad))) return rc;
     (void)hipEventRecord(c->ev1, c->stream);
   }
   if (!stochastic || a.grad) {

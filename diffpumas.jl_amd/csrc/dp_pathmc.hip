@@ -5,9 +5,11 @@
 // an atomic queue, so the lanes of a block walk the same segment list (cell,
 // length, medium and density loads are wave-uniform) and paths of any length
 // balance across the chip.  One event per lane; Philox keyed by
-// (seed, path * n_samples + sample).  Each item writes (sum w, sum w^2) of its
-// chunk through a fixed wave/LDS tree; a second kernel adds a path's chunks
-// in order, so flux and sigma do not depend on the launch shape.
+// (seed, path * n_samples + sample).  Hard-event weights of an item go to the
+// block's pool and are evaluated by all 128 lanes once the item's events are
+// done.  Each item writes (sum w, sum w^2) of its chunk through a fixed
+// wave/LDS tree; a second kernel adds a path's chunks in order, so flux and
+// sigma do not depend on the launch shape.
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
 #include "dp_transport.hpp"
@@ -21,6 +23,7 @@ using namespace dpk;
 namespace {
 
 constexpr int PM_THREADS = 128;
+constexpr int PM_POOL = 384;  // pooled hard-event records per block
 
 struct PathArgs {
   DevTables T;
@@ -53,10 +56,6 @@ struct PathArgs {
   int* queue;
   int* err;
   unsigned long long* bad;
-  DelRec* del_rec;     // deferred DEL records (null: inline)
-  int* nrec;
-  uint8_t* err_pend;
-  uint8_t* err_nrec;
 };
 
 // propagate_mc_uniform_segment / propagate_mc_air_segment, src/Tomography.jl:
@@ -123,9 +122,14 @@ __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, d
 template <bool FAST>
 __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
   __shared__ int64_t s_item;
+  __shared__ PoolRec pool[FAST ? PM_POOL : 1];
+  __shared__ int pool_n;
   const int tid = threadIdx.x;
   for (;;) {
-    if (tid == 0) s_item = atomicAdd((unsigned long long*)a.queue, 1ull);
+    if (tid == 0) {
+      s_item = atomicAdd((unsigned long long*)a.queue, 1ull);
+      pool_n = 0;
+    }
     __syncthreads();
     const int64_t item = s_item;
     if (item >= a.n_items) break;
@@ -133,66 +137,72 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
     const int sl = (int)(item % a.chunks) * PM_THREADS + tid;
     const double ct = a.cos_t[r];
     const bool live = (a.valid ? a.valid[r] != 0 : true) && ct > 0.0;
-    double wi = 0.0;
+    const bool ran = live && sl < a.n_samples;
     if (!live && sl < a.n_samples) {
       const int64_t el = (int64_t)(r - a.r0) * a.n_samples + (int)a.order[sl];
       a.w_local[el] = 0.0;
-      if (a.nrec) { a.nrec[el] = 0; a.err_pend[el] = 0; a.err_nrec[el] = 0; }
     }
-    if (live && sl < a.n_samples) {
+    double wi = 0.0;
+    int err = 0;
+    int64_t e = 0, el = 0;
+    DelLog dlg{pool, &pool_n, PM_POOL, tid, 0, 0};
+    if (ran) {
       const int s = (int)a.order[sl];
-      const int64_t e = (int64_t)r * a.n_samples + s;
+      e = (int64_t)r * a.n_samples + s;
+      el = (int64_t)(r - a.r0) * a.n_samples + s;  // event index in this launch
       Philox rng;
       rng.init(a.seed, (uint64_t)e);
       McState st;
       st.E = a.se[s]; st.w = 1.0; st.t = 0.0; st.L = 0.0;
       st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
       st.dir[0] = -sqrt(fmax(0.0, 1.0 - ct * ct)); st.dir[1] = 0.0; st.dir[2] = -ct;
-      int err = 0, g = a.T.K / 2;
-      const int64_t el = (int64_t)(r - a.r0) * a.n_samples + s;  // event index in this launch
-      DelLog log{a.del_rec ? a.del_rec + el * DEL_MAXR : nullptr, 0, 0};
-      DelLog* dl = (FAST && a.del_rec) ? &log : nullptr;
+      int g = a.T.K / 2;
+      DelLog* dl = (FAST && a.straggling) ? &dlg : nullptr;
       const int b0 = a.row_ptr[r], ns = a.row_ptr[r + 1] - b0;
       for (int k = 0; k < ns; ++k) {
         const int c = a.cell[b0 + k];
-        log.medium = a.cell_medium[c];
-        path_segment<FAST>(a, a.media[log.medium], st, a.cell_density[c], false, ct,
+        dlg.medium = a.cell_medium[c];
+        path_segment<FAST>(a, a.media[dlg.medium], st, a.cell_density[c], false, ct,
                            a.len[b0 + k], rng, err, g, dl);
         if (err || st.w <= 0.0 || !isfinite(st.w)) break;
       }
       if (!err && st.w > 0.0 && isfinite(st.w) && a.remaining[r] > 1e-8) {
-        log.medium = a.rock_medium;
+        dlg.medium = a.rock_medium;
         path_segment<FAST>(a, a.media[a.rock_medium], st, a.rho_rock, false, ct, a.remaining[r],
                            rng, err, g, dl);
       }
       if (!err && st.w > 0.0 && isfinite(st.w)) {
         const double air_d = fmax(0.0, a.palt - a.exit_z[r]) / fmax(ct, 1e-3);
-        log.medium = a.air_medium;
+        dlg.medium = a.air_medium;
         path_segment<FAST>(a, a.media[a.air_medium], st, 0.0, true, ct, air_d, rng, err, g, dl);
       }
       double f = 0.0;
       if (!err && st.w > 0.0 && isfinite(st.w) && st.E < a.elimit)
         f = st.w * flux_gccly(ct, st.E, a.sq[s]);
-      wi = 2.0 * a.sw[s] * f;
-      if (a.nrec) {
-        a.nrec[el] = log.n;
-        a.err_pend[el] = (uint8_t)(log.n > 0 ? err : 0);
-        a.err_nrec[el] = (uint8_t)log.n;
-      }
-      if (err) {
-        if (log.n == 0) {
-          atomicOr(a.err, err);
-          atomicMin(a.bad, (unsigned long long)e);
-        }
+      wi = err ? 0.0 : 2.0 * a.sw[s] * f;
+    }
+    int m = 0;
+    if (FAST) {
+      __syncthreads();  // every record of the item is in the pool
+      m = min(pool_n, PM_POOL);
+      pool_eval(pool, m, a.T.mass, [&](int md) -> const DevKernel& { return a.media[md].kern; });
+      __syncthreads();  // every factor is in the pool
+    }
+    if (ran) {
+      int fail = err;
+      if (FAST && dlg.n > 0) wi = pool_apply(pool, m, tid, dlg.n, wi, err, fail);
+      if (fail) {
+        atomicOr(a.err, fail);
+        atomicMin(a.bad, (unsigned long long)e);
         wi = 0.0;
       }
       a.w_local[el] = wi;
     }
-    __syncthreads();  // every thread has read s_item before thread 0 takes the next item
+    __syncthreads();  // every thread has read s_item and the pool before thread 0 resets them
   }
 }
 
-// (sum w, sum w^2) of each item's samples after the deferred weights are in,
+// (sum w, sum w^2) of each item's samples,
 // in the sorted sample order of the item (fixed tree).
 __global__ __launch_bounds__(PM_THREADS) void path_items_kernel(PathArgs a) {
   __shared__ double s_sum[2][PM_THREADS / 64];
@@ -379,22 +389,14 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, path_mc_kernel<false>, PM_THREADS, 0));
   if (occ < 1) occ = 1;
   // Paths are processed in batches of at most 2^22 events, which bounds the
-  // memory of the deferred hard-event records.
+  // per-event weight buffer.
   const int64_t batch_ev = (int64_t)1 << 22;
   int rays_per_batch = (int)std::max<int64_t>(1, batch_ev / nn);
   if (rays_per_batch > nr) rays_per_batch = nr;
   const int64_t ev_batch = (int64_t)rays_per_batch * nn;
-  dph::DelBuffers db;
-  // deferred hard-event weights are switched off (see the flat MC)
-  const bool defer = false && fast && straggling && ev_batch < ((int64_t)1 << 25);
-  if (defer && (rc = dph::del_buffers(c, ev_batch, db))) return rc;
   rc = dph::ensure(c->scratch, 8 * (size_t)ev_batch + 256);
   if (rc) return rc;
   a.w_local = (double*)c->scratch.ptr;
-  a.del_rec = defer ? db.rec : nullptr;
-  a.nrec = defer ? db.nrec : nullptr;
-  a.err_pend = defer ? db.err_pend : nullptr;
-  a.err_nrec = defer ? db.err_nrec : nullptr;
   (void)hipEventRecord(c->ev0, st);
   for (int r0 = 0; r0 < nr; r0 += rays_per_batch) {
     const int r1 = std::min(nr, r0 + rays_per_batch);
@@ -406,9 +408,6 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
     if (fast) hipLaunchKernelGGL(path_mc_kernel<true>, dim3((unsigned)grid), dim3(PM_THREADS), 0, st, a);
     else hipLaunchKernelGGL(path_mc_kernel<false>, dim3((unsigned)grid), dim3(PM_THREADS), 0, st, a);
     DP_HIP(hipGetLastError());
-    if (defer && (rc = dph::del_resolve(c, (int64_t)(r1 - r0) * nn, db, a.media, t->mass,
-                                        a.w_local, a.err, a.bad, (int64_t)r0 * nn)))
-      return rc;
     int64_t gi = a.n_items < (int64_t)c->n_cu * 8 ? a.n_items : (int64_t)c->n_cu * 8;
     hipLaunchKernelGGL(path_items_kernel, dim3((unsigned)gi), dim3(PM_THREADS), 0, st, a);
     DP_HIP(hipGetLastError());

@@ -346,13 +346,13 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
   int proc = 0, elem = -1;
   if (del) {
     ckf = k_del;
-    if (!scattering && dlog && dlog->n < DEL_MAXR) {
+    if (!scattering && dlog) {
       // the target choice only consumes its draw when nothing is deflected
       const DelSample d = del_sample(K, mass, k_del, rng, err);
       if (err) return;
       kf = d.anc;
       if (d.deferred) {
-        del_push(dlog, d, k_del, 1);
+        if (!del_push(dlog, d, k_del, 1)) dw = del_total(K, mass, d.anc, d.anc - k_del) / d.denom;
         (void)uniform(rng);
       } else {
         dw = d.w;
@@ -458,16 +458,16 @@ __device__ __noinline__ static void mixed_step_f(const DevTables& T, const Mediu
     k = pre;
     ik = ip;
     if (occurs) {
-      if (dlog && dlog->n < DEL_MAXR) {
+      if (dlog) {
         const DelSample d = del_sample(c, m, k, rng, err);
         if (err) return;
-        if (d.deferred) {
-          del_push(dlog, d, k, 0);
+        if (d.deferred && del_push(dlog, d, k, 0)) {
           k = d.anc;
           continue;
         }
+        const double wd = d.deferred ? del_total(c, m, d.anc, d.anc - k) / d.denom : d.w;
         k = d.anc;
-        weight *= d.w;
+        weight *= wd;
       } else {
         const DelOut o = reverse_del_event(c, m, k, rng, false, err);
         if (err) return;

@@ -194,6 +194,10 @@ _SIGS = {
                                     C.c_int32, C.POINTER(Mixture), i32_p, dbl_p,
                                     C.POINTER(EnergySamples), C.c_uint64, C.c_int32, C.c_double,
                                     dbl_p, dbl_p, dbl_p]),
+    "dp_csda_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                 C.c_int64] + [C.POINTER(C.c_double)] * 7),
+    "dp_csda_direct_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                     C.c_int64] + [C.c_void_p] * 7),
     "dp_set_csda_correction": (C.c_int, [C.c_void_p, C.POINTER(CsdaCorrection)]),
     "dp_set_csda_material_correction": (C.c_int, [C.c_void_p, C.POINTER(CsdaMaterialCorrection)]),
     "dp_csda_w_flux_jacobian": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(MaterialConfig),

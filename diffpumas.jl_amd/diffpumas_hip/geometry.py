@@ -84,3 +84,54 @@ def compute_flux(physics, rock_density, rock_thickness, elevation, energy_min, e
                         n_samples=n_samples, seed=seed, straggling=straggling,
                         scattering=scattering, energy_threshold_low=energy_threshold_low, **kw)
     return r["flux"], r["sigma"]
+
+
+def compute_flux_csda_direct(physics, rock_material, air_material, rock_density, rock_thickness,
+                             elevation, energy_initial, charge, primary_altitude=PRIMARY_ALTITUDE,
+                             gradient=False, ctx=None):
+    """compute_flux_csda_direct (src/Geometry.jl:1126-1209) on the GPU for
+    scalars or broadcastable arrays of samples.  With gradient=True returns
+    (flux, d flux / d rock_density) as in compute_flux_gradient_csda
+    (:1247-1262)."""
+    import numpy as np
+    ctx = ctx or _lib.default_context()
+    h = physics.device_handle(ctx)
+    arrs = np.broadcast_arrays(*(np.asarray(x, dtype=np.float64) for x in
+                                 (rock_density, rock_thickness, elevation, energy_initial, charge)))
+    shape = arrs[0].shape
+    a = [np.ascontiguousarray(x.ravel()) for x in arrs]
+    n = a[0].size
+    flux, d = np.zeros(n), (np.zeros(n) if gradient else None)
+    _lib.check(ctx.lib.dp_csda_direct(ctx.h, h, int(rock_material), int(air_material),
+                                      float(primary_altitude), n, *[_lib.ptr(x) for x in a],
+                                      _lib.ptr(flux), _lib.ptr(d)))
+    flux = flux.reshape(shape)
+    if not gradient:
+        return flux[()] if shape == () else flux
+    d = d.reshape(shape)
+    return (flux[()], d[()]) if shape == () else (flux, d)
+
+
+def _rock_air(physics):
+    # compute_flux_differentiable_csda (src/Geometry.jl:1220-1238): StandardRock
+    # and Air by name, else the first and the last material
+    rock = physics.get_material_index("StandardRock")
+    air = physics.get_material_index("Air")
+    return (0 if rock < 0 else rock), (len(physics.names) - 1 if air < 0 else air)
+
+
+def compute_flux_differentiable_csda(physics, rock_density, rock_thickness, elevation,
+                                     energy_final, charge, ctx=None):
+    """compute_flux_differentiable_csda (src/Geometry.jl:1220-1238)."""
+    rock, air = _rock_air(physics)
+    return compute_flux_csda_direct(physics, rock, air, rock_density, rock_thickness, elevation,
+                                    energy_final, charge, ctx=ctx)
+
+
+def compute_flux_gradient_csda(physics, rock_density, rock_thickness, elevation, energy_final,
+                               charge, ctx=None):
+    """compute_flux_gradient_csda (src/Geometry.jl:1247-1262): (flux, d flux /
+    d rock_density), the exact derivative of the same discretised program."""
+    rock, air = _rock_air(physics)
+    return compute_flux_csda_direct(physics, rock, air, rock_density, rock_thickness, elevation,
+                                    energy_final, charge, gradient=True, ctx=ctx)

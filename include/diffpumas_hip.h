@@ -269,6 +269,23 @@ int dp_mc_flat_backward_events(dp_ctx ctx, dp_tables tables, const dp_flat_geome
                                const dp_mc_config* cfg, dp_mc_result* out,
                                double* event_weights);
 
+/* Replaces: compute_flux_csda_direct (src/Geometry.jl:1126-1209), and with
+ * dflux_drho compute_flux_gradient_csda (:1247-1262), for n samples: a rock
+ * slab of material `rock` (rock_density[i], rock_thickness[i] m) under an
+ * exponential atmosphere of material `air` up to primary_altitude, muon
+ * energy[i] GeV at the detector, elevation[i] degrees, charge[i].
+ * flux[i] per GeV m^2 s sr; dflux_drho[i] = d flux / d rock_density (may be
+ * NULL).  Samples whose energy leaves the tables give 0. */
+int dp_csda_direct(dp_ctx ctx, dp_tables tables, int32_t rock, int32_t air,
+                   double primary_altitude, int64_t n, const double* rock_density,
+                   const double* rock_thickness, const double* elevation, const double* energy,
+                   const double* charge, double* flux, double* dflux_drho);
+int dp_csda_direct_dev(dp_ctx ctx, dp_tables tables, int32_t rock, int32_t air,
+                       double primary_altitude, int64_t n, const double* rock_density,
+                       const double* rock_thickness, const double* elevation,
+                       const double* energy, const double* charge, double* flux,
+                       double* dflux_drho);
+
 /* ------------------------------- water-fraction tomography CSDA (w-model)
  * MaterialConfig (src/Tomography.jl:179-188), materials 0-based;
  * porous_material < 0 disables the shallow porous recipe.                  */

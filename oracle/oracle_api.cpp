@@ -407,6 +407,21 @@ Corr make_corr(const double* c) {
 }  // namespace
 extern "C" {
 
+// compute_flux_csda_direct / compute_flux_gradient_csda (src/Geometry.jl:
+// 1126-1262) for n samples: flux[i] and d flux / d rock_density.
+void orc_csda_direct(int M, int K, const double* data, const double* scal, double mass,
+                     double ctau, int rock, int air, double primary_altitude, int n,
+                     const double* rho, const double* thick, const double* elev,
+                     const double* energy, const double* charge, double* flux, double* dflux) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  for (int i = 0; i < n; ++i) {
+    Dual f = flux_csda_direct(T, rock, air, Dual(rho[i], 1.0), thick[i], elev[i], energy[i],
+                              charge[i], primary_altitude);
+    flux[i] = f.v;
+    if (dflux) dflux[i] = f.d;
+  }
+}
+
 // One cell segment with the (w, E_in) dual walk of _cell_segment_partials
 // (src/Tomography_rules.jl:33-48).  out = E, G, ok, dE/dw, dE/dE_in, dG/dw,
 // dG/dE_in, with G the weight factor (unit incoming weight).

@@ -322,6 +322,39 @@ inline R flat_csda_flux(const Tables& T, const Layered& g, R rho, double energy,
 }
 
 
+// compute_flux_csda_direct, src/Geometry.jl:1126-1209: one range inversion
+// through the rock slab, one through the analytic air column, decay at the
+// mean energy of each layer.  R is the rock-density type (double, or a dual
+// for compute_flux_gradient_csda, :1247-1262).
+template <class R>
+inline R flux_csda_direct(const Tables& T, int rock, int air, R rock_density, double thickness,
+                          double elevation, double energy, double charge, double primary_altitude) {
+  const double el = elevation * (M_PI / 180.0);
+  const double cos_t = std::cos(M_PI / 2 - el);
+  const double cs = std::max(std::fabs(cos_t), 0.01);
+  // compute_rock_grammage, :1064-1068
+  R x_rock = (thickness / cs) * rock_density;
+  const double x0 = prop_range(T, false, rock, energy);
+  R x1 = x0 + x_rock;
+  R e1 = prop_kinetic(T, false, rock, x1);
+  if (!(val(e1) < 1e12) || !std::isfinite(val(e1))) return R(0.0);
+  const double s0 = prop_stop(T, false, rock, energy);
+  R s1 = prop_stop(T, false, rock, e1);
+  R w_rock = s1 / s0;
+  R d_rock = decay_weight(T, thickness / cs, (energy + e1) / 2.0);
+  const double x_air = air_grammage(thickness, primary_altitude, cos_t);
+  R xa0 = prop_range(T, false, air, e1);
+  R xa1 = xa0 + x_air;
+  R e2 = prop_kinetic(T, false, air, xa1);
+  if (!(val(e2) < 1e12) || !std::isfinite(val(e2))) return R(0.0);
+  R sa0 = prop_stop(T, false, air, e1);
+  R sa1 = prop_stop(T, false, air, e2);
+  R w_air = sa1 / sa0;
+  R d_air = decay_weight(T, (primary_altitude - thickness) / cs, (e1 + e2) / 2.0);
+  R w = w_rock * d_rock * w_air * d_air;
+  return w * flux_gccly(cos_t, e2, charge);
+}
+
 // ============================================= w-parameterised tomography CSDA
 // MaterialConfig (src/Tomography.jl:179-188), materials 0-based (porous < 0
 // disables the shallow porous recipe) and SiteConfig (:198-201).

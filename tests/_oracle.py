@@ -286,6 +286,22 @@ _SIGS["orc_spectral_index"] = (C.c_double, [C.c_double, C.c_double])
 _SIGS["orc_var_correction"] = (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double, d_p])
 
 
+_SIGS["orc_csda_direct"] = (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double, C.c_int,
+                                   C.c_int, C.c_double, C.c_int, d_p, d_p, d_p, d_p, d_p, d_p,
+                                   d_p])
+
+
+def csda_direct(ph, rock, air, rho, thick, elev, energy, charge, primary_altitude=3e4):
+    """(flux, d flux / d rho) of compute_flux_csda_direct for arrays of samples."""
+    a = [f64(np.broadcast_to(x, np.broadcast(rho, thick, elev, energy, charge).shape).ravel())
+         for x in (rho, thick, elev, energy, charge)]
+    n = len(a[0])
+    flux, d = np.zeros(n), np.zeros(n)
+    lib().orc_csda_direct(*tabargs(ph), ph.mass, ph.ctau, rock, air, primary_altitude, n,
+                          *[P(x) for x in a], P(flux), P(d))
+    return flux, d
+
+
 def corr_array(enabled=True, kappa_strag=0.0, kappa_hard=0.0, resid_a=0.0, resid_b=0.0,
                resid_c=0.0, resid_d=0.0, material=None):
     """CsdaCorrection (src/Tomography.jl:112-137) and, if material = (linear,
