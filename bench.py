@@ -315,6 +315,72 @@ def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32):
             "_problem": (paths, w, s, m)}
 
 
+def run_direct(dp, L, physics, ctx, torch, steps, warmup, n=1 << 24):
+    """Direct CSDA flux + density tangent (compute_flux_gradient_csda) for n
+    device-resident samples: 56 B per sample (5 inputs, 2 outputs)."""
+    import numpy as np
+    rng = np.random.default_rng(12)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64).to(dev)
+    rho, thick = T(rng.uniform(2000.0, 3000.0, n)), T(rng.uniform(10.0, 800.0, n))
+    elev = T(rng.uniform(10.0, 90.0, n))
+    energy = T(np.exp(rng.uniform(0.0, np.log(1e4), n)))
+    charge = T(np.where(rng.random(n) < 0.5, -1.0, 1.0))
+    flux = torch.zeros(n, dtype=torch.float64, device=dev)
+    dflux = torch.zeros(n, dtype=torch.float64, device=dev)
+    rock, air = physics.get_material_index("StandardRock"), physics.get_material_index("Air")
+    h = physics.device_handle(ctx)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream.cuda_stream)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        L.check(ctx.lib.dp_csda_direct_dev(ctx.h, h, rock, air, 3e4, n, rho.data_ptr(),
+                                           thick.data_ptr(), elev.data_ptr(), energy.data_ptr(),
+                                           charge.data_ptr(), flux.data_ptr(), dflux.data_ptr()))
+        if ev is not None:
+            ev[1].record(stream)
+
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(evs[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    ctx.set_stream(None)
+    ms_step = (t1 - t0) * 1e3 / steps
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    achieved = 56.0 * n / (kern_ms * 1e-3) / 1e9
+    host = [x.cpu().numpy() for x in (rho, thick, elev, energy, charge)]
+    return {"value": n / (ms_step * 1e-3), "unit": "samples/s", "ms_per_step": ms_step,
+            "kernel_ms": kern_ms, "samples": n,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+            "_problem": (rock, air, host)}
+
+
+def cpu_direct_baseline(physics, problem, budget_s=6.0):
+    """The oracle's direct CSDA flux + tangent, one thread, on a prefix of the samples."""
+    import _oracle as O
+    rock, air, host = problem
+    n = 1 << 14
+    while True:
+        t0 = time.perf_counter()
+        O.csda_direct(physics, rock, air, *(x[:n] for x in host))
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= len(host[0]):
+            return {"value": n / dt, "unit": "samples/s", "cores": 1, "kind": "port",
+                    "sample": f"first {n} samples, {dt:.2f} s"}
+        n *= 2
+
+
 def run_pathmc(dp, physics, ctx, n_rays=256, n_events=10000):
     """Fixed-path straggled MC over the lognormal-cavity density field: n_rays
     bins of the tomography workload, n_events per bin, one launch."""
@@ -443,10 +509,15 @@ def main():
     pm = run_pathmc(dp, physics, ctx)
     log(f"pathmc: {pm['value']:.4g} events/s, kernel {pm['kernel_ms']:.3f} ms")
     pprob = pm.pop("_problem")
+    dr = run_direct(dp, L, physics, ctx, torch, a.steps, a.warmup)
+    log(f"direct: {dr['value']:.4g} samples/s, kernel {dr['kernel_ms']:.3f} ms, "
+        f"{dr['roofline']['achieved']:.1f} GB/s")
+    dprob = dr.pop("_problem")
     cpu = None if a.no_cpu else cpu_mc_baseline(physics)
     cpu_adj = None if a.no_cpu else cpu_adjoint_baseline(physics, problem)
     wm["cpu_baseline"] = None if a.no_cpu else cpu_wmodel_baseline(physics, wprob)
     pm["cpu_baseline"] = None if a.no_cpu else cpu_pathmc_baseline(physics, pprob)
+    dr["cpu_baseline"] = None if a.no_cpu else cpu_direct_baseline(physics, dprob)
     out = {
         "metric": "straggled backward-MC events/s (+ adjoint rays*voxels/s), 1 GPU, % HBM roofline",
         "value": mc["value"], "unit": "events/s", "n_gpus": 1, "steps": a.steps,
@@ -471,6 +542,9 @@ def main():
                                     "directions, 32 nodes 1-1e4 GeV"),
         "pathmc": dict(pm, workload="fixed-path straggled MC, 64x64x32 lognormal density "
                                     "field with a 1800 kg/m3 cavity, 256 bins x 1e4 events"),
+        "direct": dict(dr, workload="direct CSDA flux + d/d rho (compute_flux_gradient_csda), "
+                                    "2^24 samples: rock 10-800 m at 2000-3000 kg/m3, "
+                                    "elevation 10-90 deg, E 1-1e4 GeV"),
     }
     if a.calib_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)

@@ -32,6 +32,8 @@ struct PoolRec {
                  // error), 1: straggled step (non-finite weight kills the event)
 };
 
+struct CoopBuf;
+
 struct DelLog {
   PoolRec* pool;  // the block's pool (LDS)
   int* count;     // records reserved so far in the block (LDS)
@@ -39,6 +41,9 @@ struct DelLog {
   int owner;      // this lane
   int n;          // records this lane's event has pooled
   int medium;     // medium index of the current step
+  CoopBuf* coop;  // this wave's buffer for target-selecting hard events (null: inline)
+  const Medium* media;  // media by index (device memory)
+  int n_media;
 };
 
 struct DelSample {
@@ -129,6 +134,122 @@ __device__ __forceinline__ double pool_apply(const PoolRec* pool, int m, int own
   }
   if (!dead && !fail && pend) fail = pend;
   return (fail || dead) ? 0.0 : w;
+}
+
+// ---------------------------------------------------------------------------
+// A hard event that selects a target (straggled step with scattering,
+// src/Geometry.jl:261-278, reverse_del_event with details = true) cannot wait
+// for the pool: the chosen process deflects the muon in the same step.  Its
+// components are evaluated at once by every lane of the wave that is in the
+// same straggled step; the owner sums them in order and picks the target.
+constexpr int COOP_SLOTS = 8;  // events per round
+constexpr int COOP_EL = 8;     // elements per medium (more: the owner evaluates alone)
+
+struct CoopBuf {
+  double comp[COOP_SLOTS][4 * COOP_EL];
+  double anc[COOP_SLOTS], q[COOP_SLOTS];
+  int mi[COOP_SLOTS];
+};
+
+// Orders the LDS accesses of the lanes of one wave (no hardware barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Owner-only evaluation after del_sample: total, weight and target.
+__device__ __forceinline__ DelOut del_finish(const DevKernel& c, double mass, double k,
+                                             const DelSample& d, Philox& rng) {
+  const double q = d.anc - k;
+  const double total = del_total(c, mass, d.anc, q);
+  DelOut o{d.anc, total / d.denom, 0, -1};
+  if (total <= 0) return o;
+  double u = uniform(rng) * total;
+  for (int i = 0; i < c.n_el; ++i)
+    for (int p = 1; p <= 4; ++p) {
+      u -= diff_component(c, mass, i, p, d.anc, q);
+      if (u <= 0) { o.proc = p; o.elem = i; return o; }
+    }
+  o.proc = 4; o.elem = c.n_el - 1;
+  return o;
+}
+
+// Called by every lane active in the caller (the branch to it is uniform over
+// those lanes).  Lanes with del get reverse_del_event(details = true) at
+// energy k in medium mi: the same draws, ancestor energy, weight and target as
+// the inline evaluation.
+__device__ __noinline__ static DelOut coop_del_event(const DelLog& lg, int mi, double mass,
+                                                     double k, Philox& rng, bool del, int& err) {
+  CoopBuf* cb = lg.coop;
+  DelOut out{k, 0.0, 0, -1};
+  DelSample d{k, 0.0, 1.0, false};
+  bool need = false;
+  if (del) {
+    const DevKernel& c = lg.media[mi].kern;
+    d = del_sample(c, mass, k, rng, err);
+    if (!err) {
+      if (!d.deferred) out = DelOut{d.anc, d.w, 0, -1};
+      else if (c.n_el > COOP_EL) out = del_finish(c, mass, k, d, rng);
+      else need = true;
+    }
+  }
+  const uint64_t act = __ballot(1);
+  const int lane = __lane_id();
+  const uint64_t below = lane ? (act & (~0ull >> (64 - lane))) : 0ull;
+  const int rank = __popcll(below);
+  const int A = __popcll(act);
+  uint64_t left = __ballot(need);
+  while (left) {
+    uint64_t grp = 0;
+    int ng = 0;
+    while (left && ng < COOP_SLOTS) {
+      const uint64_t b = left & (~left + 1);
+      grp |= b;
+      left ^= b;
+      ++ng;
+    }
+    const bool mine = (grp >> lane) & 1;
+    const int slot = __popcll(lane ? (grp & (~0ull >> (64 - lane))) : 0ull);
+    if (mine) {
+      cb->anc[slot] = d.anc;
+      cb->q[slot] = d.anc - k;
+      cb->mi[slot] = mi;
+    }
+    wave_sync();
+    int off[COOP_SLOTS + 1];
+    int ms[COOP_SLOTS];
+    off[0] = 0;
+    for (int s = 0; s < ng; ++s) {
+      ms[s] = min(max(cb->mi[s], 0), lg.n_media - 1);
+      off[s + 1] = off[s] + 4 * min(lg.media[ms[s]].kern.n_el, COOP_EL);
+    }
+    for (int w = rank; w < off[ng]; w += A) {
+      int s = 0;
+      while (w >= off[s + 1]) ++s;
+      const int j = w - off[s];
+      cb->comp[s][j] = diff_component(lg.media[ms[s]].kern, mass, j >> 2, (j & 3) + 1,
+                                      cb->anc[s], cb->q[s]);
+    }
+    wave_sync();
+    if (mine) {
+      const int nc = 4 * lg.media[mi].kern.n_el;
+      double total = 0.0;
+      for (int j = 0; j < nc; ++j) total += cb->comp[slot][j];
+      out = DelOut{d.anc, total / d.denom, 0, -1};
+      if (total > 0) {
+        double u = uniform(rng) * total;
+        out.proc = 4;
+        out.elem = nc / 4 - 1;
+        for (int j = 0; j < nc; ++j) {
+          u -= cb->comp[slot][j];
+          if (u <= 0) { out.proc = (j & 3) + 1; out.elem = j >> 2; break; }
+        }
+      }
+    }
+    wave_sync();  // the buffer is read before the next round writes it
+  }
+  return out;
 }
 
 }  // namespace dpk

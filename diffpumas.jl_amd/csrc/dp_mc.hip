@@ -217,6 +217,7 @@ struct StochArgs {
   double* out_w;
   int* err;            // [0] error bits, [1] spare
   unsigned long long* bad_event;
+  const Medium* media;  // rock, air, porous in device memory (indexed by DelLog::medium)
 };
 
 
@@ -372,6 +373,7 @@ template <bool FAST>
 __global__ __launch_bounds__(MC_THREADS, 4) void mc_flat_stochastic_kernel(StochArgs a) {
   __shared__ PoolRec pool[FAST ? MC_POOL : 1];
   __shared__ int pool_n;
+  __shared__ CoopBuf coop;
   const int tid = threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * MC_THREADS;
   for (int64_t base = (int64_t)blockIdx.x * MC_THREADS; base < a.n; base += stride) {
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(MC_THREADS, 4) void mc_flat_stochastic_kernel(Stoch
     const bool active = e < a.n;
     double v = 0.0;
     int err = 0;
-    DelLog dlg{pool, &pool_n, MC_POOL, tid, 0, 0};
+    DelLog dlg{pool, &pool_n, MC_POOL, tid, 0, 0, &coop, a.media, 3};
     if (active) {
       Philox rng;
       rng.init(a.seed, (uint64_t)e);
@@ -733,7 +735,8 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
   size_t o_res = dph::align_up(o_buf + 8 * 4 * (size_t)nparts, 256);
   size_t o_flag = o_res + 256;
   size_t o_bad = o_flag + 256;
-  int rc = dph::ensure(c->scratch, o_bad + 256);
+  size_t o_med = o_bad + 256;
+  int rc = dph::ensure(c->scratch, dph::align_up(o_med + 3 * sizeof(Medium), 256));
   if (rc) return rc;
   char* b = (char*)c->scratch.ptr;
   int* flags = (int*)(b + o_flag);
@@ -758,6 +761,7 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
   a.out_w = stochastic ? nullptr : out_w;
   a.out_gf = (double*)(b + o_gf); a.out_gd = (double*)(b + o_gd);
   a.err = flags;
+  Medium hm[3];  // host copy of the media, read by an async copy before the final sync
   if (stochastic) {
     StochArgs s;
     s.T = {t->data, t->scalars, t->n_knots, t->pow2, t->mass, t->ctau, t->elastic_ratio};
@@ -779,6 +783,9 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     s.thr = thr; s.sw = cfg->mode_switch;
     s.straggling = cfg->straggling != 0; s.scattering = cfg->scattering != 0;
     s.n = n; s.seed = cfg->seed; s.out_w = out_w; s.err = flags; s.bad_event = bad;
+    hm[0] = s.rock; hm[1] = s.air; hm[2] = s.porous;
+    DP_HIP(hipMemcpyAsync(b + o_med, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
+    s.media = (const Medium*)(b + o_med);
     (void)hipEventRecord(c->ev0, c->stream);
     const bool fast = t->shared_grid;
     int occ = 0;

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
     double wi = 0.0;
     int err = 0;
     int64_t e = 0, el = 0;
-    DelLog dlg{pool, &pool_n, PM_POOL, tid, 0, 0};
+    DelLog dlg{pool, &pool_n, PM_POOL, tid, 0, 0, nullptr, a.media, a.air_medium + 1};
     if (ran) {
       const int s = (int)a.order[sl];
       e = (int64_t)r * a.n_samples + s;

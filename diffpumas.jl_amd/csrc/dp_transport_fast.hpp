@@ -344,7 +344,16 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
   }
   double dw = 1.0, ckf = kf;
   int proc = 0, elem = -1;
-  if (del) {
+  if (scattering && dlog && dlog->coop) {
+    // every lane of this step call takes part in the component evaluation
+    const DelOut o = coop_del_event(*dlog, dlog->medium, mass, k_del, rng, del, err);
+    if (del) {
+      if (err) return;
+      ckf = k_del;
+      kf = o.k; dw = o.w; proc = o.proc; elem = o.elem;
+      dX = X_del;
+    }
+  } else if (del) {
     ckf = k_del;
     if (!scattering && dlog) {
       // the target choice only consumes its draw when nothing is deflected

@@ -1,7 +1,7 @@
 """GPU parity of the direct CSDA flux and its density tangent (dp_csda_direct,
 compute_flux_csda_direct / compute_flux_gradient_csda, src/Geometry.jl:
 1126-1262) against the CPU oracle: flux and derivative within 1e-12 relative
-(the same FP64 operations), scalar and array calls, out-of-table samples 0.
+(the same FP64 operations), scalar and array calls, clamped far samples.
 """
 import numpy as np
 import pytest
@@ -43,9 +43,14 @@ def test_scalar_calls_and_named_materials(ph, ctx):
     assert g == pytest.approx(do[0], rel=1e-10)
 
 
-def test_out_of_table_is_zero_and_bad_material(ph, ctx):
-    f = dp.compute_flux_csda_direct(ph, 0, 1, [2650.0], [5000.0], [5.0], [5e8], [1.0], ctx=ctx)
-    fo, _ = O.csda_direct(ph, 0, 1, 2650.0, 5000.0, 5.0, 5e8, 1.0)
-    assert f[0] == fo[0]
+def test_far_samples_and_bad_material(ph, ctx):
+    f, d = dp.compute_flux_csda_direct(ph, 0, 1, [2650.0, 2650.0], [5000.0, 1e6], [5.0, 5.0],
+                                       [5e8, 10.0], [1.0, 1.0], gradient=True, ctx=ctx)
+    fo, do = O.csda_direct(ph, 0, 1, 2650.0, np.array([5000.0, 1e6]), 5.0,
+                           np.array([5e8, 10.0]), 1.0)
+    np.testing.assert_allclose(f, fo, rtol=1e-12)
+    # 1000 km of rock: the inverse range clamps at the last knot (property_kinetic_energy),
+    # where the density tangent is 0
+    assert d[1] == do[1] == 0.0 and f[1] > 0
     with pytest.raises(dp.ArgumentError):
         dp.compute_flux_csda_direct(ph, 0, 7, 2650.0, 10.0, 45.0, 10.0, 1.0, ctx=ctx)
