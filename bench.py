@@ -40,6 +40,16 @@ sys.path.insert(0, os.path.join(ROOT, "diffpumas.jl_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# Measured HBM bytes per unit of work (FETCH_SIZE + WRITE_SIZE of one dispatch,
+# scaled by what the counters read for a 1 GiB device copy), from the counter
+# runs summarised in profiles/; `traffic` = this x the units of one launch.
+MEASURED_BYTES_PER_UNIT = {
+    "mc_flat_stochastic_kernel": 1482041.0,   # per event
+    "csda_cube_kernel": 16467.0,              # per ray-voxel (all 256 nodes)
+    "wcsda_kernel": 2132.0,                   # per ray-voxel (all 32 nodes)
+    "path_mc_kernel": 312018.0,               # per event
+    "csda_direct_kernel": 56.0,               # per sample
+}
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -125,7 +135,8 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
         "value": units / (ms_step * 1e-3), "ms_per_step": ms_step, "kernel_ms": kern_ms,
         "rays": paths.n_rays, "segments": units, "nodes": len(ne),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_BYTES_PER_UNIT["csda_cube_kernel"] * units},
         "flux_sum": float(flux_h.sum()), "grad_norm": float(grad.norm().item()),
         "_problem": (geo, paths, rho, ne, nw, nq),
     }
@@ -158,7 +169,8 @@ def run_mc(dp, physics, ctx, torch, n_events, steps, warmup):
     return {"value": n_events / (ms * 1e-3), "ms_per_step": ms, "kernel_ms": kern,
             "events": n_events, "flux": r["flux"], "sigma": r["sigma"],
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": MEASURED_BYTES_PER_UNIT["mc_flat_stochastic_kernel"] * n_events}}
 
 
 def cpu_mc_baseline(physics, budget_s=15.0):
@@ -311,7 +323,8 @@ def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32):
             "kernel_ms": kern_ms, "rays": paths.n_rays, "segments": units, "nodes": n_nodes,
             "ray_nodes_per_s": paths.n_rays * n_nodes / (ms_step * 1e-3),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": MEASURED_BYTES_PER_UNIT["wcsda_kernel"] * units},
             "_problem": (paths, w, s, m)}
 
 
@@ -362,7 +375,8 @@ def run_direct(dp, L, physics, ctx, torch, steps, warmup, n=1 << 24):
     return {"value": n / (ms_step * 1e-3), "unit": "samples/s", "ms_per_step": ms_step,
             "kernel_ms": kern_ms, "samples": n,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": MEASURED_BYTES_PER_UNIT["csda_direct_kernel"] * n},
             "_problem": (rock, air, host)}
 
 
@@ -416,7 +430,7 @@ def run_pathmc(dp, physics, ctx, n_rays=256, n_events=10000):
             "roofline": {"bound": "hbm", "achieved": (16.0 * ev + 12.0 * paths.n_segments) /
                          (kern * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": (16.0 * ev + 12.0 * paths.n_segments) / (kern * 1e-3) / 1e9 /
-                         HBM_PEAK_GBS, "traffic": None},
+                         HBM_PEAK_GBS, "traffic": MEASURED_BYTES_PER_UNIT["path_mc_kernel"] * ev},
             "_problem": (paths, rho, s, m, site)}
 
 

@@ -10,6 +10,7 @@ from .physics import (PhysicsTables, create_physics, create_physics_csda, load_d
                       MUON_MASS, MUON_C_TAU)
 from .cube import (CubeGeometry, RayPaths, compute_flux_gradient_cube, csda_cube_flux,
                    csda_cube_vjp, ray_direction, trace_cube_paths, trace_edge_paths)
+from .topography import ElevationMap, ray_surface_exit_distance
 from .geometry import (PRIMARY_ALTITUDE, compute_flux, run_backward_mc, compute_flux_csda_direct,
                        compute_flux_differentiable_csda, compute_flux_gradient_csda)
 from .tomography import (DirectionalPaths, EnergySamples, MaterialConfig, SiteConfig,
@@ -28,7 +29,7 @@ __all__ = [
     "load_mdf", "MUON_MASS", "MUON_C_TAU", "CubeGeometry", "RayPaths",
     "compute_flux_gradient_cube", "csda_cube_flux", "csda_cube_vjp", "ray_direction",
     "trace_cube_paths", "trace_edge_paths", "PRIMARY_ALTITUDE", "compute_flux",
-    "run_backward_mc", "compute_flux_csda_direct", "compute_flux_differentiable_csda",
+    "run_backward_mc", "ElevationMap", "ray_surface_exit_distance", "compute_flux_csda_direct", "compute_flux_differentiable_csda",
     "compute_flux_gradient_csda", "DirectionalPaths", "EnergySamples", "MaterialConfig", "SiteConfig",
     "assemble_forward_and_jacobian", "cell_density", "compute_directional_flux_csda",
     "csda_energy_quadrature", "direction_vector", "directional_flux_and_grad_csda",

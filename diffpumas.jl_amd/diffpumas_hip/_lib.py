@@ -138,6 +138,11 @@ class DirectionalPaths(C.Structure):
                 ("remaining_rock", dbl_p), ("exit_z", dbl_p), ("valid", C.POINTER(C.c_uint8))]
 
 
+class ElevationMap(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("x0", C.c_double), ("y0", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("data", C.c_void_p)]
+
+
 class CsdaCorrection(C.Structure):
     _fields_ = [("enabled", C.c_int32), ("kappa_strag", C.c_double), ("kappa_hard", C.c_double),
                 ("resid_a", C.c_double), ("resid_b", C.c_double), ("resid_c", C.c_double),
@@ -194,6 +199,13 @@ _SIGS = {
                                     C.c_int32, C.POINTER(Mixture), i32_p, dbl_p,
                                     C.POINTER(EnergySamples), C.c_uint64, C.c_int32, C.c_double,
                                     dbl_p, dbl_p, dbl_p]),
+    "dp_ray_surface_exit": (C.c_int, [C.c_void_p, C.POINTER(ElevationMap), C.c_double,
+                                      C.c_double, C.c_int32, C.POINTER(C.c_double), C.c_double,
+                                      C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint8),
+                                      C.POINTER(C.c_double)]),
+    "dp_ray_surface_exit_dev": (C.c_int, [C.c_void_p, C.POINTER(ElevationMap), C.c_double,
+                                          C.c_double, C.c_int32] + [C.c_void_p, C.c_double,
+                                          C.c_double] + [C.c_void_p] * 3),
     "dp_csda_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
                                  C.c_int64] + [C.POINTER(C.c_double)] * 7),
     "dp_csda_direct_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,

@@ -286,6 +286,33 @@ int dp_csda_direct_dev(dp_ctx ctx, dp_tables tables, int32_t rock, int32_t air,
                        const double* energy, const double* charge, double* flux,
                        double* dflux_drho);
 
+/* ------------------------------------------------------------ topography
+ * ElevationMap (src/Turtle.jl:469-480): nx x ny nodes from (x0, y0) with
+ * spacings dx, dy in map units; z(ix, iy) = data[iy + ny * ix] (the storage
+ * of Julia's Matrix{Float64} of size (ny, nx)).                            */
+typedef struct {
+  int32_t nx, ny;
+  double x0, y0, dx, dy;
+  const double* data;
+} dp_elevation_map;
+
+/* Replaces: ray_surface_exit_distance (examples/lvd_tomography.jl:219-252)
+ * with local_surface_height (:214-217) and map_elevation (bilinear,
+ * src/Turtle.jl:523-553) for n_rays unit directions dirs[3n] from the
+ * detector.  Map coordinates are the local metres times xy_scale (1e-3 for a
+ * map in km); heights are taken relative to z_ref (the detector elevation).
+ * distance[i] (m), found[i], exit_z[i] = dirs[3i+2] * distance[i]; NaN and 0
+ * when the ray points down or leaves the map first.  step 40 m and
+ * max_distance 25 km are the reference defaults. */
+int dp_ray_surface_exit(dp_ctx ctx, const dp_elevation_map* map, double xy_scale, double z_ref,
+                        int32_t n_rays, const double* dirs, double step, double max_distance,
+                        double* distance, uint8_t* found, double* exit_z);
+/* Device pointers (map->data, dirs and the outputs); no synchronisation. */
+int dp_ray_surface_exit_dev(dp_ctx ctx, const dp_elevation_map* map, double xy_scale,
+                            double z_ref, int32_t n_rays, const double* dirs, double step,
+                            double max_distance, double* distance, uint8_t* found,
+                            double* exit_z);
+
 /* ------------------------------- water-fraction tomography CSDA (w-model)
  * MaterialConfig (src/Tomography.jl:179-188), materials 0-based;
  * porous_material < 0 disables the shallow porous recipe.                  */

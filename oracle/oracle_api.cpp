@@ -407,6 +407,30 @@ Corr make_corr(const double* c) {
 }  // namespace
 extern "C" {
 
+// ray_surface_exit_distance (examples/lvd_tomography.jl:219-252) for n
+// directions over the map (map_elevation, src/Turtle.jl:523-553).
+void orc_ray_surface_exit(int nx, int ny, double x0, double y0, double dx, double dy,
+                          const double* data, double xy_scale, double z_ref, int n,
+                          const double* dirs, double step, double max_distance, double* dist,
+                          uint8_t* found, double* exit_z) {
+  Dem m{nx, ny, x0, y0, dx, dy, data};
+  for (int i = 0; i < n; ++i) {
+    bool f;
+    ray_surface_exit(m, xy_scale, z_ref, dirs + 3 * i, step, max_distance, dist[i], f, exit_z[i]);
+    found[i] = f;
+  }
+}
+// map_elevation at n points: out = elevation, inside.
+void orc_dem_elevation(int nx, int ny, double x0, double y0, double dx, double dy,
+                       const double* data, int n, const double* xy, double* z, uint8_t* inside) {
+  Dem m{nx, ny, x0, y0, dx, dy, data};
+  for (int i = 0; i < n; ++i) {
+    bool in;
+    z[i] = dem_elevation(m, xy[2 * i], xy[2 * i + 1], in);
+    inside[i] = in;
+  }
+}
+
 // compute_flux_csda_direct / compute_flux_gradient_csda (src/Geometry.jl:
 // 1126-1262) for n samples: flux[i] and d flux / d rock_density.
 void orc_csda_direct(int M, int K, const double* data, const double* scal, double mass,

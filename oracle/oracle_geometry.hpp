@@ -355,6 +355,73 @@ inline R flux_csda_direct(const Tables& T, int rock, int air, R rock_density, do
   return w * flux_gccly(cos_t, e2, charge);
 }
 
+// ================================================== topography: DEM ray exit
+// ElevationMap (src/Turtle.jl:469-480): nx x ny nodes from (x0, y0) with
+// spacings dx, dy; z(ix, iy) = data[iy + ny * ix] (a Julia Matrix{Float64}
+// of size (ny, nx) as stored).
+struct Dem {
+  int nx, ny;
+  double x0, y0, dx, dy;
+  const double* data;
+};
+// map_elevation, src/Turtle.jl:523-553: bilinear interpolation; inside = false
+// (elevation 0) off the map.
+inline double dem_elevation(const Dem& m, double x, double y, bool& inside) {
+  inside = false;
+  if (std::isnan(x) || std::isnan(y)) return 0.0;
+  double hx = (x - m.x0) / m.dx;
+  double hy = (y - m.y0) / m.dy;
+  if (hx > m.nx - 1 || hx < 0 || hy > m.ny - 1 || hy < 0) return 0.0;
+  int ix = (int)std::floor(hx);
+  int iy = (int)std::floor(hy);
+  if (ix == m.nx - 1) { ix -= 1; hx = 1.0; } else { hx -= ix; }
+  if (iy == m.ny - 1) { iy -= 1; hy = 1.0; } else { hy -= iy; }
+  const double z00 = m.data[iy + (size_t)m.ny * ix];
+  const double z10 = m.data[iy + (size_t)m.ny * (ix + 1)];
+  const double z01 = m.data[iy + 1 + (size_t)m.ny * ix];
+  const double z11 = m.data[iy + 1 + (size_t)m.ny * (ix + 1)];
+  inside = true;
+  return z00 * (1.0 - hx) * (1.0 - hy) + z01 * (1.0 - hx) * hy + z10 * hx * (1.0 - hy) +
+         z11 * hx * hy;
+}
+// local_surface_height, examples/lvd_tomography.jl:214-217: map coordinates
+// are the local metres times xy_scale (1e-3 for a map in km); heights are
+// relative to the detector elevation.
+inline double surface_height(const Dem& m, double xy_scale, double z_ref, double x, double y) {
+  bool inside;
+  const double e = dem_elevation(m, x * xy_scale, y * xy_scale, inside);
+  return inside ? e - z_ref : NAN;
+}
+// ray_surface_exit_distance, examples/lvd_tomography.jl:219-252: march in
+// steps of `step` up to max_distance, then 40 bisections.
+inline void ray_surface_exit(const Dem& m, double xy_scale, double z_ref, const double* d,
+                             double step, double max_distance, double& dist, bool& found,
+                             double& exit_z) {
+  dist = NAN; found = false; exit_z = NAN;
+  if (d[2] <= 0.0) return;
+  const long n = (long)std::floor(max_distance / step + 1e-9);
+  double prev = 0.0;
+  for (long k = 1; k <= n; ++k) {
+    const double s = (double)k * step;
+    const double x = d[0] * s, y = d[1] * s, z = d[2] * s;
+    const double sz = surface_height(m, xy_scale, z_ref, x, y);
+    if (!std::isfinite(sz)) return;
+    if (z > sz) {
+      double lo = prev, hi = s;
+      for (int it = 0; it < 40; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        const double ms = surface_height(m, xy_scale, z_ref, d[0] * mid, d[1] * mid);
+        if (d[2] * mid > ms) hi = mid; else lo = mid;
+      }
+      dist = 0.5 * (lo + hi);
+      found = true;
+      exit_z = d[2] * dist;
+      return;
+    }
+    prev = s;
+  }
+}
+
 // ============================================= w-parameterised tomography CSDA
 // MaterialConfig (src/Tomography.jl:179-188), materials 0-based (porous < 0
 // disables the shallow porous recipe) and SiteConfig (:198-201).

@@ -291,6 +291,41 @@ _SIGS["orc_csda_direct"] = (None, [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_d
                                    d_p])
 
 
+_SIGS["orc_ray_surface_exit"] = (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, d_p, C.c_double, C.c_double, C.c_int, d_p,
+                                        C.c_double, C.c_double, d_p, u8_p, d_p])
+_SIGS["orc_dem_elevation"] = (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, d_p, C.c_int, d_p, d_p, u8_p])
+
+
+def _dem_args(dem):
+    """dem: dict nx, ny, x0, y0, dx, dy, data (ny x nx array, z[iy, ix])."""
+    data = np.asfortranarray(dem["data"], dtype=np.float64)   # column-major: iy + ny * ix
+    return (dem["nx"], dem["ny"], dem["x0"], dem["y0"], dem["dx"], dem["dy"],
+            data.ctypes.data_as(d_p)), data
+
+
+def ray_surface_exit(dem, dirs, xy_scale=1e-3, z_ref=0.0, step=40.0, max_distance=25000.0):
+    """(distance, found, exit_z) of ray_surface_exit_distance for each direction."""
+    args, keep = _dem_args(dem)
+    d = f64(np.asarray(dirs).reshape(-1))
+    n = len(d) // 3
+    dist, ez = np.zeros(n), np.zeros(n)
+    found = np.zeros(n, np.uint8)
+    lib().orc_ray_surface_exit(*args, xy_scale, z_ref, n, P(d), step, max_distance, P(dist),
+                               P(found, C.c_uint8), P(ez))
+    return dist, found.astype(bool), ez
+
+
+def dem_elevation(dem, xy):
+    args, keep = _dem_args(dem)
+    p = f64(np.asarray(xy).reshape(-1))
+    n = len(p) // 2
+    z, inside = np.zeros(n), np.zeros(n, np.uint8)
+    lib().orc_dem_elevation(*args, n, P(p), P(z), P(inside, C.c_uint8))
+    return z, inside.astype(bool)
+
+
 def csda_direct(ph, rock, air, rho, thick, elev, energy, charge, primary_altitude=3e4):
     """(flux, d flux / d rho) of compute_flux_csda_direct for arrays of samples."""
     a = [f64(np.broadcast_to(x, np.broadcast(rho, thick, elev, energy, charge).shape).ravel())
