@@ -29,6 +29,7 @@ constexpr int MC_ARRS = 5;  // energy, log energy, csda range, csda stopping, tr
 // records evaluated by the whole wave after each round of 64 events.
 constexpr int MC_THREADS = 64;
 constexpr int MC_POOL = 192;
+constexpr int MC_BUCKETS = 256;  // source-energy buckets of the event order
 
 struct FlatGeo {
   double thick, rho;
@@ -218,6 +219,7 @@ struct StochArgs {
   int* err;            // [0] error bits, [1] spare
   unsigned long long* bad_event;
   const Medium* media;  // rock, air, porous in device memory (indexed by DelLog::medium)
+  const uint32_t* order;  // events in order of source-energy bucket (null: identity)
 };
 
 
@@ -365,6 +367,50 @@ __device__ double flat_event_f(const StochArgs& a, double energy, double charge,
   return 0.0;
 }
 
+// Events in order of source energy: a counting sort of the event indices into
+// MC_BUCKETS buckets of the first Philox draw, which fixes the log-uniform
+// source energy.  The lanes of a block then start in the same transport mode
+// and take similar numbers of steps.  Each event keeps its own stream and
+// output slot, so results do not depend on the order.
+__device__ __forceinline__ int energy_bucket(uint64_t seed, int64_t e) {
+  Philox rng;
+  rng.init(seed, (uint64_t)e);
+  const double u = rng.uniform();
+  return min(MC_BUCKETS - 1, max(0, (int)(u * MC_BUCKETS)));
+}
+__global__ void bucket_clear(int* cnt) {
+  for (int i = threadIdx.x; i < MC_BUCKETS; i += blockDim.x) cnt[i] = 0;
+}
+__global__ __launch_bounds__(256) void bucket_count(uint64_t seed, int64_t n, int* cnt) {
+  __shared__ int h[MC_BUCKETS];
+  for (int i = threadIdx.x; i < MC_BUCKETS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
+    atomicAdd(&h[energy_bucket(seed, e)], 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < MC_BUCKETS; i += blockDim.x)
+    if (h[i]) atomicAdd(&cnt[i], h[i]);
+}
+// counts -> first position of each bucket (one thread)
+__global__ void bucket_offsets(int* cnt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int acc = 0;
+  for (int i = 0; i < MC_BUCKETS; ++i) {
+    const int c = cnt[i];
+    cnt[i] = acc;
+    acc += c;
+  }
+}
+__global__ __launch_bounds__(256) void bucket_scatter(uint64_t seed, int64_t n, int* cur,
+                                                      uint32_t* order) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+    const int pos = atomicAdd(&cur[energy_bucket(seed, e)], 1);
+    if (pos >= 0 && pos < n) order[pos] = (uint32_t)e;
+  }
+}
+
 // One event per lane, rounds of MC_THREADS events per block (the loop bound is
 // uniform over the block).  On tables with a shared energy grid (FAST) the
 // deferrable hard-event weights of a round go to the block's pool and are
@@ -381,7 +427,8 @@ __global__ __launch_bounds__(MC_THREADS, 4) void mc_flat_stochastic_kernel(Stoch
       if (tid == 0) pool_n = 0;
       __syncthreads();  // the previous round's records are applied; the pool is empty
     }
-    const int64_t e = base + tid;
+    const int64_t p = base + tid;
+    const int64_t e = p < a.n ? (a.order ? (int64_t)a.order[p] : p) : a.n;
     const bool active = e < a.n;
     double v = 0.0;
     int err = 0;
@@ -736,7 +783,9 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
   size_t o_flag = o_res + 256;
   size_t o_bad = o_flag + 256;
   size_t o_med = o_bad + 256;
-  int rc = dph::ensure(c->scratch, dph::align_up(o_med + 3 * sizeof(Medium), 256));
+  size_t o_cnt = dph::align_up(o_med + 3 * sizeof(Medium), 256);
+  size_t o_ord = dph::align_up(o_cnt + 4 * MC_BUCKETS, 256);
+  int rc = dph::ensure(c->scratch, dph::align_up(o_ord + 4 * (size_t)n, 256));
   if (rc) return rc;
   char* b = (char*)c->scratch.ptr;
   int* flags = (int*)(b + o_flag);
@@ -787,6 +836,18 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     DP_HIP(hipMemcpyAsync(b + o_med, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
     s.media = (const Medium*)(b + o_med);
     (void)hipEventRecord(c->ev0, c->stream);
+    s.order = nullptr;
+    if (n < 0x7fffffff) {
+      int* cnt = (int*)(b + o_cnt);
+      uint32_t* ord = (uint32_t*)(b + o_ord);
+      const unsigned gb = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)c->n_cu * 8);
+      hipLaunchKernelGGL(bucket_clear, dim3(1), dim3(256), 0, c->stream, cnt);
+      hipLaunchKernelGGL(bucket_count, dim3(gb), dim3(256), 0, c->stream, cfg->seed, n, cnt);
+      hipLaunchKernelGGL(bucket_offsets, dim3(1), dim3(64), 0, c->stream, cnt);
+      hipLaunchKernelGGL(bucket_scatter, dim3(gb), dim3(256), 0, c->stream, cfg->seed, n, cnt, ord);
+      DP_HIP(hipGetLastError());
+      s.order = ord;
+    }
     const bool fast = t->shared_grid;
     int occ = 0;
     if (fast) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<true>, MC_THREADS, 0));

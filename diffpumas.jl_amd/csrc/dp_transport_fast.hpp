@@ -17,20 +17,6 @@
 
 namespace dpk {
 
-// Largest i in [0, n-2] with xs[i] <= x (0 if none): checks the guessed
-// interval and its neighbours, then bisects.
-__device__ __forceinline__ int locate_idx(const double* __restrict__ xs, int n, int pow2, double x,
-                                          int g) {
-  g = min(max(g, 0), n - 2);
-  if (xs[g] <= x) {
-    if (g == n - 2 || x < xs[g + 1]) return g;
-    if (g + 1 == n - 2 || x < xs[g + 2]) return g + 1;
-  } else if (g > 0 && xs[g - 1] <= x) {
-    return g - 1;
-  }
-  return bracket(xs, n, pow2, x);
-}
-
 // An energy located on the shared grid.  c = -1 below the first knot, +1 at
 // or above the last one (interpolate_table_fast clamps), 0 inside.
 struct Loc {

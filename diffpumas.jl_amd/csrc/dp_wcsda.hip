@@ -66,10 +66,11 @@ __device__ __forceinline__ R lerp_at(const double* ys, int i, R t) {
 }
 
 // cell_stopping_power, src/Tomography.jl:304-314.  With a shared energy grid
-// the three interpolations reuse one clamp test and one bracket search, which
+// the three interpolations reuse one clamp test and one interval search
+// (started from the guess g, the interval of the previous lookup), which
 // returns the same values as three separate lookups.
 template <class R>
-__device__ __forceinline__ R cell_stop(const WTab& T, R w, bool porous, R e) {
+__device__ __forceinline__ R cell_stop(const WTab& T, R w, bool porous, R e, int& g) {
   const int K = T.K;
   R le = log(e);
   R sr, sw, sp;
@@ -79,7 +80,8 @@ __device__ __forceinline__ R cell_stop(const WTab& T, R w, bool porous, R e) {
     } else if (val(e) >= T.rE[K - 1]) {
       sr = R(T.rS[K - 1]); sw = R(T.wS[K - 1]); sp = R(T.pS[K - 1]);
     } else {
-      const int i = bracket(T.rLE, K, T.pow2, val(le));
+      const int i = locate_idx(T.rLE, K, T.pow2, val(le), g);
+      g = i;
       R t = (le - T.rLE[i]) / (T.rLE[i + 1] - T.rLE[i]);
       sr = lerp_at(T.rS, i, t);
       sw = lerp_at(T.wS, i, t);
@@ -187,13 +189,25 @@ __device__ __forceinline__ R decay_r(const WTab& T, P path, R eavg) {
   return exp(-path / L);
 }
 
+// interp_r with the interval searched from the guess g (updated); locate_idx
+// returns the interval of the bisection, so values are those of interp_r.
 template <class R>
-__device__ __forceinline__ R air_stop(const WTab& T, R e) {
-  return interp_r(T.aE, T.aLE, T.aS, T.K, T.pow2, e, log(e));
+__device__ __forceinline__ R interp_rg(const double* es, const double* les, const double* ys, int n,
+                                       int pow2, R x, R lx, int& g) {
+  if (val(x) <= es[0]) return R(ys[0]);
+  if (val(x) >= es[n - 1]) return R(ys[n - 1]);
+  const int i = locate_idx(les, n, pow2, val(lx), g);
+  g = i;
+  R t = (lx - les[i]) / (les[i + 1] - les[i]);
+  return ys[i] + t * (ys[i + 1] - ys[i]);
 }
 template <class R>
-__device__ __forceinline__ R rock_stop(const WTab& T, R e) {
-  return interp_r(T.rE, T.rLE, T.rS, T.K, T.pow2, e, log(e));
+__device__ __forceinline__ R air_stop(const WTab& T, R e, int& g) {
+  return interp_rg(T.aE, T.aLE, T.aS, T.K, T.pow2, e, log(e), g);
+}
+template <class R>
+__device__ __forceinline__ R rock_stop(const WTab& T, R e, int& g) {
+  return interp_rg(T.rE, T.rLE, T.rS, T.K, T.pow2, e, log(e), g);
 }
 
 // propagate_csda_cell_segment, src/Tomography.jl:444-473.  The stopping power
@@ -201,10 +215,10 @@ __device__ __forceinline__ R rock_stop(const WTab& T, R e) {
 // function of the same energy), so it is evaluated once.
 template <class R>
 __device__ bool cell_segment(const WTab& T, const WMat& m, R w, bool porous, double distance,
-                             R& energy, R& weight) {
+                             R& energy, R& weight, int& g) {
   R density = cell_density(m, w, porous);
   R remaining = R(distance);
-  R dedx0 = cell_stop(T, w, porous, energy);
+  R dedx0 = cell_stop(T, w, porous, energy, g);
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
     R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
@@ -213,10 +227,10 @@ __device__ bool cell_segment(const WTab& T, const WMat& m, R w, bool porous, dou
     if (val(ds) <= 0.0) { weight = R(0.0); return false; }
     R gram = ds * density;
     R epred = energy + dedx0 * gram;
-    R dedx1 = cell_stop(T, w, porous, epred);
+    R dedx1 = cell_stop(T, w, porous, epred, g);
     R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
     if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
-    R dnext = cell_stop(T, w, porous, enext);
+    R dnext = cell_stop(T, w, porous, enext, g);
     weight = weight * (dnext / cmax(dedx0, 1e-30));
     weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
     energy = enext;
@@ -228,9 +242,10 @@ __device__ bool cell_segment(const WTab& T, const WMat& m, R w, bool porous, dou
 
 // propagate_csda_uniform_segment, src/Tomography.jl:481-509 (rock remainder).
 template <class R>
-__device__ bool rock_segment(const WTab& T, double density, double distance, R& energy, R& weight) {
+__device__ bool rock_segment(const WTab& T, double density, double distance, R& energy, R& weight,
+                             int& g) {
   R remaining = R(distance);
-  R dedx0 = rock_stop(T, energy);
+  R dedx0 = rock_stop(T, energy, g);
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
     R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
@@ -239,10 +254,10 @@ __device__ bool rock_segment(const WTab& T, double density, double distance, R& 
     if (val(ds) <= 0.0) { weight = R(0.0); return false; }
     R gram = ds * density;
     R epred = energy + dedx0 * gram;
-    R dedx1 = rock_stop(T, epred);
+    R dedx1 = rock_stop(T, epred, g);
     R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
     if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
-    R dnext = rock_stop(T, enext);
+    R dnext = rock_stop(T, enext, g);
     weight = weight * (dnext / cmax(dedx0, 1e-30));
     weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
     energy = enext;
@@ -257,10 +272,10 @@ __device__ bool rock_segment(const WTab& T, double density, double distance, R& 
 // src/Geometry.jl:95-106), 1 mm minimum step.
 template <class R>
 __device__ bool air_segment(const WTab& T, double z0, double cos_t, double distance, R& energy,
-                            R& weight) {
+                            R& weight, int& g) {
   R remaining = R(distance);
   R traversed = R(0.0);
-  R dedx0 = air_stop(T, energy);
+  R dedx0 = air_stop(T, energy, g);
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
     R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
@@ -272,10 +287,10 @@ __device__ bool air_segment(const WTab& T, double z0, double cos_t, double dista
     R density = 1.205 * exp(-alt / 12e3);
     R gram = ds * density;
     R epred = energy + dedx0 * gram;
-    R dedx1 = air_stop(T, epred);
+    R dedx1 = air_stop(T, epred, g);
     R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
     if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
-    R dnext = air_stop(T, enext);
+    R dnext = air_stop(T, enext, g);
     weight = weight * (dnext / cmax(dedx0, 1e-30));
     weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
     energy = enext;
@@ -432,6 +447,7 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
       const bool active = n < a.n_nodes;
       double E = active ? a.ne[n] : a.ne[0];
       double W = 1.0, V = 0.0, H = 0.0;
+      int gc = K / 2, ga = K / 2;  // interval guesses: cell/rock and air stopping power
       const bool acc = CORR && C.enabled;
       bool ok = active;  // lanes past the last sample walk nothing and add 0
       // ------------------------------------------------ forward sweep
@@ -444,7 +460,7 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
         if (JAC) {
           const D2 wd(wv, 1.0, 0.0);
           D2 e(E, 0.0, 1.0), g(W);
-          ok = cell_segment(T, M, wd, porous, d, e, g);
+          ok = cell_segment(T, M, wd, porous, d, e, g, gc);
           double* p = sc + (size_t)s * NT * bd + tid;
           p[0] = e.a;
           p[bd] = e.b;
@@ -470,7 +486,7 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
           W = g.v;
         } else {
           const double e_in = E;
-          ok = cell_segment(T, M, wv, porous, d, E, W);
+          ok = cell_segment(T, M, wv, porous, d, E, W, gc);
           if (CORR && acc && ok) {
             const double er = 0.5 * (e_in + E);
             double om, hl;
@@ -486,7 +502,7 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
         const double gx = rem * M.rho_r;
         if (JAC) {
           DualD e(E, 1.0), g(W);
-          ok = rock_segment(T, M.rho_r, rem, e, g);
+          ok = rock_segment(T, M.rho_r, rem, e, g, gc);
           rc = e.d; rl = g.d / g.v;
           if (CORR && acc && ok) {
             DualD er = 0.5 * (DualD(E, 1.0) + e);
@@ -498,7 +514,7 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
           E = e.v; W = g.v;
         } else {
           const double e_in = E;
-          ok = rock_segment(T, M.rho_r, rem, E, W);
+          ok = rock_segment(T, M.rho_r, rem, E, W, gc);
           if (CORR && acc && ok) {
             const double er = 0.5 * (e_in + E);
             double om, hl;
@@ -511,10 +527,10 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
       if (ok) {
         if (JAC) {
           DualD e(E, 1.0), g(W);
-          ok = air_segment(T, z0, ct, air_d, e, g);
+          ok = air_segment(T, z0, ct, air_d, e, g, ga);
           ac = e.d; al = g.d / g.v; E = e.v; W = g.v;
         } else {
-          ok = air_segment(T, z0, ct, air_d, E, W);
+          ok = air_segment(T, z0, ct, air_d, E, W, ga);
         }
       }
       double f = 0.0, Tad = 0.0, AV = 0.0, AH = 0.0;
