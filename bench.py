@@ -22,6 +22,12 @@ Legs
             Unit: rays*voxels/s.
   pathmc  : fixed-path straggled MC over 256 bins x 1e4 events of a lognormal
             density field with a cavity.  Unit: events/s (kernel time).
+  direct  : direct CSDA flux + density tangent for 2^24 samples.  Unit: samples/s.
+  wmodel_corrected : the wmodel leg with the MC-calibrated corrections on.
+  normal  : Gauss-Newton normal equations of the wmodel Jacobian over its
+            12 869 free cells (dense result).  Unit: GB/s of the dense matrix.
+  trace   : voxel traversal of the adjoint leg's 16384 rays.  Unit: rays*voxels/s.
+  dem     : ray exit over a 257x257 elevation map, 16384 rays.  Unit: rays/s.
 
 Inputs are resident in HBM before the timed region.  Timing: W warm-up
 steps, then K steps between device synchronisations; kernel time of the
@@ -44,11 +50,15 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy), from the counter
 # runs summarised in profiles/; `traffic` = this x the units of one launch.
 MEASURED_BYTES_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 1482041.0,   # per event
+    "mc_flat_stochastic_kernel": 435223.0,    # per event
     "csda_cube_kernel": 16467.0,              # per ray-voxel (all 256 nodes)
-    "wcsda_kernel": 2132.0,                   # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 312018.0,               # per event
+    "wcsda_kernel": 2131.0,                   # per ray-voxel (all 32 nodes)
+    "wcsda_kernel_corrected": 4310.0,         # per ray-voxel (all 32 nodes)
+    "path_mc_kernel": 312095.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
+    "trace_cube_kernel": 13.0,                # per ray-voxel (fill launch)
+    "ne_normal_kernel": 9.0,                  # per entry of the dense normal matrix
+    "ray_exit_kernel": 112.0,                 # per ray
 }
 FP64_PEAK_TFLOPS = 78.6
 
@@ -258,8 +268,142 @@ def lognormal_cavity(shape, seed=5):
     return rho
 
 
-def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32):
-    """w-model flux + sparse Jacobian over all tomography bins, device-resident."""
+def _timed(torch, stream, step, steps, warmup):
+    """W untimed steps, then K steps between device synchronisations on `stream`;
+    returns (wall ms per step, mean ms between the events each step records
+    around its dominant launch)."""
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(evs[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, sum(a.elapsed_time(b) for a, b in evs) / steps
+
+
+def run_trace(dp, L, physics, ctx, torch, steps, warmup):
+    """trace_cube_path for the adjoint workload's 16384 rays (128x128x64 voxels
+    of 10 m, 8 detectors x 2048 directions): count launch, device prefix sum of
+    the counts, fill launch.  The fill launch is the dominant kernel: 12 B per
+    ray*voxel written (cell 4, length 8) + 48 B per ray read (start 16,
+    direction 24, row offset 8)."""
+    import ctypes as C
+    from workloads import cube_rays
+    geo = dp.CubeGeometry(640.0, 128, 128, 64, physics.get_material_index("StandardRock"),
+                          physics.get_material_index("Air"), width=1280.0)
+    sxy, dirs, _ = cube_rays(geo, 8, 2048, seed=2024)
+    n = len(dirs)
+    dev = torch.device("cuda:0")
+    t_sxy = torch.as_tensor(sxy, dtype=torch.float64).to(dev)
+    t_dir = torch.as_tensor(dirs, dtype=torch.float64).to(dev)
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    z_exit = torch.zeros(n, dtype=torch.float64, device=dev)
+    row_ptr = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+    grid = geo.c_grid()
+    lib = ctx.lib
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream.cuda_stream)
+
+    def count():
+        L.check(lib.dp_trace_cube_dev(ctx.h, C.byref(grid), n, t_sxy.data_ptr(), t_dir.data_ptr(),
+                                      counts.data_ptr(), None, None, None, z_exit.data_ptr()))
+
+    with torch.cuda.stream(stream):
+        count()
+        n_seg = int(counts.sum().item())
+    cells = torch.zeros(max(n_seg, 1), dtype=torch.int32, device=dev)
+    lens = torch.zeros(max(n_seg, 1), dtype=torch.float64, device=dev)
+
+    def step(ev=None):
+        count()
+        row_ptr[1:].copy_(torch.cumsum(counts, 0, dtype=torch.int32))
+        if ev is not None:
+            ev[0].record(stream)
+        L.check(lib.dp_trace_cube_dev(ctx.h, C.byref(grid), n, t_sxy.data_ptr(), t_dir.data_ptr(),
+                                      None, row_ptr.data_ptr(), cells.data_ptr(), lens.data_ptr(),
+                                      None))
+        if ev is not None:
+            ev[1].record(stream)
+
+    ms_step, kern_ms = _timed(torch, stream, step, steps, warmup)
+    ctx.set_stream(None)
+    alg = 12.0 * n_seg + 48.0 * n
+    achieved = alg / (kern_ms * 1e-3) / 1e9
+    return {"value": n_seg / (ms_step * 1e-3), "unit": "rays*voxels/s", "ms_per_step": ms_step,
+            "kernel_ms": kern_ms, "rays": n, "segments": n_seg,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": MEASURED_BYTES_PER_UNIT["trace_cube_kernel"] * n_seg},
+            "workload": "trace_cube_path, 128x128x64 voxels of 10 m, 16384 rays (8 det x 2048 "
+                        "dirs, elevation 15-75 deg): count + prefix sum + fill",
+            "_problem": (geo, sxy, dirs)}
+
+
+def run_dem(dp, L, ctx, torch, steps, warmup, n_dirs=16384):
+    """ray_surface_exit_distance over a 257x257 diamond-square terrain (20 m
+    nodes, 400 m relief on a 300 m dome above 1200 m), detector at the map
+    centre at 963 m, n_dirs upward directions (zenith U[0, 75] deg).  41 B per
+    ray (direction in, distance, flag and exit height out); the map (0.5 MB)
+    stays in L2."""
+    import ctypes as C
+    import numpy as np
+    from workloads import diamond_square
+    n = 257
+    d = 20.0
+    xs = (np.arange(n) - (n - 1) / 2) * d
+    X, Y = np.meshgrid(xs, xs)
+    z = 1200.0 + diamond_square(8, 400.0, seed=17) + 300.0 * np.exp(-(X ** 2 + Y ** 2) / 1.5e6)
+    emap = dp.ElevationMap(n, n, xs[0] * 1e-3, xs[0] * 1e-3, d * 1e-3, d * 1e-3, z)
+    cmap, keep = emap.c()
+    rng = np.random.default_rng(23)
+    zen = np.radians(rng.uniform(0.0, 75.0, n_dirs))
+    az = np.radians(rng.uniform(0.0, 360.0, n_dirs))
+    dirs = np.stack([np.sin(zen) * np.sin(az), np.sin(zen) * np.cos(az), np.cos(zen)], axis=1)
+    dev = torch.device("cuda:0")
+    t_map = torch.as_tensor(keep.T.copy().ravel(), dtype=torch.float64).to(dev)
+    cmap.data = t_map.data_ptr()
+    t_dir = torch.as_tensor(dirs, dtype=torch.float64).to(dev)
+    dist = torch.zeros(n_dirs, dtype=torch.float64, device=dev)
+    found = torch.zeros(n_dirs, dtype=torch.uint8, device=dev)
+    ez = torch.zeros(n_dirs, dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream.cuda_stream)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        L.check(ctx.lib.dp_ray_surface_exit_dev(ctx.h, C.byref(cmap), 1e-3, 963.0, n_dirs,
+                                                t_dir.data_ptr(), 40.0, 25_000.0,
+                                                dist.data_ptr(), found.data_ptr(), ez.data_ptr()))
+        if ev is not None:
+            ev[1].record(stream)
+
+    ms_step, kern_ms = _timed(torch, stream, step, steps, warmup)
+    ctx.set_stream(None)
+    achieved = 41.0 * n_dirs / (kern_ms * 1e-3) / 1e9
+    return {"value": n_dirs / (ms_step * 1e-3), "unit": "rays/s", "ms_per_step": ms_step,
+            "kernel_ms": kern_ms, "rays": n_dirs, "found": float(found.float().mean().item()),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": MEASURED_BYTES_PER_UNIT["ray_exit_kernel"] * n_dirs},
+            "workload": "ray_surface_exit_distance, 257x257 diamond-square DEM (20 m nodes), "
+                        "16384 directions, 40 m march + 40 bisections",
+            "_problem": (dict(nx=n, ny=n, x0=emap.x0, y0=emap.y0, dx=emap.dx, dy=emap.dy,
+                              data=z), dirs)}
+
+
+def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32, corrections=False):
+    """w-model flux + sparse Jacobian over all tomography bins, device-resident.
+    With corrections=True the MC-calibrated CsdaCorrection and
+    CsdaMaterialCorrection are on (coefficients of the order a calibration
+    gives), which runs the corrected kernel instantiation."""
     import ctypes as C
     import numpy as np
     shape, paths = tomo_workload(dp)
@@ -287,6 +431,11 @@ def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32):
     mc = m.c()
     site = L.SiteConfig(1000.0, 3e4)
     h = physics.device_handle(ctx)
+    if corrections:
+        L.check(ctx.lib.dp_set_csda_correction(ctx.h, C.byref(L.CsdaCorrection(
+            1, 1.1, 0.6, 0.05, -0.02, 0.003, 0.1))))
+        L.check(ctx.lib.dp_set_csda_material_correction(ctx.h, C.byref(
+            L.CsdaMaterialCorrection(1, -0.8, 1.7))))
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
     ctx.set_stream(stream.cuda_stream)
@@ -302,20 +451,13 @@ def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32):
             ev[1].record(stream)
 
     log(f"wmodel: rays={paths.n_rays} segments={paths.n_segments} max_seg={paths.max_segments()}")
-    with torch.cuda.stream(stream):
-        for _ in range(warmup):
-            step()
-        torch.cuda.synchronize()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(steps)]
-        t0 = time.perf_counter()
-        for k in range(steps):
-            step(evs[k])
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+    ms_step, kern_ms = _timed(torch, stream, step, steps, warmup)
     ctx.set_stream(None)
-    ms_step = (t1 - t0) * 1e3 / steps
-    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    if corrections:
+        L.check(ctx.lib.dp_set_csda_correction(ctx.h, C.byref(L.CsdaCorrection(
+            0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0))))
+        L.check(ctx.lib.dp_set_csda_material_correction(ctx.h, C.byref(
+            L.CsdaMaterialCorrection(0, 0.0, 0.0))))
     units = paths.n_segments
     alg = 29.0 * units + 8.0 * paths.n_rays
     achieved = alg / (kern_ms * 1e-3) / 1e9
@@ -324,8 +466,91 @@ def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32):
             "ray_nodes_per_s": paths.n_rays * n_nodes / (ms_step * 1e-3),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "traffic": MEASURED_BYTES_PER_UNIT["wcsda_kernel"] * units},
+                         "traffic": MEASURED_BYTES_PER_UNIT[
+                             "wcsda_kernel_corrected" if corrections else "wcsda_kernel"] * units},
             "_problem": (paths, w, s, m)}
+
+
+def run_normal(dp, L, physics, ctx, torch, steps, warmup, min_crossings=4):
+    """Gauss-Newton normal equations Ja' (W .* Ja) and Ja' (W .* r) of the
+    w-model Jacobian of the tomography workload (4096 bins, 32 nodes), free
+    cells = those crossed by at least min_crossings rays, W = 1/obs with obs
+    the flux perturbed by 5 %.  The Jacobian is passed in its segment-aligned
+    device form.  The dense result dominates: 8 n_free^2 bytes written."""
+    import ctypes as C
+    import numpy as np
+    shape, paths = tomo_workload(dp)
+    n_cells = shape[0] * shape[1] * shape[2]
+    w = np.random.default_rng(9).uniform(0.0, 0.3, n_cells)
+    s = dp.csda_energy_quadrature(32, 1.0, 1e4)
+    m = dp.MaterialConfig(physics.get_material_index("StandardRock"),
+                          physics.get_material_index("Water"),
+                          physics.get_material_index("Air"), -1, 2650.0, 1000.0)
+    pred, jac = dp.flux_and_segment_jacobian(physics, np.zeros(n_cells, np.uint8), m,
+                                             dp.SiteConfig(1000.0, 3e4), paths, w, s, ctx=ctx)
+    rng = np.random.default_rng(44)
+    ok = np.isfinite(pred)
+    pred = np.where(ok, pred, 0.0)
+    obs = pred * (1.0 + 0.05 * rng.standard_normal(len(pred)))
+    W = np.where(ok, 1.0 / np.maximum(obs, 1e-30), 0.0)
+    res = pred - obs
+    kept = np.isfinite(jac) & (jac != 0.0)
+    crossings = np.bincount(paths.cells[kept], minlength=n_cells)
+    active = np.flatnonzero(crossings >= min_crossings).astype(np.int32)
+    na = len(active)
+    dev = torch.device("cuda:0")
+    T = lambda a, t: torch.as_tensor(np.ascontiguousarray(a), dtype=t).to(dev)
+    t_rp, t_col = T(paths.row_ptr, torch.int32), T(paths.cells, torch.int32)
+    t_val, t_W, t_r = T(jac, torch.float64), T(W, torch.float64), T(res, torch.float64)
+    t_act = T(active, torch.int32)
+    normal = torch.empty(na * na, dtype=torch.float64, device=dev)
+    grad = torch.empty(na, dtype=torch.float64, device=dev)
+    csr = L.Csr(paths.n_rays, n_cells, paths.n_segments, t_rp.data_ptr(), t_col.data_ptr(),
+                t_val.data_ptr())
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream.cuda_stream)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        L.check(ctx.lib.dp_normal_equations_dev(ctx.h, C.byref(csr), t_W.data_ptr(),
+                                                t_r.data_ptr(), na, t_act.data_ptr(),
+                                                normal.data_ptr(), grad.data_ptr()))
+        if ev is not None:
+            ev[1].record(stream)
+
+    ms_step, kern_ms = _timed(torch, stream, step, steps, warmup)
+    ctx.set_stream(None)
+    alg = 8.0 * na * na + 8.0 * na + 16.0 * paths.n_segments
+    achieved = alg / (kern_ms * 1e-3) / 1e9
+    return {"value": alg / (ms_step * 1e-3) / 1e9, "unit": "GB/s of normal matrix",
+            "ms_per_step": ms_step, "kernel_ms": kern_ms, "n_free": na,
+            "bins": paths.n_rays, "entries": int(kept.sum()),
+            "grad_norm": float(grad.norm().item()),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": MEASURED_BYTES_PER_UNIT["ne_normal_kernel"] * na * na},
+            "workload": f"normal equations of the w-model Jacobian (64x64x32, 4096 bins, 32 "
+                        f"nodes), {na} free cells (>= {min_crossings} crossings), weighted",
+            "_problem": (paths, jac, W, res, active, n_cells)}
+
+
+def cpu_normal_baseline(problem):
+    """The oracle's normal equations on the same system, one thread."""
+    import numpy as np
+    import scipy.sparse as sp
+    import _oracle as O
+    paths, jac, W, res, active, n_cells = problem
+    kept = np.isfinite(jac) & (jac != 0.0)
+    rows = np.repeat(np.arange(paths.n_rays), np.diff(paths.row_ptr))
+    J = sp.csr_matrix((jac[kept], (rows[kept], paths.cells[kept])), shape=(paths.n_rays, n_cells))
+    na = len(active)
+    t0 = time.perf_counter()
+    O.normal_equations(J, res, W, active)
+    dt = time.perf_counter() - t0
+    return {"value": 8.0 * na * na / dt / 1e9, "unit": "GB/s of normal matrix", "cores": 1,
+            "kind": "port", "sample": f"the full system ({na} free cells), {dt:.2f} s"}
 
 
 def run_direct(dp, L, physics, ctx, torch, steps, warmup, n=1 << 24):
@@ -434,8 +659,9 @@ def run_pathmc(dp, physics, ctx, n_rays=256, n_events=10000):
             "_problem": (paths, rho, s, m, site)}
 
 
-def cpu_wmodel_baseline(physics, problem, budget_s=10.0):
-    """The oracle's w-model flux + reverse Jacobian on a bounded sample of bins."""
+def cpu_wmodel_baseline(physics, problem, budget_s=10.0, corr=None):
+    """The oracle's w-model flux + reverse Jacobian on a bounded sample of bins
+    (corrections on when corr is given)."""
     import numpy as np
     import _oracle as O
     paths, w, s, m = problem
@@ -457,12 +683,45 @@ def cpu_wmodel_baseline(physics, problem, budget_s=10.0):
                    valid=paths.valid[idx])
         t0 = time.perf_counter()
         O.w_jacobian(physics, mc, (1000.0, 3e4), sub, w, np.zeros(len(w), np.uint8), s.energy,
-                     s.weight, s.charge, threads=threads)
+                     s.weight, s.charge, threads=threads, corr=corr)
         dt = time.perf_counter() - t0
         if dt > budget_s / 4 or n >= paths.n_rays:
             return {"value": rp[-1] / dt, "unit": "rays*voxels/s", "cores": threads,
                     "kind": "port", "sample": f"{len(idx)} of {paths.n_rays} bins, "
                                               f"{len(s.energy)} nodes, {dt:.2f} s"}
+        n *= 2
+
+
+def _wcorr_oracle():
+    """The corrected leg's coefficients in the oracle's layout."""
+    import _oracle as O
+    return O.corr_array(enabled=True, kappa_strag=1.1, kappa_hard=0.6, resid_a=0.05,
+                        resid_b=-0.02, resid_c=0.003, resid_d=0.1, material=(-0.8, 1.7))
+
+
+def cpu_trace_baseline(problem):
+    """The oracle's trace_cube_path over all rays of the trace leg, one thread."""
+    import _oracle as O
+    geo, sxy, dirs = problem
+    t0 = time.perf_counter()
+    rp, cells, lens, ze = O.trace_cube(geo, sxy, dirs)
+    dt = time.perf_counter() - t0
+    return {"value": len(cells) / dt, "unit": "rays*voxels/s", "cores": 1, "kind": "port",
+            "sample": f"all {len(dirs)} rays, {len(cells)} ray*voxels, {dt:.2f} s"}
+
+
+def cpu_dem_baseline(problem, budget_s=6.0):
+    """The oracle's ray_surface_exit_distance on a prefix of the directions, one thread."""
+    import _oracle as O
+    dem, dirs = problem
+    n = 256
+    while True:
+        t0 = time.perf_counter()
+        O.ray_surface_exit(dem, dirs[:n], z_ref=963.0)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= len(dirs):
+            return {"value": n / dt, "unit": "rays/s", "cores": 1, "kind": "port",
+                    "sample": f"first {n} directions, {dt:.2f} s"}
         n *= 2
 
 
@@ -527,11 +786,30 @@ def main():
     log(f"direct: {dr['value']:.4g} samples/s, kernel {dr['kernel_ms']:.3f} ms, "
         f"{dr['roofline']['achieved']:.1f} GB/s")
     dprob = dr.pop("_problem")
+    wc = run_wmodel(dp, L, physics, ctx, torch, a.steps, a.warmup, corrections=True)
+    wcprob = wc.pop("_problem")
+    log(f"wmodel+corrections: {wc['value']:.4g} rays*voxels/s, {wc['ms_per_step']:.3f} ms/step")
+    ne = run_normal(dp, L, physics, ctx, torch, a.steps, a.warmup)
+    log(f"normal: {ne['n_free']} free cells, {ne['kernel_ms']:.3f} ms, "
+        f"{ne['roofline']['achieved']:.1f} GB/s")
+    neprob = ne.pop("_problem")
+    tr = run_trace(dp, L, physics, ctx, torch, a.steps, a.warmup)
+    log(f"trace: {tr['value']:.4g} rays*voxels/s, fill kernel {tr['kernel_ms']:.4f} ms, "
+        f"{tr['roofline']['achieved']:.1f} GB/s")
+    tprob = tr.pop("_problem")
+    dm = run_dem(dp, L, ctx, torch, a.steps, a.warmup)
+    log(f"dem: {dm['value']:.4g} rays/s, kernel {dm['kernel_ms']:.4f} ms, found {dm['found']:.3f}")
+    demprob = dm.pop("_problem")
     cpu = None if a.no_cpu else cpu_mc_baseline(physics)
     cpu_adj = None if a.no_cpu else cpu_adjoint_baseline(physics, problem)
     wm["cpu_baseline"] = None if a.no_cpu else cpu_wmodel_baseline(physics, wprob)
     pm["cpu_baseline"] = None if a.no_cpu else cpu_pathmc_baseline(physics, pprob)
     dr["cpu_baseline"] = None if a.no_cpu else cpu_direct_baseline(physics, dprob)
+    wc["cpu_baseline"] = None if a.no_cpu else cpu_wmodel_baseline(physics, wcprob,
+                                                                   corr=_wcorr_oracle())
+    tr["cpu_baseline"] = None if a.no_cpu else cpu_trace_baseline(tprob)
+    ne["cpu_baseline"] = None if a.no_cpu else cpu_normal_baseline(neprob)
+    dm["cpu_baseline"] = None if a.no_cpu else cpu_dem_baseline(demprob)
     out = {
         "metric": "straggled backward-MC events/s (+ adjoint rays*voxels/s), 1 GPU, % HBM roofline",
         "value": mc["value"], "unit": "events/s", "n_gpus": 1, "steps": a.steps,
@@ -559,6 +837,12 @@ def main():
         "direct": dict(dr, workload="direct CSDA flux + d/d rho (compute_flux_gradient_csda), "
                                     "2^24 samples: rock 10-800 m at 2000-3000 kg/m3, "
                                     "elevation 10-90 deg, E 1-1e4 GeV"),
+        "wmodel_corrected": dict(wc, workload="the wmodel leg with CsdaCorrection (kappa 1.1/0.6, "
+                                              "residual 0.05/-0.02/0.003/0.1) and "
+                                              "CsdaMaterialCorrection (-0.8, 1.7) on"),
+        "trace": tr,
+        "dem": dm,
+        "normal": ne,
     }
     if a.calib_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)

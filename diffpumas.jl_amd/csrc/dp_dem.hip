@@ -45,7 +45,7 @@ __device__ __forceinline__ double surface_z(const DemArgs& a, double xm, double 
   return e - a.z_ref;
 }
 
-__global__ __launch_bounds__(256) void ray_exit_kernel(DemArgs a) {
+__global__ __launch_bounds__(64) void ray_exit_kernel(DemArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const double d0 = a.dirs[3 * i], d1 = a.dirs[3 * i + 1], d2 = a.dirs[3 * i + 2];
@@ -100,7 +100,7 @@ int launch_dem(dp_ctx c, const dp_elevation_map* m, double xy_scale, double z_re
   a.n_steps = (long)floor(max_distance / step + 1e-9);
   a.n = n; a.dirs = dirs; a.dist = dist; a.found = found; a.exit_z = ez;
   (void)hipEventRecord(c->ev0, c->stream);
-  hipLaunchKernelGGL(ray_exit_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(ray_exit_kernel, dim3((n + 63) / 64), dim3(64), 0, c->stream, a);
   (void)hipEventRecord(c->ev1, c->stream);
   DP_HIP(hipGetLastError());
   return DP_OK;

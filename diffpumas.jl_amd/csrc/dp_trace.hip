@@ -31,7 +31,7 @@ __device__ __forceinline__ bool cube_cell(const CubeDev& g, double x, double y, 
 // distance_to_boundary (:194-280) inlined.  The cell found at the top of an
 // iteration is the cell distance_to_boundary looks up again at the same
 // point, so it is computed once.
-__global__ __launch_bounds__(256) void trace_cube_kernel(CubeDev g, int n_rays,
+__global__ __launch_bounds__(64) void trace_cube_kernel(CubeDev g, int n_rays,
                                                          const double* __restrict__ start_xy,
                                                          const double* __restrict__ dir,
                                                          int32_t* __restrict__ counts,
@@ -102,7 +102,7 @@ __device__ __forceinline__ int ssl(const double* __restrict__ e, int n, double x
 
 // trace_path(::VoxelVolume), examples/lvd_tomography.jl:738-779, with
 // voxel_cell_indices (:583-594) and distance_to_voxel_boundary (:596-627).
-__global__ __launch_bounds__(256) void trace_edges_kernel(
+__global__ __launch_bounds__(64) void trace_edges_kernel(
     const double* __restrict__ xe, int nxe, const double* __restrict__ ye, int nye,
     const double* __restrict__ ze, int nze, int n_rays, const double* __restrict__ origin,
     const double* __restrict__ dir, const double* __restrict__ surface,
@@ -163,7 +163,8 @@ int launch_cube(dp_ctx c, const dp_cube_grid* grid, int32_t n, const double* sxy
   CubeDev g{grid->x_min, grid->x_max, grid->y_min, grid->y_max, grid->z_min, grid->z_max,
             grid->nx, grid->ny, grid->nz, grid->dx, grid->dy, grid->dz};
   if (n == 0) return DP_OK;
-  dim3 b(256), gr((n + 255) / 256);
+  // 64-lane blocks: 16 384 rays still give one wave to every CU
+  dim3 b(64), gr((n + 63) / 64);
   (void)hipEventRecord(c->ev0, c->stream);
   hipLaunchKernelGGL(trace_cube_kernel, gr, b, 0, c->stream, g, n, sxy, dir, counts, row_ptr,
                      cells, lens, z_exit);
@@ -276,7 +277,7 @@ int dp_trace_edges(dp_ctx c, const double* xe, int32_t nxe, const double* ye, in
   DP_HIP(hipMemcpyAsync(b + o_sd, surface, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
   if (cells) DP_HIP(hipMemcpyAsync(b + o_rp, row_ptr, 4 * (size_t)(n + 1), hipMemcpyHostToDevice, c->stream));
   (void)hipEventRecord(c->ev0, c->stream);
-  hipLaunchKernelGGL(trace_edges_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+  hipLaunchKernelGGL(trace_edges_kernel, dim3((n + 63) / 64), dim3(64), 0, c->stream,
                      (double*)(b + o_x), nxe, (double*)(b + o_y), nye, (double*)(b + o_z), nze, n,
                      (double*)(b + o_or), (double*)(b + o_dir), (double*)(b + o_sd),
                      cells ? nullptr : (int32_t*)(b + o_cnt), (int32_t*)(b + o_rp),

@@ -162,6 +162,11 @@ class Mixture(C.Structure):
                 ("fractions", C.c_double * 4)]
 
 
+class Csr(C.Structure):
+    _fields_ = [("n_rows", C.c_int32), ("n_cols", C.c_int32), ("nnz", C.c_int32),
+                ("row_ptr", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p)]
+
+
 # Exported symbols and their argument types.
 _SIGS = {
     "dp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -220,6 +225,10 @@ _SIGS = {
                                               C.POINTER(SiteConfig), C.POINTER(DirectionalPaths),
                                               C.c_int32, C.c_void_p, C.c_void_p,
                                               C.POINTER(EnergySamples), C.c_void_p, C.c_void_p]),
+    "dp_normal_equations": (C.c_int, [C.c_void_p, C.POINTER(Csr)] + [C.c_void_p] * 2 +
+                            [C.c_int32] + [C.c_void_p] * 3),
+    "dp_normal_equations_dev": (C.c_int, [C.c_void_p, C.POINTER(Csr)] + [C.c_void_p] * 2 +
+                                [C.c_int32] + [C.c_void_p] * 3),
     "dp_build_material_table": (C.c_int, [C.POINTER(MaterialRecipe), C.POINTER(DedxColumns),
                                           C.POINTER(PhysicsSettings), dbl_p]),
     "dp_build_composite_table": (C.c_int, [C.c_int32, C.POINTER(dbl_p), dbl_p, C.c_int32,

@@ -212,6 +212,21 @@ class DirectionalPaths:
     def max_segments(self):
         return int(np.diff(self.row_ptr).max()) if self.n_rays else 0
 
+    def select(self, bins):
+        """The paths of the given bins (0-based), in that order (paths[valid_bins])."""
+        bins = np.asarray(bins, np.int64).reshape(-1)
+        if bins.size and (bins.min() < 0 or bins.max() >= self.n_rays):
+            raise ArgumentError("bin index out of range")
+        a, b = self.row_ptr[bins], self.row_ptr[bins + 1]
+        take = np.concatenate([np.arange(x, y) for x, y in zip(a, b)]) if bins.size else \
+            np.zeros(0, np.int64)
+        rp = np.concatenate([[0], np.cumsum(b - a)])
+        pick = lambda v: None if v is None else np.asarray(v)[bins]
+        return DirectionalPaths(rp, self.cells[take], self.lengths[take], self.zenith[bins],
+                                self.azimuth[bins], self.remaining_rock[bins], self.exit_z[bins],
+                                self.valid[bins], pick(self.distance_in_volume),
+                                pick(self.surface_distance))
+
     def c_paths(self):
         return _lib.DirectionalPaths(self.n_rays, self.n_segments, self.max_segments(),
                                      ptr(self.row_ptr, C.c_int32), ptr(self.cells, C.c_int32),

@@ -431,6 +431,39 @@ int dp_mc_fixed_paths(dp_ctx ctx, dp_tables tables, const dp_material_config* ma
                       double energy_threshold_low, double* flux, double* sigma,
                       double* event_weights);
 
+/* ------------------------------------- Gauss-Newton normal equations
+ * Sparse matrix in CSR form, rows = bins, columns = cells (0-based).       */
+typedef struct {
+  int32_t n_rows, n_cols;
+  int32_t nnz;               /* = row_ptr[n_rows]                          */
+  const int32_t* row_ptr;    /* n_rows + 1, row_ptr[0] = 0                 */
+  const int32_t* col;        /* nnz; strictly ascending within a row       */
+  const double* val;         /* nnz                                        */
+} dp_csr;
+
+/* Replaces: the normal-equation assembly of gauss_newton_reconstruct
+ * (src/Tomography_inverse.jl:489-494, final gradient :595) and the Fisher
+ * information of laplace_posterior (:922-924):
+ *   Ja = J[:, active]; normal = Matrix(Ja' * (W .* Ja)); grad = Ja' * (W .* r)
+ * (weights == NULL: Matrix(Ja' * Ja) and Ja' * r).  Entries that are zero or
+ * not finite are skipped, as assemble_forward_and_jacobian
+ * (src/Tomography.jl:806-818) drops them.  active[n_active]: ascending,
+ * distinct columns; NULL with n_active = n_cols means every column.
+ * normal (n_active x n_active, column-major as Julia's Matrix) or grad may be
+ * NULL.  Every entry is summed over rows in ascending order, the order of the
+ * reference's sparse product, so results are deterministic.  The _dev form
+ * takes device pointers (J's arrays, weights, residual, active, outputs),
+ * trusts J's structure and returns without synchronising; there the columns
+ * of a row need not be sorted (the w-model's segment-aligned Jacobian, whose
+ * repeated cells carry zeros, can be passed as is) but a column may hold at
+ * most one kept entry per row.                                              */
+int dp_normal_equations(dp_ctx ctx, const dp_csr* J, const double* weights,
+                        const double* residual, int32_t n_active, const int32_t* active,
+                        double* normal, double* grad);
+int dp_normal_equations_dev(dp_ctx ctx, const dp_csr* J, const double* weights,
+                            const double* residual, int32_t n_active, const int32_t* active,
+                            double* normal, double* grad);
+
 /* Last kernel duration in ms, timed with HIP events on the context's stream
  * around the dominant kernel of the previous call (0 if none). */
 double dp_last_kernel_ms(dp_ctx ctx);

@@ -4,6 +4,7 @@
 #include "oracle_geometry.hpp"
 #include "oracle_philox.hpp"
 #include "oracle_flat_mc.hpp"
+#include "oracle_inverse.hpp"
 #include <cstring>
 #include <map>
 #include <omp.h>
@@ -419,6 +420,12 @@ void orc_ray_surface_exit(int nx, int ny, double x0, double y0, double dx, doubl
     ray_surface_exit(m, xy_scale, z_ref, dirs + 3 * i, step, max_distance, dist[i], f, exit_z[i]);
     found[i] = f;
   }
+}
+// Gauss-Newton normal equations (src/Tomography_inverse.jl:489-494).
+void orc_normal_equations(int n_rows, int n_cols, const int* rp, const int* col, const double* val,
+                          const double* weights, const double* residual, int n_act,
+                          const int* active, double* normal, double* grad) {
+  normal_equations(n_rows, n_cols, rp, col, val, weights, residual, n_act, active, normal, grad);
 }
 // map_elevation at n points: out = elevation, inside.
 void orc_dem_elevation(int nx, int ny, double x0, double y0, double dx, double dy,

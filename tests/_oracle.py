@@ -590,3 +590,25 @@ def mc_uniform_segment(ph, material, density, distance, state, straggling=True, 
                                       P(A), P(I), P(f), cutoff, material, density, distance,
                                       int(straggling), thr_low, seed, P(st))
     return rc, dict(E=st[0], w=st[1], t=st[2], L=st[3], X=st[4])
+
+
+def normal_equations(J, residual=None, weights=None, active=None, normal=True):
+    """(normal, grad) of the Gauss-Newton system (gauss_newton_reconstruct,
+    src/Tomography_inverse.jl:489-494) in the reference's summation order.
+    J: scipy CSR (columns ascending within rows); active: ascending column ids
+    or None (every column)."""
+    import scipy.sparse as sp
+    J = sp.csr_matrix(J, dtype=np.float64, copy=True)
+    J.sort_indices()
+    rp, col, val = i32(J.indptr), i32(J.indices), f64(J.data)
+    n_rows, n_cols = J.shape
+    act = None if active is None else i32(active)
+    n_act = n_cols if act is None else len(act)
+    out_n = np.zeros(n_act * n_act) if normal else None
+    out_g = np.zeros(n_act) if residual is not None else None
+    w = None if weights is None else f64(weights)
+    r = None if residual is None else f64(residual)
+    q = lambda x, t=C.c_double: None if x is None else P(x, t)
+    lib().orc_normal_equations(n_rows, n_cols, P(rp, C.c_int), P(col, C.c_int), P(val), q(w),
+                               q(r), n_act, q(act, C.c_int), q(out_n), q(out_g))
+    return (None if out_n is None else out_n.reshape(n_act, n_act).T), out_g

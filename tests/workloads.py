@@ -58,3 +58,30 @@ def lognormal_cavity_densities(geo, seed, mean=2650.0, sigma=0.1, cavity=1800.0)
     inside = ((ix - cx) / rx) ** 2 + ((iy - cy) / ry) ** 2 + ((iz - cz) / rz) ** 2 <= 1.0
     rho[inside] = cavity
     return rho
+
+
+def diamond_square(n_pow, relief, seed, roughness=0.55):
+    """Seeded diamond-square height field on (2^n_pow + 1)^2 nodes, shifted and
+    scaled to span [0, relief] m.  h[iy, ix]."""
+    rng = np.random.default_rng(seed)
+    n = 2 ** n_pow + 1
+    h = np.zeros((n, n))
+    h[0, 0], h[0, -1], h[-1, 0], h[-1, -1] = rng.uniform(-1.0, 1.0, 4)
+    step, amp = n - 1, 1.0
+    while step > 1:
+        half = step // 2
+        for i in range(half, n, step):
+            for j in range(half, n, step):
+                h[i, j] = 0.25 * (h[i - half, j - half] + h[i - half, j + half] +
+                                  h[i + half, j - half] + h[i + half, j + half])
+                h[i, j] += amp * rng.uniform(-1.0, 1.0)
+        for i in range(0, n, half):
+            for j in range((i + half) % step, n, step):
+                s, k = 0.0, 0
+                for a, b in ((i - half, j), (i + half, j), (i, j - half), (i, j + half)):
+                    if 0 <= a < n and 0 <= b < n:
+                        s += h[a, b]
+                        k += 1
+                h[i, j] = s / k + amp * rng.uniform(-1.0, 1.0)
+        step, amp = half, amp * roughness
+    return (h - h.min()) / (h.max() - h.min()) * relief
