@@ -16,7 +16,9 @@
 #include <hip/hip_runtime.h>
 #define DPM_HD __host__ __device__ inline
 #define DPM_HD_COLD __host__ __device__ inline __attribute__((noinline))
+#define DPM_HD_HOT __host__ __device__ inline __attribute__((always_inline))
 #else
+#define DPM_HD_HOT inline
 #define DPM_HD inline
 #define DPM_HD_COLD inline
 #endif
@@ -69,7 +71,7 @@ DPM_HD double gq12_w(int i) {
 
 // ------------------------------------------------------ dilogarithm
 // Chebyshev expansion of PUMAS math_dilog (src/Materials.jl:272-339).
-DPM_HD double dilog(double x) {
+DPM_HD_HOT double dilog(double x) {
   const double c[20] = {0.42996693560813697, 0.40975987533077105, -0.01858843665014592,
                         0.00145751084062268, -0.00014304184442340, 0.00001588415541880,
                         -0.00000190784959387, 0.00000024195180854, -0.00000003193341274,
@@ -278,7 +280,10 @@ DPM_HD_COLD double dcs_pair(double Z, double A, double mass, double K, double q)
   if (ri <= 0.0 || ri >= 1.0) return 0.0;
   const double tmin = log(ri);
   double I = 0.0;
-#pragma unroll
+  // Rolled on purpose: twelve inlined copies of the DDCS overflow the device
+  // register file and spill around every node; one rolled body with the
+  // dilogarithm inlined keeps its state in registers.
+#pragma unroll 1
   for (int i = 0; i < 12; ++i) {
     const double r = exp(gq12_x(i) * tmin);
     I -= dcs_pair_d2(Z, A, mass, K, q, r) * r * gq12_w(i) * tmin;
@@ -349,7 +354,7 @@ DPM_HD_COLD double dcs_photo(double Z, double A, double m, double K, double q) {
   const double lmin = log(Q2min);
   const double dx = log(Q2max) - lmin;
   double ds = 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int i = 0; i < 9; ++i) {
     const double Q2 = exp(lmin + dx * gq9_x(i));
     const double v = photo_d2(Z, A, m, K, q, Q2);

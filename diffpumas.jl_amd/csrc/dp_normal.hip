@@ -16,7 +16,8 @@
 //      rows of column j in that order and, for each, adds the row's products
 //      into an LDS accumulator (entries of one row have distinct columns, so
 //      the lanes never collide); the block then writes its tile of column j
-//      of the column-major result, zeros included;
+//      of the column-major result, zeros included (row bounds are fetched 64
+//      rows at a time and the next row's entries are loaded ahead);
 //   4. one lane per active column sums the gradient in the same order.
 //
 // Every output is accumulated in the reference's order with separate
@@ -77,6 +78,19 @@ __global__ void ne_col_ptr(int n_active, int nnz, const int32_t* __restrict__ sk
   col_ptr[a] = lo;
 }
 
+__device__ __forceinline__ int lane_of(int x, int k) { return __builtin_amdgcn_readlane(x, k); }
+__device__ __forceinline__ double lane_of(double x, int k) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), k),
+                          __builtin_amdgcn_readlane(__double2loint(x), k));
+}
+
+// One wave per (column j, tile).  The rows of column j are taken 64 at a time:
+// each lane fetches one row's bounds and (W .* Ja)[r, j], then the wave walks
+// those rows in order, reading each row's bounds from the owning lane.  The
+// first 64 entries of the next row are loaded before the current row is added,
+// so the global-load latency overlaps the LDS updates.  The block is a single
+// wave: its LDS operations complete in issue order, so row k's additions
+// precede row k+1's and every entry keeps the ascending-row order.
 __global__ __launch_bounds__(64) void ne_normal_kernel(
     int n_active, const int32_t* __restrict__ row_ptr, const double* __restrict__ val,
     const double* __restrict__ weights, const int32_t* __restrict__ key,
@@ -89,18 +103,42 @@ __global__ __launch_bounds__(64) void ne_normal_kernel(
   const int lane = threadIdx.x;
   for (int q = lane; q < tile; q += 64) acc[q] = 0.0;
   __syncthreads();
-  const int s_end = col_ptr[j + 1];
-  for (int s = col_ptr[j]; s < s_end; ++s) {
-    const int e = sent[s];
-    const int r = row_of[e];
-    const double bj = weights ? weights[r] * val[e] : val[e];   // (W .* Ja)[r, j]
-    const int e_end = row_ptr[r + 1];
-    for (int e2 = row_ptr[r] + lane; e2 < e_end; e2 += 64) {
-      const int i = key[e2] - i0;
-      if (i >= 0 && i < tile) acc[i] += val[e2] * bj;
+  const int s_beg = col_ptr[j], s_end = col_ptr[j + 1];
+  for (int s0 = s_beg; s0 < s_end; s0 += 64) {
+    const int cnt = min(64, s_end - s0);
+    int rb = 0, re = 0;
+    double bj = 0.0;
+    if (lane < cnt) {
+      const int e = sent[s0 + lane];
+      const int r = row_of[e];
+      rb = row_ptr[r];
+      re = row_ptr[r + 1];
+      bj = weights ? weights[r] * val[e] : val[e];   // (W .* Ja)[r, j]
     }
-    __syncthreads();
+    int b = lane_of(rb, 0), en = lane_of(re, 0);
+    int kk = -1;
+    double vv = 0.0;
+    if (b + lane < en) { kk = key[b + lane]; vv = val[b + lane]; }
+    for (int k = 0; k < cnt; ++k) {
+      const double m = lane_of(bj, k);
+      int nb = 0, nen = 0, nk = -1;
+      double nv = 0.0;
+      if (k + 1 < cnt) {
+        nb = lane_of(rb, k + 1);
+        nen = lane_of(re, k + 1);
+        if (nb + lane < nen) { nk = key[nb + lane]; nv = val[nb + lane]; }
+      }
+      const int i = kk - i0;
+      if (kk >= 0 && i >= 0 && i < tile) acc[i] += vv * m;
+      for (int e3 = b + 64 + lane; e3 < en; e3 += 64) {   // rows longer than 64 entries
+        const int i3 = key[e3] - i0;
+        if (i3 >= 0 && i3 < tile) acc[i3] += val[e3] * m;
+      }
+      __builtin_amdgcn_wave_barrier();
+      b = nb; en = nen; kk = nk; vv = nv;
+    }
   }
+  __syncthreads();
   double* out = normal + (size_t)j * n_active + i0;
   for (int q = lane; q < tile; q += 64) out[q] = acc[q];
 }

@@ -82,6 +82,20 @@ def test_multi_tile_and_long_column(ctx):
     assert np.array_equal(N, No) and np.array_equal(g, go)
 
 
+def test_long_rows(ctx):
+    """Rows longer than one wave (up to 200 entries) and columns with more than
+    64 rows: the row walk fetches row bounds in batches of 64 and each row in
+    chunks of 64 entries."""
+    J = _rand_csr(700, 1500, 200, 12, dense_col=3)
+    rng = np.random.default_rng(13)
+    W, r = rng.uniform(0.2, 3.0, 700), rng.normal(0.0, 1.0, 700)
+    act = np.flatnonzero(rng.random(1500) < 0.7)
+    _say(f"normal eqs (long rows): {J.shape}, {J.nnz} entries, {len(act)} active")
+    N, g = dp.normal_equations(J, r, W, act, ctx=ctx)
+    No, go = O.normal_equations(J, r, W, act)
+    assert np.array_equal(N, No) and np.array_equal(g, go)
+
+
 def test_device_entry_point(ctx):
     from _hip import DeviceArray as D
     J = _rand_csr(400, 300, 20, 8)
