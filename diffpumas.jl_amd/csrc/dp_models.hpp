@@ -185,24 +185,39 @@ DPM_HD_COLD double dcs_brems(double Z, double A, double m, double K, double q) {
 }
 
 // ----------------------------------------------------- pair production
-// _dcs_pair_production_d2_ssr, src/Materials.jl:701-800
-DPM_HD double dcs_pair_d2(double Z, double A, double mass, double K, double q, double rho_in) {
-  const double ME = 0.5109989461, RE = 2.8179403227e-13;
-  const double energy = (K + mass) * 1e3;
-  const double v = q / (K + mass);
-  const double m = mass * 1e3;
-  const double rl = radiation_logarithm(Z);
-  const double cpf = 4.0 / (3.0 * PI) * Z * (ALPHA_EM * RE) * (ALPHA_EM * RE);
-  const double Z13 = pow(Z, -1.0 / 3.0);
-  const double d_n = 1.54 * pow(A, 0.27);
-  const double rho = 1.0 - rho_in;
-  const double rho2 = rho * rho;
+// _dcs_pair_production_d2_ssr, src/Materials.jl:701-800, split into the terms
+// that do not depend on the quadrature variable (PairInv, evaluated once per
+// dcs_pair call) and the per-node part.  Each term is the same expression as
+// in the single-function form, so the values are unchanged.
+struct PairInv {
+  double Z, energy, v, m, rl, cpf, Z13, d_n, zeta, beta;
+};
+DPM_HD PairInv pair_invariants(double Z, double A, double mass, double K, double q) {
+  const double RE = 2.8179403227e-13;
+  PairInv c;
+  c.Z = Z;
+  c.energy = (K + mass) * 1e3;
+  c.v = q / (K + mass);
+  c.m = mass * 1e3;
+  c.rl = radiation_logarithm(Z);
+  c.cpf = 4.0 / (3.0 * PI) * Z * (ALPHA_EM * RE) * (ALPHA_EM * RE);
+  c.Z13 = pow(Z, -1.0 / 3.0);
+  c.d_n = 1.54 * pow(A, 0.27);
+  const double energy = c.energy, m = c.m, Z13 = c.Z13, v = c.v;
   const double g1 = Z == 1.0 ? 4.4e-5 : 1.95e-5;
   const double g2 = Z == 1.0 ? 4.8e-5 : 5.3e-5;
   const double zeta1 = 0.073 * log(energy / m / (1.0 + g1 * pow(Z, 2.0 / 3.0) * energy / m)) - 0.26;
   const double zeta2 = 0.058 * log(energy / m / (1.0 + g2 / Z13 * energy / m)) - 0.14;
-  const double zeta = (zeta1 > 0.0 && zeta2 > 0.0) ? zeta1 / zeta2 : 0.0;
-  const double beta = v * v / (2.0 * (1.0 - v));
+  c.zeta = (zeta1 > 0.0 && zeta2 > 0.0) ? zeta1 / zeta2 : 0.0;
+  c.beta = v * v / (2.0 * (1.0 - v));
+  return c;
+}
+DPM_HD double dcs_pair_d2(const PairInv& c, double rho_in) {
+  const double ME = 0.5109989461;
+  const double Z = c.Z, energy = c.energy, v = c.v, m = c.m, rl = c.rl, cpf = c.cpf;
+  const double Z13 = c.Z13, d_n = c.d_n, zeta = c.zeta, beta = c.beta;
+  const double rho = 1.0 - rho_in;
+  const double rho2 = rho * rho;
   const double xi = (m * v / (2.0 * ME)) * (m * v / (2.0 * ME)) * (1.0 - rho2) / (1.0 - v);
   const double lxi = log(1.0 + 1.0 / xi);
   const double Be = ((2.0 + rho2) * (1.0 + beta) + xi * (3.0 + rho2)) * lxi +
@@ -279,6 +294,7 @@ DPM_HD_COLD double dcs_pair(double Z, double A, double mass, double K, double q)
   const double ri = 1.0 - rmax;
   if (ri <= 0.0 || ri >= 1.0) return 0.0;
   const double tmin = log(ri);
+  const PairInv c = pair_invariants(Z, A, mass, K, q);
   double I = 0.0;
   // Rolled on purpose: twelve inlined copies of the DDCS overflow the device
   // register file and spill around every node; one rolled body with the
@@ -286,7 +302,7 @@ DPM_HD_COLD double dcs_pair(double Z, double A, double mass, double K, double q)
 #pragma unroll 1
   for (int i = 0; i < 12; ++i) {
     const double r = exp(gq12_x(i) * tmin);
-    I -= dcs_pair_d2(Z, A, mass, K, q, r) * r * gq12_w(i) * tmin;
+    I -= dcs_pair_d2(c, r) * r * gq12_w(i) * tmin;
   }
   return I < 0.0 ? 0.0 : I;
 }
@@ -307,12 +323,15 @@ DPM_HD double f2p_allm97(double x, double Q2) {
   const double t = log(log((Q2 + Q02) / L2) / log(Q02 / L2));
   const double xP = (Q2 + mP2) / (Q2 + mP2 + W2 - M2);
   const double xR = (Q2 + mR2) / (Q2 + mR2 + W2 - M2);
-  const double cP = cP1 + (cP1 - cP2) * (1.0 / (1.0 + pow(t, cP3)) - 1.0);
-  const double aP = aP1 + (aP1 - aP2) * (1.0 / (1.0 + pow(t, aP3)) - 1.0);
-  const double bP = bP1 + bP2 * pow(t, bP3);
-  const double cR = cR1 + cR2 * pow(t, cR3);
-  const double aR = aR1 + aR2 * pow(t, aR3);
-  const double bR = bR1 + bR2 * pow(t, bR3);
+  // t^c as exp(c log t): the reference evaluates these powers under @fastmath,
+  // which permits this form; t >= 0, and t = 0 gives 0 either way.
+  const double lt = log(t);
+  const double cP = cP1 + (cP1 - cP2) * (1.0 / (1.0 + exp(cP3 * lt)) - 1.0);
+  const double aP = aP1 + (aP1 - aP2) * (1.0 / (1.0 + exp(aP3 * lt)) - 1.0);
+  const double bP = bP1 + bP2 * exp(bP3 * lt);
+  const double cR = cR1 + cR2 * exp(cR3 * lt);
+  const double aR = aR1 + aR2 * exp(aR3 * lt);
+  const double bR = bR1 + bR2 * exp(bR3 * lt);
   const double F2P = cP * exp(aP * log(xP) + bP * log(1.0 - x));
   const double F2R = cR * exp(aR * log(xR) + bR * log(1.0 - x));
   return Q2 / (Q2 + m02) * (F2P + F2R);
