@@ -28,6 +28,7 @@ Legs
             12 869 free cells (dense result).  Unit: GB/s of the dense matrix.
   trace   : voxel traversal of the adjoint leg's 16384 rays.  Unit: rays*voxels/s.
   dem     : ray exit over a 257x257 elevation map, 16384 rays.  Unit: rays/s.
+  tetra   : ray walk through a 331 776-tet mesh, 16384 rays.  Unit: rays*tets/s.
 
 Inputs are resident in HBM before the timed region.  Timing: W warm-up
 steps, then K steps between device synchronisations; kernel time of the
@@ -343,6 +344,61 @@ def run_trace(dp, L, physics, ctx, torch, steps, warmup):
             "workload": "trace_cube_path, 128x128x64 voxels of 10 m, 16384 rays (8 det x 2048 "
                         "dirs, elevation 15-75 deg): count + prefix sum + fill",
             "_problem": (geo, sxy, dirs)}
+
+
+def run_tetra(dp, L, ctx, steps, warmup):
+    """trace_path(::TetraVolume) over a jittered Kuhn mesh of 48x48x24 cubes of
+    10 m (331 776 tets), 4 detectors near the floor x 4096 directions (zenith
+    U[0, 75] deg), topography 150-400 m away.  Host entry point: the mesh and
+    rays are staged each call; `kernel_ms` is the device time of the locate and
+    walk launches (count + fill), `value` ray*tets per second of kernel time."""
+    import numpy as np
+    from workloads import kuhn_mesh
+    pts, tets = kuhn_mesh(48, 48, 24, 10.0, origin=(-240.0, -240.0, -5.0), jitter=0.1, seed=8)
+    vol = dp.TetraVolume(pts, tets)
+    rng = np.random.default_rng(21)
+    dets = [(-60.0, -60.0, 0.0), (60.0, -60.0, 0.0), (-60.0, 60.0, 0.0), (60.0, 60.0, 0.0)]
+    o, d = [], []
+    for det in dets:
+        zen = np.radians(rng.uniform(0.0, 75.0, 4096))
+        az = np.radians(rng.uniform(0.0, 360.0, 4096))
+        d.append(np.column_stack([np.sin(zen) * np.sin(az), np.sin(zen) * np.cos(az), np.cos(zen)]))
+        o.append(np.tile(det, (4096, 1)))
+    o, d = np.concatenate(o), np.concatenate(d)
+    surf = rng.uniform(150.0, 400.0, len(o))
+    for _ in range(warmup):
+        dp.trace_tetra_rays(vol, o, d, surf, ctx=ctx)
+    kms, t0 = 0.0, time.perf_counter()
+    for _ in range(steps):
+        rp, cells, lens, tr = dp.trace_tetra_rays(vol, o, d, surf, ctx=ctx)
+        kms += ctx.last_kernel_ms()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    kern = kms / steps     # the fill launch pair (locate + walk) of the last call per step
+    ns = int(rp[-1])
+    alg = 12.0 * ns + 56.0 * len(o)
+    achieved = alg / (kern * 1e-3) / 1e9
+    return {"value": ns / (kern * 1e-3), "unit": "rays*tets/s", "kernel_ms": kern,
+            "wall_ms_incl_staging": wall, "rays": len(o), "segments": ns, "tets": len(tets),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+            "workload": "trace_path(::TetraVolume), jittered Kuhn mesh 48x48x24 cubes of 10 m "
+                        "(331776 tets), 4 detectors x 4096 directions",
+            "_problem": (pts, tets, vol.neighbors, o, d, surf)}
+
+
+def cpu_tetra_baseline(problem, budget_s=6.0):
+    """The oracle's tetrahedral walk (start-tet search included) on a prefix of the rays."""
+    import _oracle as O
+    pts, tets, nbr, o, d, surf = problem
+    n = 64
+    while True:
+        t0 = time.perf_counter()
+        rp = O.trace_tetra(pts, tets, nbr, o[:n], d[:n], surf[:n])[0]
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= len(o):
+            return {"value": int(rp[-1]) / dt, "unit": "rays*tets/s", "cores": 1, "kind": "port",
+                    "sample": f"first {n} rays, {dt:.2f} s"}
+        n *= 2
 
 
 def run_dem(dp, L, ctx, torch, steps, warmup, n_dirs=16384):
@@ -800,6 +856,9 @@ def main():
     dm = run_dem(dp, L, ctx, torch, a.steps, a.warmup)
     log(f"dem: {dm['value']:.4g} rays/s, kernel {dm['kernel_ms']:.4f} ms, found {dm['found']:.3f}")
     demprob = dm.pop("_problem")
+    tt = run_tetra(dp, L, ctx, a.steps, a.warmup)
+    log(f"tetra: {tt['value']:.4g} rays*tets/s, kernel {tt['kernel_ms']:.3f} ms")
+    ttprob = tt.pop("_problem")
     cpu = None if a.no_cpu else cpu_mc_baseline(physics)
     cpu_adj = None if a.no_cpu else cpu_adjoint_baseline(physics, problem)
     wm["cpu_baseline"] = None if a.no_cpu else cpu_wmodel_baseline(physics, wprob)
@@ -810,6 +869,7 @@ def main():
     tr["cpu_baseline"] = None if a.no_cpu else cpu_trace_baseline(tprob)
     ne["cpu_baseline"] = None if a.no_cpu else cpu_normal_baseline(neprob)
     dm["cpu_baseline"] = None if a.no_cpu else cpu_dem_baseline(demprob)
+    tt["cpu_baseline"] = None if a.no_cpu else cpu_tetra_baseline(ttprob)
     out = {
         "metric": "straggled backward-MC events/s (+ adjoint rays*voxels/s), 1 GPU, % HBM roofline",
         "value": mc["value"], "unit": "events/s", "n_gpus": 1, "steps": a.steps,
@@ -843,6 +903,7 @@ def main():
         "trace": tr,
         "dem": dm,
         "normal": ne,
+        "tetra": tt,
     }
     if a.calib_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)

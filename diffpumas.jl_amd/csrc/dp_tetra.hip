@@ -1,0 +1,424 @@
+// Ray walk through a tetrahedral mesh: trace_path(::TetraVolume)
+// (examples/lvd_tomography.jl:694-736) for many rays.
+//
+//   tet_locate_kernel  one wave per ray: find_tetrahedron_containing_point of
+//                      the detector + POINT_EPS along the ray.  The lanes test
+//                      64 tets at a time; the lowest set bit of the ballot is
+//                      the first tet in index order, as the reference's scan.
+//   tet_trace_kernel   one ray per lane: nearest exit face (Moller-Trumbore,
+//                      src/TriangleIntersect.jl:86-127, with the tolerant
+//                      plane hit of :484-501 as fallback), step to it, nudge
+//                      by POINT_EPS and continue in the neighbour across that
+//                      face.  Launched twice: counts, then the CSR segments.
+//
+// Built with -ffp-contract=off; every product, sum, division and square root
+// is rounded as in the host restatement, so cells and lengths are bitwise
+// those of the oracle.  The mesh (points, tets, neighbours) stays in HBM and
+// is read through the caches.  A walk is capped at 4 n_tets + 16 steps (a
+// straight ray in a valid mesh enters each tet at most once); reaching the
+// cap is reported as DP_E_EXHAUSTED.
+#include "dp_internal.hpp"
+#include "dp_device.hpp"
+#include <algorithm>
+#include <cmath>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+struct TetDev {
+  int np, nt;
+  const double* pts;
+  const int32_t* tets;
+  const int32_t* nbr;
+};
+
+constexpr double TET_POINT_EPS = 1e-4;   // examples/lvd_tomography.jl:147
+
+__device__ __forceinline__ double tdot(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ __forceinline__ void tcross(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// point_in_tetrahedron (atol 1e-8): Julia's det and `\` on M = [b-a c-a d-a]
+// (triangular M: diagonal product and a triangular solve; otherwise LU with
+// partial pivoting).
+__device__ bool point_in_tet(const TetDev& m, int t, const double* p) {
+  const double atol = 1e-8;
+  const int32_t* v = m.tets + 4 * (size_t)t;
+  const double* a = m.pts + 3 * (size_t)v[0];
+  double A[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* q = m.pts + 3 * (size_t)v[c + 1];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) A[r][c] = q[r] - a[r];
+  }
+  double b[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+  const bool lower = A[0][1] == 0.0 && A[0][2] == 0.0 && A[1][2] == 0.0;
+  const bool upper = A[1][0] == 0.0 && A[2][0] == 0.0 && A[2][1] == 0.0;
+  if (lower || upper) {
+    const double det = (A[0][0] * A[1][1]) * A[2][2];
+    if (fabs(det) < atol) return false;
+    if (lower && upper) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) b[k] = b[k] / A[k][k];
+    } else if (lower) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (b[k] != 0.0) {
+          b[k] /= A[k][k];
+#pragma unroll
+          for (int r = k + 1; r < 3; ++r) b[r] -= b[k] * A[r][k];
+        }
+    } else {
+#pragma unroll
+      for (int k = 2; k >= 0; --k)
+        if (b[k] != 0.0) {
+          b[k] /= A[k][k];
+#pragma unroll
+          for (int r = k - 1; r >= 0; --r) b[r] -= b[k] * A[r][k];
+        }
+    }
+  } else {
+    int piv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      int pr = j;
+#pragma unroll
+      for (int r = j + 1; r < 3; ++r)
+        if (fabs(A[r][j]) > fabs(A[pr][j])) pr = r;
+      piv[j] = pr;
+      if (A[pr][j] != 0.0) {
+        if (pr != j) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double tmp = A[j][c];
+            A[j][c] = A[pr][c];
+            A[pr][c] = tmp;
+          }
+        }
+        const double rcp = 1.0 / A[j][j];
+#pragma unroll
+        for (int r = j + 1; r < 3; ++r) A[r][j] *= rcp;
+      }
+#pragma unroll
+      for (int r = j + 1; r < 3; ++r)
+#pragma unroll
+        for (int c = j + 1; c < 3; ++c) A[r][c] -= A[r][j] * A[j][c];
+    }
+    double det = 1.0;
+    int swaps = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      det *= A[j][j];
+      if (piv[j] != j) ++swaps;
+    }
+    det = det * ((swaps & 1) ? -1.0 : 1.0);
+    if (fabs(det) < atol) return false;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (piv[j] != j) {
+        const double tmp = b[j];
+        b[j] = b[piv[j]];
+        b[piv[j]] = tmp;
+      }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (b[k] != 0.0)
+#pragma unroll
+        for (int r = k + 1; r < 3; ++r) b[r] -= b[k] * A[r][k];
+#pragma unroll
+    for (int k = 2; k >= 0; --k)
+      if (b[k] != 0.0) {
+        b[k] /= A[k][k];
+#pragma unroll
+        for (int r = k - 1; r >= 0; --r) b[r] -= b[k] * A[r][k];
+      }
+  }
+  return b[0] >= -atol && b[1] >= -atol && b[2] >= -atol && (b[0] + b[1]) + b[2] <= 1.0 + atol;
+}
+
+// intersect(ray, triangle), EPSILON 1e-10.
+__device__ bool moller_trumbore(const double* o, const double* d, const double* v0,
+                                const double* v1, const double* v2, double& t, double* ip) {
+  const double EPS = 1e-10;
+  const double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+  const double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+  double h[3];
+  tcross(d, e2, h);
+  const double a = tdot(e1, h);
+  if (fabs(a) < EPS) return false;
+  const double f = 1.0 / a;
+  const double s[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+  const double u = f * tdot(s, h);
+  if (u < 0.0 || u > 1.0) return false;
+  double q[3];
+  tcross(s, e1, q);
+  const double v = f * tdot(d, q);
+  if (v < 0.0 || u + v > 1.0) return false;
+  const double tt = f * tdot(e2, q);
+  if (!(tt > EPS)) return false;
+  t = tt;
+  for (int i = 0; i < 3; ++i) ip[i] = o[i] + tt * d[i];
+  return true;
+}
+
+// tolerant_tetra_face_hit: plane hit with barycentric slack 1e-10.
+__device__ bool tolerant_hit(const double* o, const double* d, const double* p1,
+                             const double* p2, const double* p3, double& dist, double* ip) {
+  const double e1[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+  const double e2[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+  double n[3];
+  tcross(e1, e2, n);
+  const double den = tdot(n, d);
+  if (!(fabs(den) > 1e-12 * sqrt(tdot(n, n)))) return false;
+  const double ao[3] = {p1[0] - o[0], p1[1] - o[1], p1[2] - o[2]};
+  const double distance = tdot(n, ao) / den;
+  if (!(distance > 1e-8)) return false;
+  const double off[3] = {o[0] + distance * d[0] - p1[0], o[1] + distance * d[1] - p1[1],
+                         o[2] + distance * d[2] - p1[2]};
+  const double aa = tdot(e1, e1), ab = tdot(e1, e2), bb = tdot(e2, e2);
+  const double det = aa * bb - ab * ab;
+  if (!(det > 0)) return false;
+  const double u = (bb * tdot(off, e1) - ab * tdot(off, e2)) / det;
+  const double v = (aa * tdot(off, e2) - ab * tdot(off, e1)) / det;
+  const double w3 = 1 - u - v;
+  if (!(u >= -1e-10 && v >= -1e-10 && w3 >= -1e-10)) return false;
+  dist = distance;
+  for (int i = 0; i < 3; ++i) ip[i] = o[i] + distance * d[i];
+  return true;
+}
+
+// nearest_tetra_face_intersection over faces (1,2,3), (1,2,4), (1,3,4), (2,3,4).
+__device__ int nearest_face(const TetDev& m, int t, const double* p, const double* d,
+                            double& best, double* bp) {
+  const int32_t* v = m.tets + 4 * (size_t)t;
+  const double* P[4] = {m.pts + 3 * (size_t)v[0], m.pts + 3 * (size_t)v[1],
+                        m.pts + 3 * (size_t)v[2], m.pts + 3 * (size_t)v[3]};
+  int bf = -1;
+  best = __builtin_huge_val();
+  bp[0] = p[0]; bp[1] = p[1]; bp[2] = p[2];
+  for (int f = 0; f < 4; ++f) {
+    const double* p1 = P[f < 3 ? 0 : 1];
+    const double* p2 = P[f < 2 ? 1 : 2];
+    const double* p3 = P[f == 0 ? 2 : 3];
+    double dist = 0.0, ip[3];
+    bool ok = moller_trumbore(p, d, p1, p2, p3, dist, ip);
+    if (!ok) ok = tolerant_hit(p, d, p1, p2, p3, dist, ip);
+    if (ok && dist > 1e-8 && dist < best) {
+      bf = f;
+      best = dist;
+      bp[0] = ip[0]; bp[1] = ip[1]; bp[2] = ip[2];
+    }
+  }
+  return bf;
+}
+
+__global__ __launch_bounds__(64) void tet_locate_kernel(TetDev m, int n,
+                                                        const double* __restrict__ origin,
+                                                        const double* __restrict__ dir,
+                                                        int32_t* __restrict__ start) {
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    const double p[3] = {origin[3 * r] + dir[3 * r] * TET_POINT_EPS,
+                         origin[3 * r + 1] + dir[3 * r + 1] * TET_POINT_EPS,
+                         origin[3 * r + 2] + dir[3 * r + 2] * TET_POINT_EPS};
+    int found = -1;
+    for (int c0 = 0; c0 < m.nt; c0 += 64) {
+      const int t = c0 + (int)threadIdx.x;
+      const bool in = t < m.nt && point_in_tet(m, t, p);
+      const unsigned long long mask = __ballot(in);
+      if (mask) {
+        found = c0 + __ffsll((unsigned long long)mask) - 1;
+        break;
+      }
+    }
+    if (threadIdx.x == 0) start[r] = found;
+  }
+}
+
+__global__ __launch_bounds__(64) void tet_trace_kernel(
+    TetDev m, int n, const double* __restrict__ origin, const double* __restrict__ dir,
+    const double* __restrict__ surface, const int32_t* __restrict__ start,
+    int32_t* __restrict__ counts, const int32_t* __restrict__ row_ptr,
+    int32_t* __restrict__ cells, double* __restrict__ lens, double* __restrict__ traveled_out,
+    int* __restrict__ flag) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const bool write = cells != nullptr;
+  const int out = write ? row_ptr[r] : 0;
+  const int cap = write ? row_ptr[r + 1] - out : 0;
+  const double d[3] = {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
+  double cur[3] = {origin[3 * r] + d[0] * TET_POINT_EPS, origin[3 * r + 1] + d[1] * TET_POINT_EPS,
+                   origin[3 * r + 2] + d[2] * TET_POINT_EPS};
+  const double sd = surface[r];
+  int tet = start[r];
+  double traveled = tet < 0 ? 0.0 : TET_POINT_EPS;
+  int ns = 0;
+  const long max_steps = 4L * m.nt + 16;
+  long it = 0;
+  while (tet >= 0 && traveled < sd - TET_POINT_EPS) {
+    if (++it > max_steps) {
+      atomicOr(flag, 1);
+      break;
+    }
+    double dist, ip[3];
+    const int f = nearest_face(m, tet, cur, d, dist, ip);
+    if (f < 0 || !isfinite(dist)) break;
+    const double remaining = sd - traveled;
+    const double step = fmin(dist, remaining);
+    if (step > 1e-8) {
+      if (write && ns < cap) {
+        cells[out + ns] = tet;
+        lens[out + ns] = step;
+      }
+      ++ns;
+      traveled += step;
+    }
+    if (dist >= remaining - 1e-6) break;
+    const int nt = m.nbr[4 * (size_t)tet + f];
+    if (nt < 0) {
+      tet = -1;
+    } else {
+      cur[0] = ip[0] + d[0] * TET_POINT_EPS;
+      cur[1] = ip[1] + d[1] * TET_POINT_EPS;
+      cur[2] = ip[2] + d[2] * TET_POINT_EPS;
+      tet = nt;
+    }
+  }
+  if (!write) counts[r] = ns;
+  if (traveled_out) traveled_out[r] = traveled;
+}
+
+int check_mesh(const char* fn, const dp_tet_mesh* m) {
+  const std::string f(fn);
+  if (!m || !m->points || !m->tets || !m->neighbors)
+    return dph::fail(DP_E_ARGUMENT, f + ": null mesh array");
+  if (m->n_points < 4 || m->n_tets < 1)
+    return dph::fail(DP_E_ARGUMENT, f + ": the mesh needs at least 4 points and one tet");
+  for (int64_t i = 0; i < 4 * (int64_t)m->n_tets; ++i) {
+    if (m->tets[i] < 0 || m->tets[i] >= m->n_points)
+      return dph::fail(DP_E_ARGUMENT, f + ": tet vertex index out of range");
+    if (m->neighbors[i] < -1 || m->neighbors[i] >= m->n_tets)
+      return dph::fail(DP_E_ARGUMENT, f + ": neighbour index out of range");
+  }
+  return DP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dp_tet_neighbors(int32_t n_tets, const int32_t* tets, int32_t* neighbors) {
+  if (n_tets < 0 || (n_tets > 0 && (!tets || !neighbors)))
+    return dph::fail(DP_E_ARGUMENT, "dp_tet_neighbors: bad arguments");
+  static const int F[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  std::vector<std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t>> faces;
+  faces.reserve(4 * (size_t)n_tets);
+  for (int32_t t = 0; t < n_tets; ++t)
+    for (int f = 0; f < 4; ++f) {
+      int32_t a = tets[4 * (size_t)t + F[f][0]], b = tets[4 * (size_t)t + F[f][1]],
+              c = tets[4 * (size_t)t + F[f][2]];
+      if (a > b) std::swap(a, b);
+      if (b > c) std::swap(b, c);
+      if (a > b) std::swap(a, b);
+      faces.emplace_back(a, b, c, t, f);
+    }
+  std::sort(faces.begin(), faces.end());
+  std::fill(neighbors, neighbors + 4 * (size_t)n_tets, -1);
+  size_t i = 0;
+  while (i < faces.size()) {
+    size_t j = i + 1;
+    while (j < faces.size() && std::get<0>(faces[j]) == std::get<0>(faces[i]) &&
+           std::get<1>(faces[j]) == std::get<1>(faces[i]) &&
+           std::get<2>(faces[j]) == std::get<2>(faces[i]))
+      ++j;
+    if (j - i == 2) {   // a face shared by exactly two tets, as the reference links them
+      const auto& x = faces[i];
+      const auto& y = faces[i + 1];
+      neighbors[4 * (size_t)std::get<3>(x) + std::get<4>(x)] = std::get<3>(y);
+      neighbors[4 * (size_t)std::get<3>(y) + std::get<4>(y)] = std::get<3>(x);
+    }
+    i = j;
+  }
+  return DP_OK;
+}
+
+int dp_trace_tetra(dp_ctx c, const dp_tet_mesh* mesh, int32_t n, const double* origin,
+                   const double* dir, const double* surface_distance, int32_t* counts,
+                   const int32_t* row_ptr, int32_t* cells, double* lengths, double* traveled) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_trace_tetra: null context");
+  int rc = check_mesh("dp_trace_tetra", mesh);
+  if (rc) return rc;
+  if (n < 0) return dph::fail(DP_E_ARGUMENT, "dp_trace_tetra: negative ray count");
+  if (n == 0) return DP_OK;
+  if (!origin || !dir || !surface_distance || (!cells && !counts) || (cells && (!row_ptr || !lengths)))
+    return dph::fail(DP_E_ARGUMENT, "dp_trace_tetra: null argument");
+  int nseg = 0;
+  if (cells) {
+    if (row_ptr[0] != 0) return dph::fail(DP_E_ARGUMENT, "dp_trace_tetra: row_ptr[0] must be 0");
+    for (int r = 0; r < n; ++r)
+      if (row_ptr[r + 1] < row_ptr[r])
+        return dph::fail(DP_E_ARGUMENT, "dp_trace_tetra: row_ptr must be non-decreasing");
+    nseg = row_ptr[n];
+  }
+  DP_HIP(hipSetDevice(c->device));
+  const size_t np = (size_t)mesh->n_points, nt = (size_t)mesh->n_tets;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
+  const size_t o_p = take(24 * np), o_t = take(16 * nt), o_nb = take(16 * nt),
+               o_or = take(24 * (size_t)n), o_dir = take(24 * (size_t)n),
+               o_sd = take(8 * (size_t)n), o_st = take(4 * (size_t)n), o_cnt = take(4 * (size_t)n),
+               o_rp = take(4 * (size_t)(n + 1)), o_cell = take(4 * (size_t)nseg),
+               o_len = take(8 * (size_t)nseg), o_tr = take(8 * (size_t)n), o_fl = take(4);
+  rc = dph::ensure(c->staging, o);
+  if (rc) return rc;
+  char* b = (char*)c->staging.ptr;
+  hipStream_t st = c->stream;
+  DP_HIP(hipMemcpyAsync(b + o_p, mesh->points, 24 * np, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_t, mesh->tets, 16 * nt, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_nb, mesh->neighbors, 16 * nt, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_or, origin, 24 * (size_t)n, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_dir, dir, 24 * (size_t)n, hipMemcpyHostToDevice, st));
+  DP_HIP(hipMemcpyAsync(b + o_sd, surface_distance, 8 * (size_t)n, hipMemcpyHostToDevice, st));
+  if (cells)
+    DP_HIP(hipMemcpyAsync(b + o_rp, row_ptr, 4 * (size_t)(n + 1), hipMemcpyHostToDevice, st));
+  const int zero = 0;
+  DP_HIP(hipMemcpyAsync(b + o_fl, &zero, 4, hipMemcpyHostToDevice, st));
+  TetDev m{mesh->n_points, mesh->n_tets, (const double*)(b + o_p), (const int32_t*)(b + o_t),
+           (const int32_t*)(b + o_nb)};
+  const double* d_or = (const double*)(b + o_or);
+  const double* d_dir = (const double*)(b + o_dir);
+  int32_t* d_st = (int32_t*)(b + o_st);
+  const unsigned loc_grid = (unsigned)std::min(n, 65536);
+  (void)hipEventRecord(c->ev0, st);
+  hipLaunchKernelGGL(tet_locate_kernel, dim3(loc_grid), dim3(64), 0, st, m, n, d_or, d_dir, d_st);
+  hipLaunchKernelGGL(tet_trace_kernel, dim3((n + 63) / 64), dim3(64), 0, st, m, n, d_or, d_dir,
+                     (const double*)(b + o_sd), (const int32_t*)d_st,
+                     cells ? nullptr : (int32_t*)(b + o_cnt), (const int32_t*)(b + o_rp),
+                     cells ? (int32_t*)(b + o_cell) : nullptr, (double*)(b + o_len),
+                     (double*)(b + o_tr), (int*)(b + o_fl));
+  (void)hipEventRecord(c->ev1, st);
+  DP_HIP(hipGetLastError());
+  int flag = 0;
+  if (!cells) {
+    DP_HIP(hipMemcpyAsync(counts, b + o_cnt, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+  } else if (nseg > 0) {
+    DP_HIP(hipMemcpyAsync(cells, b + o_cell, 4 * (size_t)nseg, hipMemcpyDeviceToHost, st));
+    DP_HIP(hipMemcpyAsync(lengths, b + o_len, 8 * (size_t)nseg, hipMemcpyDeviceToHost, st));
+  }
+  if (traveled) DP_HIP(hipMemcpyAsync(traveled, b + o_tr, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipMemcpyAsync(&flag, b + o_fl, 4, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipStreamSynchronize(st));
+  (void)hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1);
+  if (flag)
+    return dph::fail(DP_E_EXHAUSTED,
+                     "dp_trace_tetra: a ray exceeded 4 n_tets + 16 steps (inconsistent mesh)");
+  return DP_OK;
+}
+
+}  // extern "C"

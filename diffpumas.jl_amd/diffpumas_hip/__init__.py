@@ -13,6 +13,7 @@ from .cube import (CubeGeometry, RayPaths, compute_flux_gradient_cube, csda_cube
 from .topography import ElevationMap, ray_surface_exit_distance
 from .inverse import make_csda_operator, normal_equations
 from .calibration import calibrate_csda_correction, path_rock_length
+from .tetra import TetraVolume, build_tetra_neighbors, trace_tetra_paths, trace_tetra_rays
 from .geometry import (PRIMARY_ALTITUDE, compute_flux, run_backward_mc, compute_flux_csda_direct,
                        compute_flux_differentiable_csda, compute_flux_gradient_csda)
 from .tomography import (DirectionalPaths, EnergySamples, MaterialConfig, SiteConfig,
@@ -40,4 +41,5 @@ __all__ = [
     "flux_and_segment_jacobian", "sample_energy_set", "trace_voxel_paths",
     "mc_cell_mixture_and_density", "build_cell_properties_for_mc", "compute_directional_flux_mc",
     "make_csda_operator", "normal_equations", "calibrate_csda_correction", "path_rock_length",
+    "TetraVolume", "build_tetra_neighbors", "trace_tetra_paths", "trace_tetra_rays",
 ]

@@ -162,6 +162,11 @@ class Mixture(C.Structure):
                 ("fractions", C.c_double * 4)]
 
 
+class TetMesh(C.Structure):
+    _fields_ = [("n_points", C.c_int32), ("n_tets", C.c_int32), ("points", dbl_p),
+                ("tets", i32_p), ("neighbors", i32_p)]
+
+
 class Csr(C.Structure):
     _fields_ = [("n_rows", C.c_int32), ("n_cols", C.c_int32), ("nnz", C.c_int32),
                 ("row_ptr", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p)]
@@ -225,6 +230,9 @@ _SIGS = {
                                               C.POINTER(SiteConfig), C.POINTER(DirectionalPaths),
                                               C.c_int32, C.c_void_p, C.c_void_p,
                                               C.POINTER(EnergySamples), C.c_void_p, C.c_void_p]),
+    "dp_tet_neighbors": (C.c_int, [C.c_int32, i32_p, i32_p]),
+    "dp_trace_tetra": (C.c_int, [C.c_void_p, C.POINTER(TetMesh), C.c_int32, dbl_p, dbl_p, dbl_p,
+                                 i32_p, i32_p, i32_p, dbl_p, dbl_p]),
     "dp_normal_equations": (C.c_int, [C.c_void_p, C.POINTER(Csr)] + [C.c_void_p] * 2 +
                             [C.c_int32] + [C.c_void_p] * 3),
     "dp_normal_equations_dev": (C.c_int, [C.c_void_p, C.POINTER(Csr)] + [C.c_void_p] * 2 +

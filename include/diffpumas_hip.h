@@ -431,6 +431,32 @@ int dp_mc_fixed_paths(dp_ctx ctx, dp_tables tables, const dp_material_config* ma
                       double energy_threshold_low, double* flux, double* sigma,
                       double* event_weights);
 
+/* ---------------------------------------------- tetrahedral traversal
+ * TetraVolume (examples/lvd_tomography.jl:172-179) as plain arrays, 0-based.
+ * Face f of a tet is the triangle without its vertex 3 - f, i.e. the faces
+ * (1,2,3), (1,2,4), (1,3,4), (2,3,4) of tetra_face_indices (:415-422).     */
+typedef struct {
+  int32_t n_points, n_tets;
+  const double* points;      /* 3 * n_points: x y z                        */
+  const int32_t* tets;       /* 4 * n_tets: vertex indices                 */
+  const int32_t* neighbors;  /* 4 * n_tets: tet across face f, -1 = none   */
+} dp_tet_mesh;
+
+/* Replaces: build_tetra_neighbors (examples/lvd_tomography.jl:431-451): a face
+ * shared by exactly two tets links them.  Host only (no device work).      */
+int dp_tet_neighbors(int32_t n_tets, const int32_t* tets, int32_t* neighbors);
+
+/* Replaces: trace_path(::TetraVolume) (examples/lvd_tomography.jl:694-736),
+ * one ray per entry: from origin[3n] (the detector) along the unit dir[3n]
+ * up to surface_distance[n] (ray_surface_exit_distance).  The start tet is
+ * the first in index order that contains origin + 1e-4 dir.  Two-phase as
+ * dp_trace_edges: with cells == NULL only counts[n] (segments per ray) and
+ * traveled[n]; otherwise the segments (tet, length) at row_ptr.  A ray that
+ * needs more than 4 n_tets + 16 steps (an inconsistent mesh) is an error.  */
+int dp_trace_tetra(dp_ctx ctx, const dp_tet_mesh* mesh, int32_t n_rays, const double* origin,
+                   const double* dir, const double* surface_distance, int32_t* counts,
+                   const int32_t* row_ptr, int32_t* cells, double* lengths, double* traveled);
+
 /* ------------------------------------- Gauss-Newton normal equations
  * Sparse matrix in CSR form, rows = bins, columns = cells (0-based).       */
 typedef struct {

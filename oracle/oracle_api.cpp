@@ -5,6 +5,7 @@
 #include "oracle_philox.hpp"
 #include "oracle_flat_mc.hpp"
 #include "oracle_inverse.hpp"
+#include "oracle_tetra.hpp"
 #include <cstring>
 #include <map>
 #include <omp.h>
@@ -426,6 +427,39 @@ void orc_normal_equations(int n_rows, int n_cols, const int* rp, const int* col,
                           const double* weights, const double* residual, int n_act,
                           const int* active, double* normal, double* grad) {
   normal_equations(n_rows, n_cols, rp, col, val, weights, residual, n_act, active, normal, grad);
+}
+// build_tetra_neighbors (examples/lvd_tomography.jl:431-451), 0-based, -1 = none.
+void orc_tet_neighbors(int nt, const int* tets, int* nbr) { build_neighbors(nt, tets, nbr); }
+// point_in_tetrahedron for every (tet, point) pair: inside[t * n + i].
+void orc_point_in_tet(int np, int nt, const double* pts, const int* tets, int n, const double* p,
+                      uint8_t* inside) {
+  TetMesh m{np, nt, pts, tets, nullptr};
+  for (int t = 0; t < nt; ++t)
+    for (int i = 0; i < n; ++i) inside[(size_t)t * n + i] = point_in_tet(m, t, p + 3 * i);
+}
+// trace_path(::TetraVolume) for n rays.  counts/traveled/start always; with
+// cells != NULL the segments are written at row_ptr.
+void orc_trace_tetra(int np, int nt, const double* pts, const int* tets, const int* nbr, int n,
+                     const double* origin, const double* dir, const double* surface, int* counts,
+                     const int* row_ptr, int* cells, double* lens, double* traveled, int* start) {
+  TetMesh m{np, nt, pts, tets, nbr};
+  for (int r = 0; r < n; ++r) {
+    const double* o = origin + 3 * r;
+    const double* d = dir + 3 * r;
+    const double p0[3] = {o[0] + d[0] * 1e-4, o[1] + d[1] * 1e-4, o[2] + d[2] * 1e-4};
+    const int s0 = find_tet(m, p0);
+    std::vector<int> c;
+    std::vector<double> l;
+    const double tr = trace_tet(m, o, d, surface[r], s0, c, l);
+    counts[r] = (int)c.size();
+    traveled[r] = tr;
+    start[r] = s0;
+    if (cells)
+      for (size_t k = 0; k < c.size(); ++k) {
+        cells[row_ptr[r] + k] = c[k];
+        lens[row_ptr[r] + k] = l[k];
+      }
+  }
 }
 // map_elevation at n points: out = elevation, inside.
 void orc_dem_elevation(int nx, int ny, double x0, double y0, double dx, double dy,

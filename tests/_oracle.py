@@ -612,3 +612,40 @@ def normal_equations(J, residual=None, weights=None, active=None, normal=True):
     lib().orc_normal_equations(n_rows, n_cols, P(rp, C.c_int), P(col, C.c_int), P(val), q(w),
                                q(r), n_act, q(act, C.c_int), q(out_n), q(out_g))
     return (None if out_n is None else out_n.reshape(n_act, n_act).T), out_g
+
+
+def tet_neighbors(tets):
+    """build_tetra_neighbors (examples/lvd_tomography.jl:431-451): 0-based,
+    -1 for a boundary face; face f of a tet omits vertex 3 - f."""
+    t = i32(np.asarray(tets).reshape(-1))
+    nbr = np.zeros(len(t), np.int32)
+    lib().orc_tet_neighbors(len(t) // 4, P(t, C.c_int), P(nbr, C.c_int))
+    return nbr.reshape(-1, 4)
+
+
+def point_in_tet(points, tets, p):
+    """point_in_tetrahedron for every (tet, point): bool[n_tets, n_points]."""
+    pts, t, q = f64(np.asarray(points).reshape(-1)), i32(np.asarray(tets).reshape(-1)), \
+        f64(np.asarray(p).reshape(-1))
+    out = np.zeros((len(t) // 4) * (len(q) // 3), np.uint8)
+    lib().orc_point_in_tet(len(pts) // 3, len(t) // 4, P(pts), P(t, C.c_int), len(q) // 3, P(q),
+                           P(out, C.c_uint8))
+    return out.reshape(len(t) // 4, len(q) // 3).astype(bool)
+
+
+def trace_tetra(points, tets, nbr, origins, dirs, surface):
+    """trace_path(::TetraVolume) per ray: (row_ptr, cells, lengths, traveled, start)."""
+    pts, t, nb = f64(np.asarray(points).reshape(-1)), i32(np.asarray(tets).reshape(-1)), \
+        i32(np.asarray(nbr).reshape(-1))
+    o, d, s = f64(np.asarray(origins).reshape(-1)), f64(np.asarray(dirs).reshape(-1)), f64(surface)
+    n = len(s)
+    counts, start = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    tr = np.zeros(n)
+    args = (len(pts) // 3, len(t) // 4, P(pts), P(t, C.c_int), P(nb, C.c_int), n, P(o), P(d), P(s))
+    lib().orc_trace_tetra(*args, P(counts, C.c_int), None, None, None, P(tr), P(start, C.c_int))
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(counts, out=rp[1:])
+    cells, lens = np.zeros(max(rp[-1], 1), np.int32), np.zeros(max(rp[-1], 1))
+    lib().orc_trace_tetra(*args, P(counts, C.c_int), P(rp, C.c_int), P(cells, C.c_int), P(lens),
+                          P(tr), P(start, C.c_int))
+    return rp, cells[:rp[-1]], lens[:rp[-1]], tr, start

@@ -85,3 +85,33 @@ def diamond_square(n_pow, relief, seed, roughness=0.55):
                 h[i, j] = s / k + amp * rng.uniform(-1.0, 1.0)
         step, amp = half, amp * roughness
     return (h - h.min()) / (h.max() - h.min()) * relief
+
+
+def kuhn_mesh(nx, ny, nz, h, origin=(0.0, 0.0, 0.0), jitter=0.0, seed=0):
+    """Conforming tetrahedral mesh of an nx x ny x nz block of cubes of side h:
+    each cube is cut into the 6 Kuhn tetrahedra (corner 000 to corner 111 along
+    the 6 axis orders).  Tet 6 c + k lies in cube c = i + nx (j + ny k).
+    jitter > 0 moves interior vertices by up to jitter * h (seeded).
+    Returns (points[n, 3], tets[n_tets, 4]) with 0-based indices."""
+    import itertools
+    ox, oy, oz = origin
+    gx, gy, gz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                             indexing="ij")
+    vid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
+    pts = np.stack([ox + h * gx.ravel(), oy + h * gy.ravel(), oz + h * gz.ravel()], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        inner = ((gx > 0) & (gx < nx) & (gy > 0) & (gy < ny) & (gz > 0) & (gz < nz)).ravel()
+        pts[inner] += rng.uniform(-jitter * h, jitter * h, (inner.sum(), 3))
+    tets = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                for perm in itertools.permutations(range(3)):
+                    v = [i, j, k]
+                    ids = [vid(*v)]
+                    for ax in perm:
+                        v[ax] += 1
+                        ids.append(vid(*v))
+                    tets.append(ids)
+    return pts, np.array(tets, np.int32)
