@@ -51,15 +51,16 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy), from the counter
 # runs summarised in profiles/; `traffic` = this x the units of one launch.
 MEASURED_BYTES_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 435223.0,    # per event
-    "csda_cube_kernel": 16467.0,              # per ray-voxel (all 256 nodes)
+    "mc_flat_stochastic_kernel": 286316.0,    # per event
+    "csda_cube_kernel": 16477.0,              # per ray-voxel (all 256 nodes)
     "wcsda_kernel": 2131.0,                   # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 4310.0,         # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 312095.0,               # per event
+    "path_mc_kernel": 273316.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
     "trace_cube_kernel": 13.0,                # per ray-voxel (fill launch)
-    "ne_normal_kernel": 9.0,                  # per entry of the dense normal matrix
+    "ne_normal_kernel": 10.0,                 # per entry of the dense normal matrix
     "ray_exit_kernel": 112.0,                 # per ray
+    "tet_trace_kernel": 62.0,                 # per ray-tet (fill launch)
 }
 FP64_PEAK_TFLOPS = 78.6
 
@@ -380,7 +381,8 @@ def run_tetra(dp, L, ctx, steps, warmup):
     return {"value": ns / (kern * 1e-3), "unit": "rays*tets/s", "kernel_ms": kern,
             "wall_ms_incl_staging": wall, "rays": len(o), "segments": ns, "tets": len(tets),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": MEASURED_BYTES_PER_UNIT["tet_trace_kernel"] * ns},
             "workload": "trace_path(::TetraVolume), jittered Kuhn mesh 48x48x24 cubes of 10 m "
                         "(331776 tets), 4 detectors x 4096 directions",
             "_problem": (pts, tets, vol.neighbors, o, d, surf)}

@@ -88,6 +88,64 @@ int dp_synchronize(dp_ctx c) {
   return DP_OK;
 }
 
+// ------------------------------------------------------------ graph capture
+struct dp_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int dp_capture_begin(dp_ctx c) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_capture_begin: null context");
+  if (c->capturing) return dph::fail(DP_E_ARGUMENT, "dp_capture_begin: a capture is already open");
+  if (!c->stream) return dph::fail(DP_E_ARGUMENT, "dp_capture_begin: the null stream cannot be captured");
+  DP_HIP(hipSetDevice(c->device));
+  DP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  return DP_OK;
+}
+
+int dp_capture_end(dp_ctx c, dp_graph* out) {
+  if (!c || !out) return dph::fail(DP_E_ARGUMENT, "dp_capture_end: null argument");
+  if (!c->capturing) return dph::fail(DP_E_ARGUMENT, "dp_capture_end: no capture is open");
+  *out = nullptr;
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return dph::fail(DP_E_HIP, std::string("dp_capture_end: the capture failed (only _dev entry "
+                                           "points, with buffers already sized, can be "
+                                           "captured): ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return dph::fail(DP_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  dp_graph h = new dp_graph_s;
+  h->graph = g;
+  h->exec = x;
+  *out = h;
+  return DP_OK;
+}
+
+int dp_graph_launch(dp_ctx c, dp_graph g) {
+  if (!c || !g) return dph::fail(DP_E_ARGUMENT, "dp_graph_launch: null argument");
+  if (c->capturing) return dph::fail(DP_E_ARGUMENT, "dp_graph_launch: a capture is open");
+  DP_HIP(hipSetDevice(c->device));
+  DP_HIP(hipGraphLaunch(g->exec, c->stream));
+  return DP_OK;
+}
+
+int dp_graph_destroy(dp_graph g) {
+  if (!g) return DP_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return DP_OK;
+}
+
 int dp_device_info(dp_ctx c, int32_t* n_cu, int32_t* wave) {
   if (!c) return dph::fail(DP_E_ARGUMENT, "dp_device_info: null context");
   if (n_cu) *n_cu = c->n_cu;

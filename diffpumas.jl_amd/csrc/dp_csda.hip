@@ -19,7 +19,7 @@
 // nodes use a fixed wave/LDS tree, so results are deterministic.
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
-#include <hipcub/hipcub.hpp>
+#include "dp_sort.hpp"
 
 using namespace dpk;
 
@@ -60,19 +60,67 @@ struct StepOut {
 };
 
 // One CSDA segment: range inversion at grammage X_in(E) + gram, then the
-// stopping-power ratio and the decay factor at the mean energy.
-__device__ __forceinline__ StepOut csda_segment(const double* sE, const double* sLE,
-                                                const double* sR, const double* sS, int K,
-                                                int pow2, double E, double lE, double gram,
-                                                double path, double mass, double ctau) {
+// stopping-power ratio and the decay factor at the mean energy.  The interval
+// searches start from what the walk already knows: ie is the interval of E
+// on the log-energy grid (-1: unknown or E clamped), jr the last interval
+// found on the range grid.  locate_idx returns the interval the bisection
+// would (interp_e2, inverse_range, interp_e), so every value is unchanged.
+// On return ie is the interval of the exit energy, which is the next
+// segment's entry energy on the same tables.
+__device__ __forceinline__ StepOut csda_segment_g(const double* sE, const double* sLE,
+                                                  const double* sR, const double* sS, int K,
+                                                  int pow2, double E, double lE, double gram,
+                                                  double path, double mass, double ctau, int& ie,
+                                                  int& jr) {
   StepOut o;
-  Lookup2 in = interp_e2(sE, sLE, sR, sS, K, pow2, E, lE);  // y0 = range, y1 = stopping
-  double Xf = in.y0 + gram;
-  Lookup inv = inverse_range(sE, sLE, sR, K, pow2, Xf);
+  Lookup2 in;
+  if (E <= sE[0]) {
+    in.y0 = sR[0]; in.y1 = sS[0]; in.dy0 = in.dy1 = 0.0;
+  } else if (E >= sE[K - 1]) {
+    in.y0 = sR[K - 1]; in.y1 = sS[K - 1]; in.dy0 = in.dy1 = 0.0;
+  } else {
+    const int i = ie >= 0 ? ie : locate_idx(sLE, K, pow2, lE, jr);
+    const double dl = sLE[i + 1] - sLE[i];
+    const double t = (lE - sLE[i]) / dl;
+    const double d0 = sR[i + 1] - sR[i], d1 = sS[i + 1] - sS[i];
+    in.y0 = sR[i] + t * d0;
+    in.y1 = sS[i] + t * d1;
+    in.dy0 = (d0 / dl) / E;
+    in.dy1 = (d1 / dl) / E;
+  }
+  const double Xf = in.y0 + gram;
+  Lookup inv;
+  if (Xf <= sR[0]) {
+    inv.y = sE[0]; inv.dy = 0.0;
+  } else if (Xf >= sR[K - 1]) {
+    inv.y = sE[K - 1]; inv.dy = 0.0;
+  } else {
+    const int j = locate_idx(sR, K, pow2, Xf, jr);
+    jr = j;
+    const double dr = sR[j + 1] - sR[j];
+    const double t = (Xf - sR[j]) / dr;
+    const double dle = sLE[j + 1] - sLE[j];
+    const double le = sLE[j] + t * dle;
+    inv.y = exp(le);
+    inv.dy = inv.y * (dle / dr);
+  }
   o.E = inv.y;
   o.ok = !(o.E >= 1e12) && isfinite(o.E);
   o.lE = log(o.E);
-  Lookup so = interp_e(sE, sLE, sS, K, pow2, o.E, o.lE);
+  Lookup so;
+  if (o.E <= sE[0]) {
+    so.y = sS[0]; so.dy = 0.0; ie = -1;
+  } else if (o.E >= sE[K - 1]) {
+    so.y = sS[K - 1]; so.dy = 0.0; ie = -1;
+  } else {
+    const int i = locate_idx(sLE, K, pow2, o.lE, jr);
+    ie = i;
+    const double dl = sLE[i + 1] - sLE[i];
+    const double t = (o.lE - sLE[i]) / dl;
+    const double dyy = sS[i + 1] - sS[i];
+    so.y = sS[i] + t * dyy;
+    so.dy = (dyy / dl) / o.E;
+  }
   double dlog;
   o.dec = decay_weight_dlog(path, (E + o.E) / 2.0, mass, ctau, &dlog);
   o.ratio = so.y / in.y1;
@@ -146,12 +194,14 @@ __global__ __launch_bounds__(256) void csda_cube_kernel(CsdaArgs a) {
       double lE = log(E);
       double W = 1.0;
       bool ok = true;
+      int ie = -1, jr = K / 2;   // interval guesses (rock tables)
       // ------------------------------------------------ forward sweep
       for (int s = 0; s < ns; ++s) {
         const int cell = a.cell[b0 + s];
         const double d = a.len[b0 + s];
         const double rho = a.dens[cell];
-        StepOut o = csda_segment(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, a.ctau);
+        StepOut o = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, a.ctau,
+                                   ie, jr);
         ok = ok && o.ok;
         W = W * o.ratio;
         W = W * o.dec;
@@ -166,14 +216,17 @@ __global__ __launch_bounds__(256) void csda_cube_kernel(CsdaArgs a) {
       StepOut orem;
       orem.c = orem.b = orem.e = orem.k = 0.0;
       if (has_rem) {
-        orem = csda_segment(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass, a.ctau);
+        orem = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass, a.ctau,
+                              ie, jr);
         ok = ok && orem.ok;
         W = W * orem.ratio;
         W = W * orem.dec;
         E = orem.E;
         lE = orem.lE;
       }
-      StepOut oair = csda_segment(aE, aLE, aR, aS, K, a.pow2, E, lE, Xair, path_air, a.mass, a.ctau);
+      int ie_air = -1, jr_air = jr;   // air tables: a fresh entry search
+      StepOut oair = csda_segment_g(aE, aLE, aR, aS, K, a.pow2, E, lE, Xair, path_air, a.mass,
+                                    a.ctau, ie_air, jr_air);
       ok = ok && oair.ok;
       W = W * oair.ratio;
       W = W * oair.dec;
@@ -231,6 +284,11 @@ __global__ void reset_counters(int* q, int n) {
 }
 
 // ------------------------------------------------------------------ VJP
+__global__ void vjp_zero(int n, double* __restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) g[i] = 0.0;
+}
+
 __global__ void vjp_keys(int n_rays, int n_seg, const int32_t* __restrict__ row_ptr,
                          const int32_t* __restrict__ cell, int32_t* __restrict__ keys,
                          int32_t* __restrict__ vals, int32_t* __restrict__ ray_of) {
@@ -320,9 +378,9 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
   a.scratch = (double*)c->scratch.ptr;
   a.queue = qerr; a.err = qerr + 1;
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, qerr, 2);
-  (void)hipEventRecord(c->ev0, c->stream);
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(csda_cube_kernel, dim3(grid), dim3(bd), lds, c->stream, a);
-  (void)hipEventRecord(c->ev1, c->stream);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   return DP_OK;
 }
@@ -420,30 +478,29 @@ int dp_csda_cube_vjp_dev(dp_ctx c, const dp_paths* p, const double* jac, const d
   if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: need at least one cell");
   DP_HIP(hipSetDevice(c->device));
   const int nr = p->n_rays, ns = p->n_segments, ne = ns + nr;
-  if (nr == 0) {
-    DP_HIP(hipMemsetAsync(grad, 0, 8 * (size_t)n_cells, c->stream));
+  if (nr == 0) {   // a fill kernel, not a memset node, so the call can sit in a captured graph
+    hipLaunchKernelGGL(vjp_zero, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
+                       grad);
+    DP_HIP(hipGetLastError());
     return DP_OK;
   }
   int bits = 1;
   while ((1ll << bits) < (long long)n_cells) ++bits;
-  size_t tmp = 0;
-  DP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (int32_t*)nullptr, (int32_t*)nullptr,
-                                            (int32_t*)nullptr, (int32_t*)nullptr, ne, 0, bits,
-                                            c->stream));
-  size_t o_k = 0, o_v = dph::align_up(4 * (size_t)ne, 256);
-  size_t o_sk = dph::align_up(o_v + 4 * (size_t)ne, 256);
-  size_t o_sv = dph::align_up(o_sk + 4 * (size_t)ne, 256);
-  size_t o_ro = dph::align_up(o_sv + 4 * (size_t)ne, 256);
-  size_t o_t = dph::align_up(o_ro + 4 * (size_t)ne, 256);
-  int rc = dph::ensure(c->scratch2, o_t + tmp + 256);
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
+  const size_t o_k = take(4 * (size_t)ne), o_v = take(4 * (size_t)ne), o_ro = take(4 * (size_t)ne),
+               o_p0 = take(8 * (size_t)ne), o_p1 = take(8 * (size_t)ne),
+               o_h = take(4 * dps::sort_hist_ints(ne));
+  int rc = dph::ensure(c->scratch2, o);
   if (rc) return rc;
   char* b = (char*)c->scratch2.ptr;
-  int32_t *k = (int32_t*)(b + o_k), *v = (int32_t*)(b + o_v), *sk = (int32_t*)(b + o_sk),
-          *sv = (int32_t*)(b + o_sv), *ro = (int32_t*)(b + o_ro);
+  int32_t *k = (int32_t*)(b + o_k), *v = (int32_t*)(b + o_v), *ro = (int32_t*)(b + o_ro);
   hipLaunchKernelGGL(vjp_keys, dim3((nr + 255) / 256), dim3(256), 0, c->stream, nr, ns,
                      p->row_ptr, p->cell, k, v, ro);
   DP_HIP(hipGetLastError());
-  DP_HIP(hipcub::DeviceRadixSort::SortPairs(b + o_t, tmp, k, sk, v, sv, ne, 0, bits, c->stream));
+  const int32_t *sk = k, *sv = v;
+  dps::radix_sort_pairs(k, v, ne, bits, (int32_t*)(b + o_p0), (int32_t*)(b + o_p1),
+                        (int32_t*)(b + o_h), c->stream, &sk, &sv);
   hipLaunchKernelGGL(vjp_gather, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
                      ns, ne, sk, sv, ro, jac, jac_rem, fbar, grad);
   DP_HIP(hipGetLastError());

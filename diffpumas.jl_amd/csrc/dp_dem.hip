@@ -99,9 +99,9 @@ int launch_dem(dp_ctx c, const dp_elevation_map* m, double xy_scale, double z_re
   a.xy_scale = xy_scale; a.z_ref = z_ref; a.step = step;
   a.n_steps = (long)floor(max_distance / step + 1e-9);
   a.n = n; a.dirs = dirs; a.dist = dist; a.found = found; a.exit_z = ez;
-  (void)hipEventRecord(c->ev0, c->stream);
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(ray_exit_kernel, dim3((n + 63) / 64), dim3(64), 0, c->stream, a);
-  (void)hipEventRecord(c->ev1, c->stream);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   return DP_OK;
 }

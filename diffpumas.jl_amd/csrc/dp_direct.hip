@@ -131,9 +131,9 @@ int launch_direct(dp_ctx c, dp_tables t, int32_t rock, int32_t air, double palt,
   a.rock = rock; a.air = air; a.mass = t->mass; a.ctau = t->ctau; a.palt = palt;
   a.n = n; a.rho = rho; a.thick = thick; a.elev = elev; a.energy = energy; a.charge = charge;
   a.flux = flux; a.dflux = dflux;
-  (void)hipEventRecord(c->ev0, c->stream);
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(csda_direct_kernel, dim3((unsigned)grid), dim3(256), lds, c->stream, a);
-  (void)hipEventRecord(c->ev1, c->stream);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   return DP_OK;
 }

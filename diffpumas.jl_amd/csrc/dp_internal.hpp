@@ -73,6 +73,7 @@ struct dp_ctx_s {
   dph::DevBuf staging;       // host-API staging of inputs/outputs
   dp_csda_correction corr{};            // w-model corrections (off)
   dp_csda_material_correction mcorr{};
+  bool capturing = false;               // between dp_capture_begin and dp_capture_end
 };
 
 namespace dph {
@@ -83,4 +84,9 @@ inline int pow2_for(int n) {
   return n - 2 >= 1 ? p : 0;
 }
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+// Timing event on the context's stream; not recorded while the stream is
+// being captured into a graph.
+inline void record(dp_ctx c, hipEvent_t e) {
+  if (!c->capturing) (void)hipEventRecord(e, c->stream);
+}
 }  // namespace dph

@@ -835,7 +835,7 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     hm[0] = s.rock; hm[1] = s.air; hm[2] = s.porous;
     DP_HIP(hipMemcpyAsync(b + o_med, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
     s.media = (const Medium*)(b + o_med);
-    (void)hipEventRecord(c->ev0, c->stream);
+    dph::record(c, c->ev0);
     s.order = nullptr;
     if (n < 0x7fffffff) {
       int* cnt = (int*)(b + o_cnt);
@@ -859,7 +859,7 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     if (fast) hipLaunchKernelGGL(mc_flat_stochastic_kernel<true>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
     else hipLaunchKernelGGL(mc_flat_stochastic_kernel<false>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
     DP_HIP(hipGetLastError());
-    (void)hipEventRecord(c->ev1, c->stream);
+    dph::record(c, c->ev1);
   }
   if (!stochastic || a.grad) {
     size_t lds = (size_t)3 * MC_ARRS * t->n_knots * sizeof(double);
@@ -869,9 +869,9 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     int64_t want = (n + 255) / 256;
     int64_t grid = (int64_t)c->n_cu * occ * 4;
     if (grid > want) grid = want;
-    if (!stochastic) (void)hipEventRecord(c->ev0, c->stream);
+    if (!stochastic) dph::record(c, c->ev0);
     hipLaunchKernelGGL(mc_flat_csda_kernel, dim3((unsigned)grid), dim3(256), lds, c->stream, a);
-    if (!stochastic) (void)hipEventRecord(c->ev1, c->stream);
+    if (!stochastic) dph::record(c, c->ev1);
     DP_HIP(hipGetLastError());
   }
   double* res = (double*)(b + o_res);

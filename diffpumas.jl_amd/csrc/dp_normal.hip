@@ -10,7 +10,8 @@
 //   1. one lane per row tags each entry with its active column (sentinel
 //      n_active for dropped entries: zero, non-finite, inactive column) and
 //      records the entry's row;
-//   2. a stable radix sort of (active column, entry) pairs lists every
+//   2. a stable radix sort of (active column, entry) pairs (dp_sort.hpp: no
+//      memset, so the whole assembly can be captured in a graph) lists every
 //      column's entries in ascending row order;
 //   3. one 64-lane block per (output column j, tile of 4096 rows i) walks the
 //      rows of column j in that order and, for each, adds the row's products
@@ -26,7 +27,7 @@
 // traffic: 8 n_active^2 bytes written.
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
-#include <hipcub/hipcub.hpp>
+#include "dp_sort.hpp"
 #include <cmath>
 
 namespace {
@@ -180,23 +181,20 @@ int run_normal(dp_ctx c, const dp_csr* J, const double* weights, const double* r
   const int nr = J->n_rows, nc = J->n_cols, nnz = J->nnz, na = n_active;
   int bits = 1;
   while ((1ll << bits) <= (long long)na) ++bits;   // keys 0 .. n_active
-  size_t tmp = 0;
-  DP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (int32_t*)nullptr, (int32_t*)nullptr,
-                                            (int32_t*)nullptr, (int32_t*)nullptr, nnz, 0, bits,
-                                            st));
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
   const size_t o_ai = take(4 * (size_t)nc), o_k = take(4 * (size_t)nnz),
-               o_e = take(4 * (size_t)nnz), o_sk = take(4 * (size_t)nnz),
-               o_se = take(4 * (size_t)nnz), o_ro = take(4 * (size_t)nnz),
-               o_cp = take(4 * (size_t)(na + 1)), o_t = take(tmp);
+               o_e = take(4 * (size_t)nnz), o_ro = take(4 * (size_t)nnz),
+               o_p0 = take(8 * (size_t)nnz), o_p1 = take(8 * (size_t)nnz),
+               o_h = take(4 * dps::sort_hist_ints(nnz)),
+               o_cp = take(4 * (size_t)(na + 1));
   int rc = dph::ensure(c->scratch2, o);
   if (rc) return rc;
   char* b = (char*)c->scratch2.ptr;
   int32_t *ai = (int32_t*)(b + o_ai), *k = (int32_t*)(b + o_k), *e = (int32_t*)(b + o_e),
-          *sk = (int32_t*)(b + o_sk), *se = (int32_t*)(b + o_se), *ro = (int32_t*)(b + o_ro),
-          *cp = (int32_t*)(b + o_cp);
-  (void)hipEventRecord(c->ev0, st);
+          *ro = (int32_t*)(b + o_ro), *cp = (int32_t*)(b + o_cp);
+  const int32_t *sk = k, *se = e;
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(ne_fill_index, dim3((nc + 255) / 256), dim3(256), 0, st, nc, ai);
   hipLaunchKernelGGL(ne_scatter_index, dim3((na + 255) / 256), dim3(256), 0, st, na, nc, active,
                      ai);
@@ -205,7 +203,8 @@ int run_normal(dp_ctx c, const dp_csr* J, const double* weights, const double* r
                        J->col, J->val, ai, k, e, ro);
   DP_HIP(hipGetLastError());
   if (nnz > 0)
-    DP_HIP(hipcub::DeviceRadixSort::SortPairs(b + o_t, tmp, k, sk, e, se, nnz, 0, bits, st));
+    dps::radix_sort_pairs(k, e, nnz, bits, (int32_t*)(b + o_p0), (int32_t*)(b + o_p1),
+                          (int32_t*)(b + o_h), st, &sk, &se);
   hipLaunchKernelGGL(ne_col_ptr, dim3((na + 1 + 255) / 256), dim3(256), 0, st, na, nnz, sk, cp);
   if (normal) {
     const dim3 grid(na, (na + NE_TILE - 1) / NE_TILE);
@@ -215,7 +214,7 @@ int run_normal(dp_ctx c, const dp_csr* J, const double* weights, const double* r
   if (grad)
     hipLaunchKernelGGL(ne_grad_kernel, dim3((na + 255) / 256), dim3(256), 0, st, na, J->val,
                        weights, residual, se, ro, cp, grad);
-  (void)hipEventRecord(c->ev1, st);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   return DP_OK;
 }

@@ -397,7 +397,7 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   rc = dph::ensure(c->scratch, 8 * (size_t)ev_batch + 256);
   if (rc) return rc;
   a.w_local = (double*)c->scratch.ptr;
-  (void)hipEventRecord(c->ev0, st);
+  dph::record(c, c->ev0);
   for (int r0 = 0; r0 < nr; r0 += rays_per_batch) {
     const int r1 = std::min(nr, r0 + rays_per_batch);
     a.r0 = r0;
@@ -412,7 +412,7 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
     hipLaunchKernelGGL(path_items_kernel, dim3((unsigned)gi), dim3(PM_THREADS), 0, st, a);
     DP_HIP(hipGetLastError());
   }
-  (void)hipEventRecord(c->ev1, st);
+  dph::record(c, c->ev1);
   hipLaunchKernelGGL(path_reduce, dim3((nr + 255) / 256), dim3(256), 0, st, a.part, nr, chunks,
                      nn, a.cos_t, a.valid, (double*)(b + o_flux), (double*)(b + o_sig));
   DP_HIP(hipGetLastError());

@@ -395,14 +395,14 @@ int dp_trace_tetra(dp_ctx c, const dp_tet_mesh* mesh, int32_t n, const double* o
   const double* d_dir = (const double*)(b + o_dir);
   int32_t* d_st = (int32_t*)(b + o_st);
   const unsigned loc_grid = (unsigned)std::min(n, 65536);
-  (void)hipEventRecord(c->ev0, st);
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(tet_locate_kernel, dim3(loc_grid), dim3(64), 0, st, m, n, d_or, d_dir, d_st);
   hipLaunchKernelGGL(tet_trace_kernel, dim3((n + 63) / 64), dim3(64), 0, st, m, n, d_or, d_dir,
                      (const double*)(b + o_sd), (const int32_t*)d_st,
                      cells ? nullptr : (int32_t*)(b + o_cnt), (const int32_t*)(b + o_rp),
                      cells ? (int32_t*)(b + o_cell) : nullptr, (double*)(b + o_len),
                      (double*)(b + o_tr), (int*)(b + o_fl));
-  (void)hipEventRecord(c->ev1, st);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   int flag = 0;
   if (!cells) {

@@ -165,10 +165,10 @@ int launch_cube(dp_ctx c, const dp_cube_grid* grid, int32_t n, const double* sxy
   if (n == 0) return DP_OK;
   // 64-lane blocks: 16 384 rays still give one wave to every CU
   dim3 b(64), gr((n + 63) / 64);
-  (void)hipEventRecord(c->ev0, c->stream);
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(trace_cube_kernel, gr, b, 0, c->stream, g, n, sxy, dir, counts, row_ptr,
                      cells, lens, z_exit);
-  (void)hipEventRecord(c->ev1, c->stream);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   return DP_OK;
 }
@@ -276,14 +276,14 @@ int dp_trace_edges(dp_ctx c, const double* xe, int32_t nxe, const double* ye, in
   DP_HIP(hipMemcpyAsync(b + o_dir, dir, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
   DP_HIP(hipMemcpyAsync(b + o_sd, surface, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
   if (cells) DP_HIP(hipMemcpyAsync(b + o_rp, row_ptr, 4 * (size_t)(n + 1), hipMemcpyHostToDevice, c->stream));
-  (void)hipEventRecord(c->ev0, c->stream);
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(trace_edges_kernel, dim3((n + 63) / 64), dim3(64), 0, c->stream,
                      (double*)(b + o_x), nxe, (double*)(b + o_y), nye, (double*)(b + o_z), nze, n,
                      (double*)(b + o_or), (double*)(b + o_dir), (double*)(b + o_sd),
                      cells ? nullptr : (int32_t*)(b + o_cnt), (int32_t*)(b + o_rp),
                      cells ? (int32_t*)(b + o_cell) : nullptr, (double*)(b + o_len),
                      (double*)(b + o_tr));
-  (void)hipEventRecord(c->ev1, c->stream);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   if (!cells) {
     DP_HIP(hipMemcpyAsync(counts, b + o_cnt, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));

@@ -718,9 +718,9 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
   a.flux = flux; a.jac = jac;
   a.queue = qerr; a.err = qerr + 1;
   hipLaunchKernelGGL(reset_counters_w, dim3(1), dim3(64), 0, c->stream, qerr, 2);
-  (void)hipEventRecord(c->ev0, c->stream);
+  dph::record(c, c->ev0);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(bd), lds, c->stream, a);
-  (void)hipEventRecord(c->ev1, c->stream);
+  dph::record(c, c->ev1);
   DP_HIP(hipGetLastError());
   return DP_OK;
 }

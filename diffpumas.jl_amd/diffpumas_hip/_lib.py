@@ -180,6 +180,10 @@ _SIGS = {
     "dp_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dp_synchronize": (C.c_int, [C.c_void_p]),
     "dp_device_info": (C.c_int, [C.c_void_p, i32_p, i32_p]),
+    "dp_capture_begin": (C.c_int, [C.c_void_p]),
+    "dp_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "dp_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dp_graph_destroy": (C.c_int, [C.c_void_p]),
     "dp_upload_tables": (C.c_int, [C.c_void_p, C.POINTER(TableDesc), C.POINTER(C.c_void_p)]),
     "dp_free_tables": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dp_trace_cube": (C.c_int, [C.c_void_p, C.POINTER(CubeGrid), C.c_int32, dbl_p, dbl_p, i32_p,
@@ -326,6 +330,42 @@ class Context:
         w = C.c_int32()
         check(self.lib.dp_device_info(self.h, C.byref(n), C.byref(w)))
         return n.value, w.value
+
+    def capture(self, work):
+        """Run work() (calls of _dev entry points on this context) under
+        stream capture and return the resulting Graph; nothing runs until
+        Graph.launch().  Call the same work once eagerly first so that every
+        buffer already has its size."""
+        check(self.lib.dp_capture_begin(self.h))
+        try:
+            work()
+        finally:
+            g = C.c_void_p()
+            rc = self.lib.dp_capture_end(self.h, C.byref(g))
+        check(rc)
+        return Graph(self, g)
+
+
+class Graph:
+    """A captured HIP graph of device work, replayed on its context's stream."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def launch(self):
+        check(self.ctx.lib.dp_graph_launch(self.ctx.h, self.h))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.dp_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default_ctx = None

@@ -77,6 +77,22 @@ const char* dp_last_error(void);
  * the context's own stream. */
 int dp_set_stream(dp_ctx ctx, void* stream);
 int dp_synchronize(dp_ctx ctx);
+/* HIP graphs of device work (no counterpart in the reference: Julia re-runs
+ * the calls).  Between dp_capture_begin and dp_capture_end the context's
+ * stream is captured instead of run; only _dev entry points may be called
+ * in between, each already called once with the same sizes so that no
+ * buffer grows during the capture (a growth or a host entry point makes the
+ * capture fail, reported by dp_capture_end).  dp_graph_launch replays the
+ * captured work on the context's stream with the device buffers' current
+ * contents (e.g. the per-iteration forward model + Jacobian + normal
+ * equations of a Gauss-Newton loop, gauss_newton_reconstruct,
+ * src/Tomography_inverse.jl:448-603).  Kernel timings are not recorded
+ * inside a graph.                                                          */
+typedef struct dp_graph_s* dp_graph;
+int dp_capture_begin(dp_ctx ctx);
+int dp_capture_end(dp_ctx ctx, dp_graph* out);
+int dp_graph_launch(dp_ctx ctx, dp_graph graph);
+int dp_graph_destroy(dp_graph graph);
 /* Number of compute units and wavefronts the autotuned launches assume. */
 int dp_device_info(dp_ctx ctx, int32_t* n_cu, int32_t* wave_size);
 

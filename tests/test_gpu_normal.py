@@ -68,10 +68,10 @@ def test_wmodel_operator_bit_exact(ctx):
 
 
 def test_multi_tile_and_long_column(ctx):
-    J = _rand_csr(3000, 6000, 40, 5, dense_col=17)
+    J = _rand_csr(4000, 6000, 40, 5, dense_col=17)   # > 16 384 digit counts: a multi-chunk scan
     rng = np.random.default_rng(6)
-    W = rng.uniform(0.2, 3.0, 3000)
-    r = rng.normal(0.0, 1.0, 3000)
+    W = rng.uniform(0.2, 3.0, 4000)
+    r = rng.normal(0.0, 1.0, 4000)
     _say(f"normal eqs (random): {J.shape}, {J.nnz} entries, 6000 active (2 tiles)")
     N, g = dp.normal_equations(J, r, W, None, ctx=ctx)
     No, go = O.normal_equations(J, r, W, None)
