@@ -63,71 +63,86 @@ struct StepOut {
 // stopping-power ratio and the decay factor at the mean energy.  The interval
 // searches start from what the walk already knows: ie is the interval of E
 // on the log-energy grid (-1: unknown or E clamped), jr the last interval
-// found on the range grid.  locate_idx returns the interval the bisection
-// would (interp_e2, inverse_range, interp_e), so every value is unchanged.
-// On return ie is the interval of the exit energy, which is the next
-// segment's entry energy on the same tables.
+// found on the range grid; locate_idx returns the interval the bisection
+// would.  On return ie is the interval of the exit energy, which is the next
+// segment's entry energy on the same tables.  Quotients that share a divisor
+// use one reciprocal (FP64 division costs ~10 quarter-rate instructions on
+// gfx950): values agree with the division form to a few ulp.
 __device__ __forceinline__ StepOut csda_segment_g(const double* sE, const double* sLE,
                                                   const double* sR, const double* sS, int K,
                                                   int pow2, double E, double lE, double gram,
-                                                  double path, double mass, double ctau, int& ie,
-                                                  int& jr) {
+                                                  double path, double mass, double inv_mass,
+                                                  double ctau, int& ie, int& jr) {
   StepOut o;
-  Lookup2 in;
+  double y0, dy0, y1, dy1;
   if (E <= sE[0]) {
-    in.y0 = sR[0]; in.y1 = sS[0]; in.dy0 = in.dy1 = 0.0;
+    y0 = sR[0]; y1 = sS[0]; dy0 = dy1 = 0.0;
   } else if (E >= sE[K - 1]) {
-    in.y0 = sR[K - 1]; in.y1 = sS[K - 1]; in.dy0 = in.dy1 = 0.0;
+    y0 = sR[K - 1]; y1 = sS[K - 1]; dy0 = dy1 = 0.0;
   } else {
     const int i = ie >= 0 ? ie : locate_idx(sLE, K, pow2, lE, jr);
-    const double dl = sLE[i + 1] - sLE[i];
-    const double t = (lE - sLE[i]) / dl;
+    const double rdl = 1.0 / (sLE[i + 1] - sLE[i]);
+    const double t = (lE - sLE[i]) * rdl;
     const double d0 = sR[i + 1] - sR[i], d1 = sS[i + 1] - sS[i];
-    in.y0 = sR[i] + t * d0;
-    in.y1 = sS[i] + t * d1;
-    in.dy0 = (d0 / dl) / E;
-    in.dy1 = (d1 / dl) / E;
+    y0 = sR[i] + t * d0;
+    y1 = sS[i] + t * d1;
+    const double re = rdl / E;
+    dy0 = d0 * re;
+    dy1 = d1 * re;
   }
-  const double Xf = in.y0 + gram;
-  Lookup inv;
+  const double Xf = y0 + gram;
+  double Eo, kdy;
   if (Xf <= sR[0]) {
-    inv.y = sE[0]; inv.dy = 0.0;
+    Eo = sE[0]; kdy = 0.0;
   } else if (Xf >= sR[K - 1]) {
-    inv.y = sE[K - 1]; inv.dy = 0.0;
+    Eo = sE[K - 1]; kdy = 0.0;
   } else {
     const int j = locate_idx(sR, K, pow2, Xf, jr);
     jr = j;
-    const double dr = sR[j + 1] - sR[j];
-    const double t = (Xf - sR[j]) / dr;
+    const double rdr = 1.0 / (sR[j + 1] - sR[j]);
+    const double t = (Xf - sR[j]) * rdr;
     const double dle = sLE[j + 1] - sLE[j];
-    const double le = sLE[j] + t * dle;
-    inv.y = exp(le);
-    inv.dy = inv.y * (dle / dr);
+    Eo = exp(sLE[j] + t * dle);
+    kdy = Eo * (dle * rdr);
   }
-  o.E = inv.y;
-  o.ok = !(o.E >= 1e12) && isfinite(o.E);
-  o.lE = log(o.E);
-  Lookup so;
-  if (o.E <= sE[0]) {
-    so.y = sS[0]; so.dy = 0.0; ie = -1;
-  } else if (o.E >= sE[K - 1]) {
-    so.y = sS[K - 1]; so.dy = 0.0; ie = -1;
+  o.E = Eo;
+  o.ok = !(Eo >= 1e12) && isfinite(Eo);
+  o.lE = log(Eo);
+  double so_y, so_dy;
+  if (Eo <= sE[0]) {
+    so_y = sS[0]; so_dy = 0.0; ie = -1;
+  } else if (Eo >= sE[K - 1]) {
+    so_y = sS[K - 1]; so_dy = 0.0; ie = -1;
   } else {
     const int i = locate_idx(sLE, K, pow2, o.lE, jr);
     ie = i;
-    const double dl = sLE[i + 1] - sLE[i];
-    const double t = (o.lE - sLE[i]) / dl;
+    const double rdl = 1.0 / (sLE[i + 1] - sLE[i]);
+    const double t = (o.lE - sLE[i]) * rdl;
     const double dyy = sS[i + 1] - sS[i];
-    so.y = sS[i] + t * dyy;
-    so.dy = (dyy / dl) / o.E;
+    so_y = sS[i] + t * dyy;
+    so_dy = dyy * (rdl / Eo);
   }
-  double dlog;
-  o.dec = decay_weight_dlog(path, (E + o.E) / 2.0, mass, ctau, &dlog);
-  o.ratio = so.y / in.y1;
-  o.k = inv.dy;
-  o.c = inv.dy * in.dy0;
-  o.b = -in.dy1 / in.y1 + 0.5 * dlog;
-  o.e = so.dy / so.y + 0.5 * dlog;
+  // decay_weight with its log-derivative (compute_decay_weight_from_path,
+  // src/Geometry.jl:1083-1107): beta^2 = 1 - 1/gamma^2 floored at 1e-12
+  const double g = 1.0 + 0.5 * (E + Eo) * inv_mass;
+  const double b2 = 1.0 - 1.0 / (g * g);
+  double beta, dbg_dg;
+  if (b2 >= 1e-12) {
+    beta = sqrt(b2);
+    dbg_dg = 1.0 / beta;
+  } else {
+    beta = 1e-6;
+    dbg_dg = beta;
+  }
+  const double rL = 1.0 / (beta * g * ctau);
+  o.dec = exp(-path * rL);
+  const double dlog = path * rL * rL * (ctau * dbg_dg * inv_mass);
+  const double ry1 = 1.0 / y1, rso = 1.0 / so_y;
+  o.ratio = so_y * ry1;
+  o.k = kdy;
+  o.c = kdy * dy0;
+  o.b = -dy1 * ry1 + 0.5 * dlog;
+  o.e = so_dy * rso + 0.5 * dlog;
   return o;
 }
 
@@ -159,6 +174,7 @@ __global__ __launch_bounds__(256) void csda_cube_kernel(CsdaArgs a) {
   }
   double* sc = a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd;
   const double rho0 = a.dens[0];
+  const double inv_mass = 1.0 / a.mass;
 
   for (;;) {
     if (tid == 0) s_ray = atomicAdd(a.queue, 1);
@@ -200,8 +216,8 @@ __global__ __launch_bounds__(256) void csda_cube_kernel(CsdaArgs a) {
         const int cell = a.cell[b0 + s];
         const double d = a.len[b0 + s];
         const double rho = a.dens[cell];
-        StepOut o = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, a.ctau,
-                                   ie, jr);
+        StepOut o = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, inv_mass,
+                                   a.ctau, ie, jr);
         ok = ok && o.ok;
         W = W * o.ratio;
         W = W * o.dec;
@@ -216,8 +232,8 @@ __global__ __launch_bounds__(256) void csda_cube_kernel(CsdaArgs a) {
       StepOut orem;
       orem.c = orem.b = orem.e = orem.k = 0.0;
       if (has_rem) {
-        orem = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass, a.ctau,
-                              ie, jr);
+        orem = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass,
+                              inv_mass, a.ctau, ie, jr);
         ok = ok && orem.ok;
         W = W * orem.ratio;
         W = W * orem.dec;
@@ -226,7 +242,7 @@ __global__ __launch_bounds__(256) void csda_cube_kernel(CsdaArgs a) {
       }
       int ie_air = -1, jr_air = jr;   // air tables: a fresh entry search
       StepOut oair = csda_segment_g(aE, aLE, aR, aS, K, a.pow2, E, lE, Xair, path_air, a.mass,
-                                    a.ctau, ie_air, jr_air);
+                                    inv_mass, a.ctau, ie_air, jr_air);
       ok = ok && oair.ok;
       W = W * oair.ratio;
       W = W * oair.dec;
@@ -299,9 +315,6 @@ __global__ void vjp_keys(int n_rays, int n_seg, const int32_t* __restrict__ row_
     vals[s] = s;
     ray_of[s] = r;
   }
-  keys[n_seg + r] = 0;
-  vals[n_seg + r] = n_seg + r;
-  ray_of[n_seg + r] = r;
 }
 
 __device__ __forceinline__ int lower_bound_i(const int32_t* a, int n, int x) {
@@ -313,20 +326,58 @@ __device__ __forceinline__ int lower_bound_i(const int32_t* a, int n, int x) {
   return lo;
 }
 
-__global__ void vjp_gather(int n_cells, int n_seg, int n_ent, const int32_t* __restrict__ skeys,
+__global__ void vjp_gather(int n_cells, int n_seg, const int32_t* __restrict__ skeys,
                            const int32_t* __restrict__ svals, const int32_t* __restrict__ ray_of,
-                           const double* __restrict__ jac, const double* __restrict__ jac_rem,
-                           const double* __restrict__ fbar, double* __restrict__ grad) {
+                           const double* __restrict__ jac, const double* __restrict__ fbar,
+                           double* __restrict__ grad) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cells) return;
-  int p = lower_bound_i(skeys, n_ent, c);
+  int p = lower_bound_i(skeys, n_seg, c);
   double acc = 0.0;
-  for (; p < n_ent && skeys[p] == c; ++p) {
-    int v = svals[p];
-    double j = v < n_seg ? jac[v] : (jac_rem ? jac_rem[v - n_seg] : 0.0);
-    acc += fbar[ray_of[v]] * j;
+  for (; p < n_seg && skeys[p] == c; ++p) {
+    const int v = svals[p];
+    acc += fbar[ray_of[v]] * jac[v];
   }
   grad[c] = acc;
+}
+
+// The remaining-rock derivatives all belong to cell 0 (the remainder is
+// transported at rho[0]): a fixed-shape two-stage tree sums
+// fbar[r] jac_rem[r] over the rays and the result is added to grad[0], so
+// one cell does not serialise n_rays terms.
+constexpr int VJP_REM_THREADS = 256, VJP_REM_PER = 16;
+__device__ __forceinline__ double block_sum_fixed(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = VJP_REM_THREADS / 2; off > 0; off >>= 1) {
+    if (t < off) sh[t] += sh[t + off];
+    __syncthreads();
+  }
+  return sh[0];
+}
+__global__ __launch_bounds__(VJP_REM_THREADS) void vjp_rem_partial(int n_rays,
+                                                                    const double* __restrict__ jac_rem,
+                                                                    const double* __restrict__ fbar,
+                                                                    double* __restrict__ part) {
+  __shared__ double sh[VJP_REM_THREADS];
+  const int base = blockIdx.x * VJP_REM_THREADS * VJP_REM_PER;
+  double v = 0.0;
+  for (int k = 0; k < VJP_REM_PER; ++k) {
+    const int r = base + k * VJP_REM_THREADS + (int)threadIdx.x;
+    if (r < n_rays) v += fbar[r] * jac_rem[r];
+  }
+  const double tot = block_sum_fixed(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(VJP_REM_THREADS) void vjp_rem_final(int n_part,
+                                                                  const double* __restrict__ part,
+                                                                  double* __restrict__ grad) {
+  __shared__ double sh[VJP_REM_THREADS];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n_part; i += VJP_REM_THREADS) v += part[i];
+  const double tot = block_sum_fixed(v, sh);
+  if (threadIdx.x == 0) grad[0] = grad[0] + tot;
 }
 
 int check_paths(const dp_paths* p, const dp_csda_nodes* nd, int32_t n_cells, dp_tables t) {
@@ -477,7 +528,7 @@ int dp_csda_cube_vjp_dev(dp_ctx c, const dp_paths* p, const double* jac, const d
   if (!c || !p || !jac || !fbar || !grad) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: null argument");
   if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: need at least one cell");
   DP_HIP(hipSetDevice(c->device));
-  const int nr = p->n_rays, ns = p->n_segments, ne = ns + nr;
+  const int nr = p->n_rays, ns = p->n_segments, ne = ns;
   if (nr == 0) {   // a fill kernel, not a memset node, so the call can sit in a captured graph
     hipLaunchKernelGGL(vjp_zero, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
                        grad);
@@ -491,6 +542,8 @@ int dp_csda_cube_vjp_dev(dp_ctx c, const dp_paths* p, const double* jac, const d
   const size_t o_k = take(4 * (size_t)ne), o_v = take(4 * (size_t)ne), o_ro = take(4 * (size_t)ne),
                o_p0 = take(8 * (size_t)ne), o_p1 = take(8 * (size_t)ne),
                o_h = take(4 * dps::sort_hist_ints(ne));
+  const int n_part = (nr + VJP_REM_THREADS * VJP_REM_PER - 1) / (VJP_REM_THREADS * VJP_REM_PER);
+  const size_t o_part = take(8 * (size_t)n_part);
   int rc = dph::ensure(c->scratch2, o);
   if (rc) return rc;
   char* b = (char*)c->scratch2.ptr;
@@ -499,10 +552,18 @@ int dp_csda_cube_vjp_dev(dp_ctx c, const dp_paths* p, const double* jac, const d
                      p->row_ptr, p->cell, k, v, ro);
   DP_HIP(hipGetLastError());
   const int32_t *sk = k, *sv = v;
-  dps::radix_sort_pairs(k, v, ne, bits, (int32_t*)(b + o_p0), (int32_t*)(b + o_p1),
-                        (int32_t*)(b + o_h), c->stream, &sk, &sv);
+  if (ns > 0)
+    dps::radix_sort_pairs(k, v, ns, bits, (int32_t*)(b + o_p0), (int32_t*)(b + o_p1),
+                          (int32_t*)(b + o_h), c->stream, &sk, &sv);
   hipLaunchKernelGGL(vjp_gather, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
-                     ns, ne, sk, sv, ro, jac, jac_rem, fbar, grad);
+                     ns, sk, sv, ro, jac, fbar, grad);
+  if (jac_rem) {
+    double* part = (double*)(b + o_part);
+    hipLaunchKernelGGL(vjp_rem_partial, dim3(n_part), dim3(VJP_REM_THREADS), 0, c->stream, nr,
+                       jac_rem, fbar, part);
+    hipLaunchKernelGGL(vjp_rem_final, dim3(1), dim3(VJP_REM_THREADS), 0, c->stream, n_part, part,
+                       grad);
+  }
   DP_HIP(hipGetLastError());
   return DP_OK;
 }
