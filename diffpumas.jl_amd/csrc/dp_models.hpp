@@ -190,93 +190,108 @@ DPM_HD_COLD double dcs_brems(double Z, double A, double m, double K, double q) {
 // dcs_pair call) and the per-node part.  Each term is the same expression as
 // in the single-function form, so the values are unchanged.
 struct PairInv {
-  double Z, energy, v, m, rl, cpf, Z13, d_n, zeta, beta;
+  double Z, v, rl, cpf, Z13, d_n, zeta, beta;
+  // products of the above that every node uses (the reference evaluates this
+  // model under @fastmath, which permits regrouping them)
+  double kxi;       // (m v / 2 m_e)^2 / (1 - v): xi = kxi (1 - rho^2)
+  double kq1, kq2;  // 2 m_e e^{1/2} rl Z^{-1/3} / (E v) and the same with e^{1/3}
+  double rz;        // rl Z^{-1/3}
+  double lm0;       // (m / m_e) rl Z^{-1/3} / d_n
+  double dmm;       // (m_e / m d_n)^2
+  double pe, pm;    // cpf (Z + zeta) (1 - v) / v, and times (m_e / m)^2
+  double scale;     // 0.1 / E
 };
 DPM_HD PairInv pair_invariants(double Z, double A, double mass, double K, double q) {
-  const double RE = 2.8179403227e-13;
+  const double ME = 0.5109989461, RE = 2.8179403227e-13;
   PairInv c;
   c.Z = Z;
-  c.energy = (K + mass) * 1e3;
+  const double energy = (K + mass) * 1e3;
   c.v = q / (K + mass);
-  c.m = mass * 1e3;
+  const double m = mass * 1e3;
   c.rl = radiation_logarithm(Z);
   c.cpf = 4.0 / (3.0 * PI) * Z * (ALPHA_EM * RE) * (ALPHA_EM * RE);
   c.Z13 = pow(Z, -1.0 / 3.0);
   c.d_n = 1.54 * pow(A, 0.27);
-  const double energy = c.energy, m = c.m, Z13 = c.Z13, v = c.v;
+  const double Z13 = c.Z13, v = c.v;
   const double g1 = Z == 1.0 ? 4.4e-5 : 1.95e-5;
   const double g2 = Z == 1.0 ? 4.8e-5 : 5.3e-5;
   const double zeta1 = 0.073 * log(energy / m / (1.0 + g1 * pow(Z, 2.0 / 3.0) * energy / m)) - 0.26;
   const double zeta2 = 0.058 * log(energy / m / (1.0 + g2 / Z13 * energy / m)) - 0.14;
   c.zeta = (zeta1 > 0.0 && zeta2 > 0.0) ? zeta1 / zeta2 : 0.0;
   c.beta = v * v / (2.0 * (1.0 - v));
+  const double mv = m * v / (2.0 * ME);
+  c.kxi = mv * mv / (1.0 - v);
+  c.rz = c.rl * Z13;
+  const double ev = energy * v;
+  c.kq1 = 2.0 * ME * exp(0.5) * c.rz / ev;
+  c.kq2 = 2.0 * ME * exp(1.0 / 3.0) * c.rz / ev;
+  c.lm0 = m / ME * c.rz / c.d_n;
+  const double md = ME / m * c.d_n;
+  c.dmm = md * md;
+  c.pe = c.cpf * (Z + c.zeta) * (1.0 - v) / v;
+  c.pm = c.pe * (ME / m) * (ME / m);
+  c.scale = 0.1 / energy;
   return c;
 }
+// _dcs_pair_production_d2_ssr at rho = 1 - rho_in, with the terms shared by
+// several expressions (1/(1+xi), 1/xi, 1/(1-rho^2)) formed once.
 DPM_HD double dcs_pair_d2(const PairInv& c, double rho_in) {
-  const double ME = 0.5109989461;
-  const double Z = c.Z, energy = c.energy, v = c.v, m = c.m, rl = c.rl, cpf = c.cpf;
-  const double Z13 = c.Z13, d_n = c.d_n, zeta = c.zeta, beta = c.beta;
+  const double beta = c.beta;
   const double rho = 1.0 - rho_in;
   const double rho2 = rho * rho;
-  const double xi = (m * v / (2.0 * ME)) * (m * v / (2.0 * ME)) * (1.0 - rho2) / (1.0 - v);
-  const double lxi = log(1.0 + 1.0 / xi);
+  const double omr = 1.0 - rho2;
+  const double xi = c.kxi * omr;
+  const double ixi = 1.0 / xi;
+  const double opx = 1.0 + xi;
+  const double iopx = 1.0 / opx;
+  const double iomr = 1.0 / omr;
+  const double lxi = log(1.0 + ixi);
   const double Be = ((2.0 + rho2) * (1.0 + beta) + xi * (3.0 + rho2)) * lxi +
-                    (1.0 - rho2 - beta) / (1.0 + xi) - (3.0 + rho2);
-  const double Ce2 = ((1.0 - rho2) * (1.0 + beta) + xi * (3.0 - rho2)) * lxi +
-                     2.0 * (1.0 - beta - rho2) / (1.0 + xi) - (3.0 - rho2);
+                    (omr - beta) * iopx - (3.0 + rho2);
+  const double Ce2 = (omr * (1.0 + beta) + xi * (3.0 - rho2)) * lxi +
+                     2.0 * (1.0 - beta - rho2) * iopx - (3.0 - rho2);
   const double Ce1 = Be - Ce2;
-  const double De = ((2.0 + rho2) * (1.0 + beta) + xi * (3.0 + rho2)) * dilog(1.0 / (1.0 + xi)) -
-                    (2.0 + rho2) * xi * lxi - (xi + rho2 + beta) / (1.0 + xi);
-  const double den = energy * v * (1.0 - rho2);
-  const double dmm = (ME / m * d_n) * (ME / m * d_n);
+  const double De = ((2.0 + rho2) * (1.0 + beta) + xi * (3.0 + rho2)) * dilog(iopx) -
+                    (2.0 + rho2) * xi * lxi - (xi + rho2 + beta) * iopx;
+  const double q1 = c.kq1 * opx * iomr, q2 = c.kq2 * opx * iomr;   // 2 m_e e^k rl Z13 (1+xi)/den
+  const double sq = c.rz * sqrt(opx);
+  const double tmpe = De / Be;
   double Le1, Le2;
-  if (De / Be > 0.0) {
-    const double tmp = De / Be;
-    const double Xe = tmp < 100.0 ? exp(-tmp) : 0.0;
-    Le1 = log(rl * Z13 * sqrt(1.0 + xi) /
-              (Xe + 2.0 * ME * exp(0.5) * rl * Z13 * (1.0 + xi) / den)) -
-          tmp - 0.5 * log(Xe + dmm * (1.0 + xi));
-    Le2 = log(rl * Z13 * exp(-1.0 / 6.0) * sqrt(1.0 + xi) /
-              (Xe + 2.0 * ME * exp(1.0 / 3.0) * rl * Z13 * (1.0 + xi) / den)) -
-          tmp - 0.5 * log(Xe + dmm * exp(-1.0 / 3.0) * (1.0 + xi));
+  if (tmpe > 0.0) {
+    const double Xe = tmpe < 100.0 ? exp(-tmpe) : 0.0;
+    Le1 = log(sq / (Xe + q1)) - tmpe - 0.5 * log(Xe + c.dmm * opx);
+    Le2 = log(exp(-1.0 / 6.0) * sq / (Xe + q2)) - tmpe -
+          0.5 * log(Xe + c.dmm * exp(-1.0 / 3.0) * opx);
   } else {
-    const double tmp = De / Be;
-    const double Xi = tmp > -100.0 ? exp(tmp) : 0.0;
-    Le1 = log(rl * Z13 * sqrt(1.0 + xi) /
-              (1.0 + Xi * 2.0 * ME * exp(0.5) * rl * Z13 * (1.0 + xi) / den)) -
-          0.5 * tmp - 0.5 * log(1.0 + Xi * dmm * (1.0 + xi));
-    Le2 = log(rl * Z13 * exp(-1.0 / 6.0) * sqrt(1.0 + xi) /
-              (1.0 + Xi * 2.0 * ME * exp(1.0 / 3.0) * rl * Z13 * (1.0 + xi) / den)) -
-          0.5 * tmp - 0.5 * log(1.0 + Xi * dmm * exp(-1.0 / 3.0) * (1.0 + xi));
+    const double Xi = tmpe > -100.0 ? exp(tmpe) : 0.0;
+    Le1 = log(sq / (1.0 + Xi * q1)) - 0.5 * tmpe - 0.5 * log(1.0 + Xi * c.dmm * opx);
+    Le2 = log(exp(-1.0 / 6.0) * sq / (1.0 + Xi * q2)) - 0.5 * tmpe -
+          0.5 * log(1.0 + Xi * c.dmm * exp(-1.0 / 3.0) * opx);
   }
-  double diag_e = cpf * (Z + zeta) * (1.0 - v) / v * (Ce1 * Le1 + Ce2 * Le2);
+  double diag_e = c.pe * (Ce1 * Le1 + Ce2 * Le2);
   if (diag_e < 0.0) diag_e = 0.0;
-  const double l1xi = log(1.0 + xi);
-  const double Bm = ((1.0 + rho2) * (1.0 + 1.5 * beta) - 1.0 / xi * (1.0 + 2.0 * beta) * (1.0 - rho2)) * l1xi +
-                    xi * (1.0 - rho2 - beta) / (1.0 + xi) + (1.0 + 2.0 * beta) * (1.0 - rho2);
-  const double Cm2 = ((1.0 - beta) * (1.0 - rho2) - xi * (1.0 + rho2)) * l1xi / xi -
-                     2.0 * (1.0 - beta - rho2) / (1.0 + xi) + 1.0 - beta - (1.0 + beta) * rho2;
+  const double l1xi = log(opx);
+  const double bm0 = (1.0 + rho2) * (1.0 + 1.5 * beta) - ixi * (1.0 + 2.0 * beta) * omr;
+  const double Bm = bm0 * l1xi + xi * (omr - beta) * iopx + (1.0 + 2.0 * beta) * omr;
+  const double Cm2 = ((1.0 - beta) * omr - xi * (1.0 + rho2)) * l1xi * ixi -
+                     2.0 * (1.0 - beta - rho2) * iopx + 1.0 - beta - (1.0 + beta) * rho2;
   const double Cm1 = Bm - Cm2;
-  const double Dm = ((1.0 + rho2) * (1.0 + 1.5 * beta) - 1.0 / xi * (1.0 + 2.0 * beta) * (1.0 - rho2)) *
-                        dilog(xi / (1.0 + xi)) +
-                    (1.0 + 1.5 * beta) * (1.0 - rho2) / xi * l1xi +
-                    (1.0 - rho2 - 0.5 * beta * (1.0 + rho2) + (1.0 - rho2) / (2.0 * xi) * beta) *
-                        xi / (1.0 + xi);
+  const double Dm = bm0 * dilog(xi * iopx) + (1.0 + 1.5 * beta) * omr * ixi * l1xi +
+                    (omr - 0.5 * beta * (1.0 + rho2) + 0.5 * omr * ixi * beta) * xi * iopx;
+  const double tmpm = Dm / Bm;
   double Lm1, Lm2;
-  if (Dm / Bm > 0.0) {
-    const double tmp = Dm / Bm;
-    const double Xm = tmp < 100.0 ? exp(-tmp) : 0.0;
-    Lm1 = log(m / ME * rl * Z13 / d_n / (Xm + 2.0 * ME * exp(0.5) * rl * Z13 * (1.0 + xi) / den)) - tmp;
-    Lm2 = log(m / ME * rl * Z13 / d_n / (Xm + 2.0 * ME * exp(1.0 / 3.0) * rl * Z13 * (1.0 + xi) / den)) - tmp;
+  if (tmpm > 0.0) {
+    const double Xm = tmpm < 100.0 ? exp(-tmpm) : 0.0;
+    Lm1 = log(c.lm0 / (Xm + q1)) - tmpm;
+    Lm2 = log(c.lm0 / (Xm + q2)) - tmpm;
   } else {
-    const double tmp = Dm / Bm;
-    const double Xmv = tmp > -100.0 ? exp(tmp) : 0.0;
-    Lm1 = log(m / ME * rl * Z13 / d_n / (1.0 + 2.0 * ME * exp(0.5) * rl * Z13 * (1.0 + xi) / den * Xmv));
-    Lm2 = log(m / ME * rl * Z13 / d_n / (1.0 + 2.0 * ME * exp(1.0 / 3.0) * rl * Z13 * (1.0 + xi) / den * Xmv));
+    const double Xmv = tmpm > -100.0 ? exp(tmpm) : 0.0;
+    Lm1 = log(c.lm0 / (1.0 + q1 * Xmv));
+    Lm2 = log(c.lm0 / (1.0 + q2 * Xmv));
   }
-  double diag_mu = cpf * (Z + zeta) * (1.0 - v) / v * (ME / m) * (ME / m) * (Cm1 * Lm1 + Cm2 * Lm2);
+  double diag_mu = c.pm * (Cm1 * Lm1 + Cm2 * Lm2);
   if (diag_mu < 0.0) diag_mu = 0.0;
-  return (diag_e + diag_mu) * 0.1 / energy;
+  return (diag_e + diag_mu) * c.scale;
 }
 
 // dcs_pair_production_ssr, src/Materials.jl:802-832
