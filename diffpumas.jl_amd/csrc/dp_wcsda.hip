@@ -335,18 +335,35 @@ __host__ __device__ constexpr int w_tab_cols(bool corr) { return corr ? 18 : 12;
 // Tape doubles per segment and lane.
 __host__ __device__ constexpr int w_tape(bool corr) { return corr ? 8 : 4; }
 
-template <bool JAC, bool CORR>
+// Half-wave sum (lanes 0-31 and 32-63 separately).
+__device__ __forceinline__ double half_sum(double v) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// PACK (at most 32 energy nodes): a 64-lane block carries two bins, one per
+// half-wave; each half sums its nodes with half_sum, which equals the
+// full-wave sum of the unpacked layout (whose upper half only adds zeros).
+template <bool JAC, bool CORR, bool PACK>
 __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
   extern __shared__ double lds[];
   const int K = a.K;
   const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwv = bd >> 6;
+  const int grp = PACK ? (tid >> 5) : 0;      // bin slot of this lane
+  const int gt = PACK ? (tid & 31) : tid;     // lane within the slot
+  const int gsize = PACK ? 32 : bd;           // lanes per bin
+  const int nwv = PACK ? 2 : (bd >> 6);       // partial-sum rows in LDS
+  const int rows = PACK ? 1 : nwv;            // rows per bin
+  const int row0 = PACK ? grp : 0;            // first row of this lane's bin
+  const int wrow = PACK ? grp : wave;         // row this lane's wave (or half) adds to
+  const int wl = PACK ? gt : lane;            // lane index within that wave (or half)
   constexpr int NT = w_tape(CORR);
   double* wsum = lds + w_tab_cols(CORR) * K;              // [nwv][max_seg]
   double* wflux = wsum + (size_t)nwv * a.max_seg;         // [nwv]
-  int* scell = (int*)(wflux + nwv);                       // [max_seg]
-  __shared__ int s_ray;
-  __shared__ double s_G, s_M, s_q, s_mass;                // per-bin multipliers (CORR)
+  int* scell = (int*)(wflux + nwv) + grp * a.max_seg;     // [slots][max_seg]
+  __shared__ int s_ray[2];
+  __shared__ double s_G[2], s_M[2], s_q[2], s_mass[2];    // per-bin multipliers (CORR)
   {
     const int mats[4] = {a.rock, a.water, a.porous >= 0 ? a.porous : a.rock, a.air};
     for (int j = 0; j < 4; ++j) {
@@ -383,39 +400,44 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
   double* sc = JAC ? a.scratch + (size_t)blockIdx.x * a.max_seg * NT * bd : nullptr;
 
   for (;;) {
-    if (tid == 0) s_ray = atomicAdd(a.queue, 1);
+    if (gt == 0) s_ray[grp] = atomicAdd(a.queue, 1);
     __syncthreads();
-    const int r = s_ray;
-    if (r >= a.n_rays) break;
-    const int b0 = a.row_ptr[r];
-    const int ns = a.row_ptr[r + 1] - b0;
-    if (ns < 0 || ns > a.max_seg) {
-      if (tid == 0) { atomicOr(a.err, ERR_CAP); a.flux[r] = __builtin_nan(""); }
-      __syncthreads();
-      continue;
+    const int r = s_ray[grp];
+    if (PACK ? (s_ray[0] >= a.n_rays && s_ray[1] >= a.n_rays) : r >= a.n_rays) break;
+    // every lane of the block meets the same barriers: a slot without a usable
+    // bin carries live = false (ns = 0) through the iteration
+    bool live = r < a.n_rays;
+    int b0 = 0, ns = 0;
+    if (live) {
+      b0 = a.row_ptr[r];
+      ns = a.row_ptr[r + 1] - b0;
+      if (ns < 0 || ns > a.max_seg) {
+        if (gt == 0) { atomicOr(a.err, ERR_CAP); a.flux[r] = __builtin_nan(""); }
+        live = false;
+        ns = 0;
+      }
     }
-    const double ct = a.cos_t[r];
-    const bool valid = (a.valid ? a.valid[r] != 0 : true) && ct > 0.0;
-    if (!valid) {
-      if (tid == 0) a.flux[r] = __builtin_nan("");
-      if (JAC) for (int s = tid; s < ns; s += bd) a.jac[b0 + s] = 0.0;
-      __syncthreads();
-      continue;
+    const double ct = live ? a.cos_t[r] : 1.0;
+    if (live && !((a.valid ? a.valid[r] != 0 : true) && ct > 0.0)) {
+      if (gt == 0) a.flux[r] = __builtin_nan("");
+      if (JAC) for (int s = gt; s < ns; s += gsize) a.jac[b0 + s] = 0.0;
+      live = false;
+      ns = 0;
     }
-    for (int s = tid; s < ns; s += bd) {
+    for (int s = gt; s < ns; s += gsize) {
       int c = a.cell[b0 + s];
       if (c < 0 || c >= a.n_cells) { atomicOr(a.err, ERR_CELL); c = -1; }
       scell[s] = c;
     }
-    if (JAC) for (int s = lane; s < ns; s += 64) wsum[wave * a.max_seg + s] = 0.0;
-    if (lane == 0) wflux[wave] = 0.0;
+    if (JAC) for (int s = wl; s < ns; s += (PACK ? 32 : 64)) wsum[wrow * a.max_seg + s] = 0.0;
+    if (wl == 0) wflux[wrow] = 0.0;
     __syncthreads();
 
-    const double rem = a.remaining[r];
-    const double zex = a.exit_z[r];
+    const double rem = live ? a.remaining[r] : 0.0;
+    const double zex = live ? a.exit_z[r] : 0.0;
     const double air_d = fmax(0.0, a.palt - zex) / fmax(ct, 1e-3);
     const double z0 = a.det_elev + zex;
-    if (CORR && tid == 0) {
+    if (CORR && live && gt == 0) {
       // csda_geom_multiplier (path_rock_length) and _column_water_fraction,
       // summed in path order
       double L = rem, water = 0.0, mass = rem * M.rho_r;
@@ -436,15 +458,15 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
         G = exp(C.ra + C.rb * lL + C.rc * lL * lL + C.rd * ct);
       }
       const double q = bad ? 0.0 : water / fmax(mass, 1e-30);
-      s_G = G;
-      s_q = q;
-      s_mass = mass;
-      s_M = C.mat ? exp(C.lin * q + C.quad * q * q) : 1.0;
+      s_G[grp] = G;
+      s_q[grp] = q;
+      s_mass[grp] = mass;
+      s_M[grp] = C.mat ? exp(C.lin * q + C.quad * q * q) : 1.0;
     }
 
-    for (int n0 = 0; n0 < a.n_nodes; n0 += bd) {
-      const int n = n0 + tid;
-      const bool active = n < a.n_nodes;
+    for (int n0 = 0; n0 < a.n_nodes; n0 += gsize) {
+      const int n = n0 + gt;
+      const bool active = live && n < a.n_nodes;
       double E = active ? a.ne[n] : a.ne[0];
       double W = 1.0, V = 0.0, H = 0.0;
       int gc = K / 2, ga = K / 2;  // interval guesses: cell/rock and air stopping power
@@ -561,8 +583,8 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
           f = 2.0 * a.nw[n] * (W * cv * flux_gccly(ct, E, a.nq[n]));
         }
       }
-      const double fs = wave_sum(f);
-      if (lane == 0) wflux[wave] += fs;
+      const double fs = PACK ? half_sum(f) : wave_sum(f);
+      if (wl == 0) wflux[wrow] += fs;
       if (JAC) {
         // ---------------------------------------------- reverse sweep
         for (int s = ns - 1; s >= 0; --s) {
@@ -577,21 +599,21 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
               Tad = p[3 * bd] + Tad * p[bd];
             }
           }
-          const double vs = wave_sum(v);
-          if (lane == 0) wsum[wave * a.max_seg + s] += vs;
+          const double vs = PACK ? half_sum(v) : wave_sum(v);
+          if (wl == 0) wsum[wrow * a.max_seg + s] += vs;
         }
       }
     }
     __syncthreads();
     double fsum = 0.0;
-    for (int j = 0; j < nwv; ++j) fsum += wflux[j];
+    for (int j = 0; j < rows; ++j) fsum += wflux[row0 + j];
     if (JAC) {
       // total per segment, then repeated cells merged into the first occurrence
-      for (int s = tid; s < ns; s += bd) {
+      for (int s = gt; s < ns; s += gsize) {
         double v = 0.0;
-        for (int j = 0; j < nwv; ++j) v += wsum[j * a.max_seg + s];
+        for (int j = 0; j < rows; ++j) v += wsum[(row0 + j) * a.max_seg + s];
         if (CORR) {
-          v = s_G * s_M * (v / a.n_nodes);
+          v = s_G[grp] * s_M[grp] * (v / a.n_nodes);
           const int c = scell[s];
           if (C.mat && c >= 0) {
             // d q / d w_s of the column water fraction, times M'(q)
@@ -599,31 +621,34 @@ __global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
             const DualD gx = a.len[b0 + s] * cell_density(M, DualD(wk, 1.0),
                                                           M.porous_on && a.shallow[c] != 0);
             const double dwater = gx.v + wk * gx.d;
-            const double dq = s_mass >= 1e-30 ? (dwater - s_q * gx.d) / s_mass : dwater / 1e-30;
-            const double dM = s_M * (C.lin + 2.0 * C.quad * s_q);
-            v += s_G * dM * dq * (fsum / a.n_nodes);
+            const double dq = s_mass[grp] >= 1e-30 ? (dwater - s_q[grp] * gx.d) / s_mass[grp]
+                                                   : dwater / 1e-30;
+            const double dM = s_M[grp] * (C.lin + 2.0 * C.quad * s_q[grp]);
+            v += s_G[grp] * dM * dq * (fsum / a.n_nodes);
           }
-          wsum[s] = v;
+          wsum[row0 * a.max_seg + s] = v;
         } else {
-          wsum[s] = v / a.n_nodes;  // row 0 holds the totals; each s is read then written by one thread
+          // the bin's first row holds the totals; each s is read then written by one thread
+          wsum[row0 * a.max_seg + s] = v / a.n_nodes;
         }
       }
       __syncthreads();
-      for (int s = tid; s < ns; s += bd) {
+      for (int s = gt; s < ns; s += gsize) {
         const int c = scell[s];
         bool first = true;
         for (int t = 0; t < s; ++t)
           if (scell[t] == c) { first = false; break; }
         double v = 0.0;
         if (first) {
-          v = wsum[s];
+          v = wsum[row0 * a.max_seg + s];
           for (int t = s + 1; t < ns; ++t)
-            if (scell[t] == c) v += wsum[t];
+            if (scell[t] == c) v += wsum[row0 * a.max_seg + t];
         }
         a.jac[b0 + s] = v;
       }
     }
-    if (tid == 0) a.flux[r] = CORR ? s_G * s_M * fsum / a.n_nodes : fsum / a.n_nodes;
+    if (live && gt == 0)
+      a.flux[r] = CORR ? s_G[grp] * s_M[grp] * fsum / a.n_nodes : fsum / a.n_nodes;
     __syncthreads();
   }
 }
@@ -633,9 +658,9 @@ __global__ void reset_counters_w(int* q, int n) {
   if (i < n) q[i] = 0;
 }
 
-size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr) {
+size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr, int slots) {
   return ((size_t)w_tab_cols(corr) * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
-         (size_t)max_seg * sizeof(int);
+         (size_t)slots * max_seg * sizeof(int);
 }
 
 int check_w(const char* fn, dp_tables t, const dp_material_config* m, const dp_site_config* site,
@@ -672,21 +697,26 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
              int* qerr) {
   if (p->n_rays == 0) return DP_OK;
   const int bd = es->n >= 256 ? 256 : ((es->n + 63) / 64) * 64;
-  const int nwv = bd / 64;
+  const bool PK = es->n <= 32;   // two bins per 64-lane block
+  const int nwv = PK ? 2 : bd / 64;
   const int max_seg = p->max_segments > 0 ? p->max_segments : 1;
   const bool J = jac != nullptr;
   const bool CR = c->corr.enabled || c->mcorr.enabled;
-  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg, CR);
+  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg, CR, PK ? 2 : 1);
   if (lds > 160 * 1024)
     return dph::fail(DP_E_CAPACITY, "dp_csda_w_flux_jacobian: max_segments " +
                                         std::to_string(max_seg) + " exceeds the LDS budget");
-  auto kern = J ? (CR ? wcsda_kernel<true, true> : wcsda_kernel<true, false>)
-                : (CR ? wcsda_kernel<false, true> : wcsda_kernel<false, false>);
+  auto kern = PK ? (J ? (CR ? wcsda_kernel<true, true, true> : wcsda_kernel<true, false, true>)
+                      : (CR ? wcsda_kernel<false, true, true> : wcsda_kernel<false, false, true>))
+                 : (J ? (CR ? wcsda_kernel<true, true, false> : wcsda_kernel<true, false, false>)
+                      : (CR ? wcsda_kernel<false, true, false> : wcsda_kernel<false, false, false>));
   int occ = 0;
   DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, bd, lds));
   if (occ < 1) occ = 1;
   int grid = c->n_cu * occ;
-  if (grid > p->n_rays) grid = p->n_rays;
+  const int bins_per_block = PK ? 2 : 1;
+  if (grid > (p->n_rays + bins_per_block - 1) / bins_per_block)
+    grid = (p->n_rays + bins_per_block - 1) / bins_per_block;
   WArgs a;
   a.scratch = nullptr;
   if (J) {

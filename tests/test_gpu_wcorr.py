@@ -93,6 +93,18 @@ def test_traced_paths_corrected_many_nodes(ph4, ctx):
     _compare(ph4, _cfg(ph4, True), SiteConfig(600.0, 3e4), paths, w, sh, s, ctx, corr)
 
 
+def test_traced_paths_corrected_two_bins_per_wave(ph4, ctx):
+    """32 samples: two bins per 64-lane block, odd ray count."""
+    paths, n_cells = _traced(ctx, 37, 19)
+    rng = np.random.default_rng(23)
+    w = rng.uniform(0.0, 0.6, n_cells)
+    sh = (rng.random(n_cells) < 0.4).astype(np.uint8)
+    s = dp.csda_energy_quadrature(32, 1.0, 1e4)
+    corr = _set()
+    _say(f"w-model corrected packed: {paths.n_rays} rays, {paths.n_segments} segments")
+    _compare(ph4, _cfg(ph4, True), SiteConfig(600.0, 3e4), paths, w, sh, s, ctx, corr)
+
+
 def test_switching_off_restores_plain_model(ph4, ctx):
     paths, n_cells = _traced(ctx, 16, 5)
     w = np.random.default_rng(2).uniform(0.0, 0.5, n_cells)

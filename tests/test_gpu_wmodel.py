@@ -117,6 +117,24 @@ def test_traced_paths_many_nodes_porous(ph3, ctx):
     _compare(ph3, _cfg(ph3, True), SiteConfig(0.0, 3e4), paths, w, sh, s, ctx)
 
 
+@pytest.mark.parametrize("n_nodes", [32, 17])
+def test_two_bins_per_wave(ph3, ctx, n_nodes):
+    """At most 32 samples put two bins in one 64-lane block; an odd ray count
+    leaves the last block with one live bin."""
+    paths, n_cells = _traced(ctx, 45, 17)
+    rng = np.random.default_rng(31)
+    w = rng.uniform(0.0, 0.6, n_cells)
+    sh = (rng.random(n_cells) < 0.3).astype(np.uint8)
+    s = dp.csda_energy_quadrature(n_nodes, 1.0, 1e4)
+    site = SiteConfig(700.0, 3e4)
+    m = _cfg(ph3, True)
+    _say(f"w-model packed: {paths.n_rays} rays, {n_nodes} nodes")
+    _, j = _compare(ph3, m, site, paths, w, sh, s, ctx)
+    f0 = dp.compute_directional_flux_csda(ph3, sh, m, site, paths, w, s, ctx=ctx)
+    f1, j1 = dp.flux_and_segment_jacobian(ph3, sh, m, site, paths, w, s, ctx=ctx)
+    assert np.array_equal(f0, f1) and np.array_equal(j, j1)
+
+
 def test_forward_only_equals_jacobian_flux_and_deterministic(ph3, ctx):
     paths, n_cells = _traced(ctx, 32, 5)
     w = np.random.default_rng(2).uniform(0.0, 0.5, n_cells)
