@@ -331,8 +331,11 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
   double dw = 1.0, ckf = kf;
   int proc = 0, elem = -1;
   if (scattering && dlog && dlog->coop) {
-    // every lane of this step call takes part in the component evaluation
-    const DelOut o = coop_del_event(*dlog, dlog->medium, mass, k_del, rng, del, err);
+    // every lane of this step call takes part in the component evaluation;
+    // the call is skipped (uniformly) when no lane has a DEL event, where it
+    // would draw nothing and return nothing that is used
+    DelOut o{k_del, 0.0, 0, -1};
+    if (__ballot(del)) o = coop_del_event(*dlog, dlog->medium, mass, k_del, rng, del, err);
     if (del) {
       if (err) return;
       ckf = k_del;
