@@ -51,11 +51,11 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy), from the counter
 # runs summarised in profiles/; `traffic` = this x the units of one launch.
 MEASURED_BYTES_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 286316.0,    # per event
-    "csda_cube_kernel": 16477.0,              # per ray-voxel (all 256 nodes)
-    "wcsda_kernel": 2131.0,                   # per ray-voxel (all 32 nodes)
-    "wcsda_kernel_corrected": 4310.0,         # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 273316.0,               # per event
+    "mc_flat_stochastic_kernel": 385725.0,    # per event
+    "csda_cube_kernel": 16474.0,              # per ray-voxel (all 256 nodes)
+    "wcsda_kernel": 2126.0,                   # per ray-voxel (all 32 nodes)
+    "wcsda_kernel_corrected": 4314.0,         # per ray-voxel (all 32 nodes)
+    "path_mc_kernel": 272931.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
     "trace_cube_kernel": 13.0,                # per ray-voxel (fill launch)
     "ne_normal_kernel": 10.0,                 # per entry of the dense normal matrix
@@ -63,6 +63,35 @@ MEASURED_BYTES_PER_UNIT = {
     "tet_trace_kernel": 62.0,                 # per ray-tet (fill launch)
 }
 FP64_PEAK_TFLOPS = 78.6
+# VALU issue ceiling of an MI355X: 256 CUs x 4 SIMDs, 16 lanes each, so one
+# wave64 VALU instruction per SIMD every 4 cycles at 2.4 GHz (the rate behind
+# the 78.6 TFLOP/s FP64 vector peak: 16 FP64 FMA lanes per SIMD per cycle).
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+# Measured VALU wave instructions per unit of work (SQ_INSTS_VALU of one
+# dispatch of the bench workload / its units, from the counter runs summarised
+# in profiles/).  The kernels are FP64 VALU bound rather than HBM bound; the
+# `valu` object prices the live kernel time against VALU_PEAK_WAVE_INSTS.
+MEASURED_VALU_PER_UNIT = {
+    "mc_flat_stochastic_kernel": 36628.0,     # per event
+    "csda_cube_kernel": 1843.5,               # per ray-voxel (all 256 nodes)
+    "wcsda_kernel": 10120.0,                  # per ray-voxel (all 32 nodes)
+    "wcsda_kernel_corrected": 10495.0,        # per ray-voxel (all 32 nodes)
+    "path_mc_kernel": 14027.0,                # per event
+    "csda_direct_kernel": 39.76,              # per sample
+    "trace_cube_kernel": 5.19,                # per ray-voxel (fill launch)
+    "ray_exit_kernel": 951.5,                 # per ray
+    "tet_trace_kernel": 24.22,                # per ray-tet (fill launch)
+}
+
+
+def valu_roofline(kernel, units, kern_ms):
+    per = MEASURED_VALU_PER_UNIT.get(kernel)
+    if per is None or kern_ms <= 0:
+        return None
+    achieved = per * units / (kern_ms * 1e-3)
+    return {"bound": "valu", "achieved": achieved / 1e9, "peak": VALU_PEAK_WAVE_INSTS / 1e9,
+            "unit": "G wave64 VALU instructions/s", "frac": achieved / VALU_PEAK_WAVE_INSTS,
+            "instructions": per * units}
 
 
 def log(msg):
@@ -149,6 +178,7 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_BYTES_PER_UNIT["csda_cube_kernel"] * units},
+        "valu": valu_roofline("csda_cube_kernel", units, kern_ms),
         "flux_sum": float(flux_h.sum()), "grad_norm": float(grad.norm().item()),
         "_problem": (geo, paths, rho, ne, nw, nq),
     }
@@ -182,7 +212,8 @@ def run_mc(dp, physics, ctx, torch, n_events, steps, warmup):
             "events": n_events, "flux": r["flux"], "sigma": r["sigma"],
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "traffic": MEASURED_BYTES_PER_UNIT["mc_flat_stochastic_kernel"] * n_events}}
+                         "traffic": MEASURED_BYTES_PER_UNIT["mc_flat_stochastic_kernel"] * n_events},
+            "valu": valu_roofline("mc_flat_stochastic_kernel", n_events, kern)}
 
 
 def cpu_mc_baseline(physics, budget_s=15.0):
@@ -342,6 +373,7 @@ def run_trace(dp, L, physics, ctx, torch, steps, warmup):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": MEASURED_BYTES_PER_UNIT["trace_cube_kernel"] * n_seg},
+            "valu": valu_roofline("trace_cube_kernel", n_seg, kern_ms),
             "workload": "trace_cube_path, 128x128x64 voxels of 10 m, 16384 rays (8 det x 2048 "
                         "dirs, elevation 15-75 deg): count + prefix sum + fill",
             "_problem": (geo, sxy, dirs)}
@@ -383,6 +415,7 @@ def run_tetra(dp, L, ctx, steps, warmup):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": MEASURED_BYTES_PER_UNIT["tet_trace_kernel"] * ns},
+            "valu": valu_roofline("tet_trace_kernel", ns, kern),
             "workload": "trace_path(::TetraVolume), jittered Kuhn mesh 48x48x24 cubes of 10 m "
                         "(331776 tets), 4 detectors x 4096 directions",
             "_problem": (pts, tets, vol.neighbors, o, d, surf)}
@@ -451,6 +484,7 @@ def run_dem(dp, L, ctx, torch, steps, warmup, n_dirs=16384):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": MEASURED_BYTES_PER_UNIT["ray_exit_kernel"] * n_dirs},
+            "valu": valu_roofline("ray_exit_kernel", n_dirs, kern_ms),
             "workload": "ray_surface_exit_distance, 257x257 diamond-square DEM (20 m nodes), "
                         "16384 directions, 40 m march + 40 bisections",
             "_problem": (dict(nx=n, ny=n, x0=emap.x0, y0=emap.y0, dx=emap.dx, dy=emap.dy,
@@ -526,6 +560,8 @@ def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32, correction
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": MEASURED_BYTES_PER_UNIT[
                              "wcsda_kernel_corrected" if corrections else "wcsda_kernel"] * units},
+            "valu": valu_roofline("wcsda_kernel_corrected" if corrections else "wcsda_kernel",
+                                  units, kern_ms),
             "_problem": (paths, w, s, m)}
 
 
@@ -660,6 +696,7 @@ def run_direct(dp, L, physics, ctx, torch, steps, warmup, n=1 << 24):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": MEASURED_BYTES_PER_UNIT["csda_direct_kernel"] * n},
+            "valu": valu_roofline("csda_direct_kernel", n, kern_ms),
             "_problem": (rock, air, host)}
 
 
@@ -714,6 +751,7 @@ def run_pathmc(dp, physics, ctx, n_rays=256, n_events=10000):
                          (kern * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": (16.0 * ev + 12.0 * paths.n_segments) / (kern * 1e-3) / 1e9 /
                          HBM_PEAK_GBS, "traffic": MEASURED_BYTES_PER_UNIT["path_mc_kernel"] * ev},
+            "valu": valu_roofline("path_mc_kernel", ev, kern),
             "_problem": (paths, rho, s, m, site)}
 
 
@@ -882,6 +920,7 @@ def main():
                                "scattering + DEL/EHS on",
                    "events": mc["events"]},
         "roofline": mc["roofline"],
+        "valu": mc["valu"],
         "cpu_baseline": cpu,
         "kernel_ms": mc["kernel_ms"],
         "mc": {k: mc[k] for k in ("events", "flux", "sigma", "kernel_ms", "ms_per_step")},
@@ -890,7 +929,7 @@ def main():
                                 "16384 rays (8 det x 2048 dirs), 256 nodes 1-1e4 GeV",
                     "rays": adj["rays"], "segments": adj["segments"], "nodes": adj["nodes"],
                     "ms_per_step": adj["ms_per_step"], "kernel_ms": adj["kernel_ms"],
-                    "roofline": adj["roofline"], "cpu_baseline": cpu_adj},
+                    "roofline": adj["roofline"], "valu": adj["valu"], "cpu_baseline": cpu_adj},
         "wmodel": dict(wm, workload="water-fraction CSDA flux + sparse Jacobian, 64x64x32 "
                                     "voxels of 10 m, w U[0, 0.3], 4 detectors x 1024 "
                                     "directions, 32 nodes 1-1e4 GeV"),

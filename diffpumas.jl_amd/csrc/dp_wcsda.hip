@@ -345,8 +345,11 @@ __device__ __forceinline__ double half_sum(double v) {
 // PACK (at most 32 energy nodes): a 64-lane block carries two bins, one per
 // half-wave; each half sums its nodes with half_sum, which equals the
 // full-wave sum of the unpacked layout (whose upper half only adds zeros).
+// The corrected Jacobian instantiation asks for 2 waves per SIMD (a register
+// budget of 256 per lane; unbounded it takes 258 and runs 1 wave/SIMD).  The
+// others keep the default bound, which measured faster for the plain Jacobian.
 template <bool JAC, bool CORR, bool PACK>
-__global__ __launch_bounds__(256) void wcsda_kernel(WArgs a) {
+__global__ __launch_bounds__(256, (JAC && CORR) ? 2 : 1) void wcsda_kernel(WArgs a) {
   extern __shared__ double lds[];
   const int K = a.K;
   const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
