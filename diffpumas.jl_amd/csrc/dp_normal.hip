@@ -13,7 +13,7 @@
 //   2. a stable radix sort of (active column, entry) pairs (dp_sort.hpp: no
 //      memset, so the whole assembly can be captured in a graph) lists every
 //      column's entries in ascending row order;
-//   3. one 64-lane block per (output column j, tile of 4096 rows i) walks the
+//   3. one 64-lane block per (output column j, tile of 1024 rows i) walks the
 //      rows of column j in that order and, for each, adds the row's products
 //      into an LDS accumulator (entries of one row have distinct columns, so
 //      the lanes never collide); the block then writes its tile of column j
@@ -32,7 +32,10 @@
 
 namespace {
 
-constexpr int NE_TILE = 4096;   // rows of the output column per block (32 KB of LDS)
+// Rows of the output column per block (8 KB of LDS).  Measured on a 12 869-column
+// system: 0.91 ms per assembly with 4096-row tiles, 0.81 ms with 2048, 0.78 ms
+// with 1024 (more resident blocks per CU outweigh the repeated column walks).
+constexpr int NE_TILE = 1024;
 
 __global__ void ne_fill_index(int n_cols, int32_t* __restrict__ act_index) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;

@@ -3,8 +3,8 @@ gauss_newton_reconstruct, src/Tomography_inverse.jl:489-494) against the CPU
 oracle, bit for bit: both sum every entry over rows in ascending order.
 
 Cases: the w-model Jacobian of traced paths through make_csda_operator with a
-free mask and Poisson-like weights; a random Jacobian with more than one
-4096-row tile per output column and a column crossed by every row; the
+free mask and Poisson-like weights; a random Jacobian with several row
+tiles per output column and a column crossed by every row; the
 device-pointer entry point; zero and non-finite entries; argument errors.
 """
 import ctypes as C
@@ -72,11 +72,11 @@ def test_multi_tile_and_long_column(ctx):
     rng = np.random.default_rng(6)
     W = rng.uniform(0.2, 3.0, 4000)
     r = rng.normal(0.0, 1.0, 4000)
-    _say(f"normal eqs (random): {J.shape}, {J.nnz} entries, 6000 active (2 tiles)")
+    _say(f"normal eqs (random): {J.shape}, {J.nnz} entries, 6000 active (several row tiles)")
     N, g = dp.normal_equations(J, r, W, None, ctx=ctx)
     No, go = O.normal_equations(J, r, W, None)
     assert np.array_equal(N, No) and np.array_equal(g, go)
-    act = np.flatnonzero(rng.random(6000) < 0.8)     # 4800 active: 2 tiles
+    act = np.flatnonzero(rng.random(6000) < 0.8)     # 4800 active: several row tiles
     N, g = dp.normal_equations(J, r, None, act, ctx=ctx)
     No, go = O.normal_equations(J, r, None, act)
     assert np.array_equal(N, No) and np.array_equal(g, go)
