@@ -51,14 +51,14 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy), from the counter
 # runs summarised in profiles/; `traffic` = this x the units of one launch.
 MEASURED_BYTES_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 385672.0,    # per event
+    "mc_flat_stochastic_kernel": 385764.0,    # per event
     "csda_cube_kernel": 17299.0,              # per ray-voxel (all 256 nodes)
     "wcsda_kernel": 2127.0,                   # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 4314.0,         # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 272932.0,               # per event
+    "path_mc_kernel": 272984.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
     "trace_cube_kernel": 13.0,                # per ray-voxel (fill launch)
-    "ne_normal_kernel": 10.0,                 # per entry of the dense normal matrix
+    "ne_normal_kernel": 11.0,                 # per entry of the dense normal matrix
     "ray_exit_kernel": 112.0,                 # per ray
     "tet_trace_kernel": 62.0,                 # per ray-tet (fill launch)
 }
@@ -72,11 +72,11 @@ VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 # in profiles/).  The kernels are FP64 VALU bound rather than HBM bound; the
 # `valu` object prices the live kernel time against VALU_PEAK_WAVE_INSTS.
 MEASURED_VALU_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 36632.0,     # per event
+    "mc_flat_stochastic_kernel": 36627.0,     # per event
     "csda_cube_kernel": 1868.5,               # per ray-voxel (all 256 nodes)
     "wcsda_kernel": 10120.0,                  # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 10495.0,        # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 14026.0,                # per event
+    "path_mc_kernel": 14027.0,                # per event
     "csda_direct_kernel": 39.99,              # per sample
     "trace_cube_kernel": 5.19,                # per ray-voxel (fill launch)
     "ray_exit_kernel": 951.5,                 # per ray

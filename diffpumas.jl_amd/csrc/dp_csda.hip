@@ -20,6 +20,7 @@
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
 #include "dp_sort.hpp"
+#include <climits>
 
 using namespace dpk;
 
@@ -307,40 +308,94 @@ __global__ void vjp_zero(int n, double* __restrict__ g) {
   if (i < n) g[i] = 0.0;
 }
 
-__global__ void vjp_keys(int n_rays, int n_seg, const int32_t* __restrict__ row_ptr,
-                         const int32_t* __restrict__ cell, int32_t* __restrict__ keys,
-                         int32_t* __restrict__ vals, int32_t* __restrict__ ray_of) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// One 64-lane block per ray: the ray's segments are written by consecutive
+// lanes (coalesced).
+__global__ __launch_bounds__(64) void vjp_keys(int n_rays, int n_seg,
+                                               const int32_t* __restrict__ row_ptr,
+                                               const int32_t* __restrict__ cell,
+                                               int32_t* __restrict__ keys,
+                                               int32_t* __restrict__ vals,
+                                               int32_t* __restrict__ ray_of) {
+  const int r = blockIdx.x;
   if (r >= n_rays) return;
-  for (int s = row_ptr[r]; s < row_ptr[r + 1]; ++s) {
+  const int s1 = row_ptr[r + 1];
+  for (int s = row_ptr[r] + (int)threadIdx.x; s < s1; s += 64) {
     keys[s] = cell[s];
     vals[s] = s;
     ray_of[s] = r;
   }
 }
 
-__device__ __forceinline__ int lower_bound_i(const int32_t* a, int n, int x) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (a[mid] < x) lo = mid + 1; else hi = mid;
+// Gradient sums over the cell-sorted segment list (keys = cells, ascending)
+// in a fixed shape.  vjp_chunk: one block per chunk of VJP_CHUNK positions
+// gathers the terms fbar[ray] * jac[segment] in parallel and scans them by
+// key (segmented Hillis-Steele in LDS).  A run of one cell that lies inside
+// the chunk is written to grad; the first run's partial goes to head[chunk]
+// and the last run's to tail[chunk].  vjp_join: for each run that crosses a
+// chunk boundary, the chunk where it starts adds tail and the following
+// heads in chunk order.  Cells crossed by no segment keep the zero written by
+// vjp_zero.  Cells next to a detector are crossed by thousands of rays: the
+// terms of such a run are summed by a block, not walked by one lane.
+constexpr int VJP_CHUNK = 256;
+
+__global__ __launch_bounds__(VJP_CHUNK) void vjp_chunk(int n_seg, int n_cells,
+                                                       const int32_t* __restrict__ skeys,
+                                                       const int32_t* __restrict__ svals,
+                                                       const int32_t* __restrict__ ray_of,
+                                                       const double* __restrict__ jac,
+                                                       const double* __restrict__ fbar,
+                                                       double* __restrict__ grad,
+                                                       double* __restrict__ head,
+                                                       double* __restrict__ tail) {
+  __shared__ double sv[VJP_CHUNK];
+  __shared__ int sk[VJP_CHUNK];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int p = b * VJP_CHUNK + t;
+  const bool live = p < n_seg;
+  const int key = live ? skeys[p] : INT_MAX;
+  double v = 0.0;
+  if (live) {
+    const int sg = svals[p];
+    v = fbar[ray_of[sg]] * jac[sg];
   }
-  return lo;
+  sk[t] = key;
+  sv[t] = v;
+  __syncthreads();
+  for (int d = 1; d < VJP_CHUNK; d <<= 1) {
+    const bool same = t >= d && sk[t - d] == key;
+    const double add = same ? sv[t - d] : 0.0;
+    __syncthreads();
+    if (same) sv[t] += add;
+    __syncthreads();
+  }
+  if (!live) return;
+  const int last = min(VJP_CHUNK, n_seg - b * VJP_CHUNK) - 1;
+  if (t != last && sk[t + 1] == key) return;   // not the last position of its run here
+  const double run = sv[t];
+  const bool from_prev = key == sk[0] && b > 0 && skeys[b * VJP_CHUNK - 1] == key;
+  const bool to_next = t == last && p + 1 < n_seg && skeys[p + 1] == key;
+  if (key == sk[0]) head[b] = run;
+  if (t == last) tail[b] = run;
+  if (!from_prev && !to_next && key >= 0 && key < n_cells) grad[key] = run;
 }
 
-__global__ void vjp_gather(int n_cells, int n_seg, const int32_t* __restrict__ skeys,
-                           const int32_t* __restrict__ svals, const int32_t* __restrict__ ray_of,
-                           const double* __restrict__ jac, const double* __restrict__ fbar,
-                           double* __restrict__ grad) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cells) return;
-  int p = lower_bound_i(skeys, n_seg, c);
-  double acc = 0.0;
-  for (; p < n_seg && skeys[p] == c; ++p) {
-    const int v = svals[p];
-    acc += fbar[ray_of[v]] * jac[v];
+__global__ void vjp_join(int n_seg, int n_chunks, int n_cells, const int32_t* __restrict__ skeys,
+                         const double* __restrict__ head, const double* __restrict__ tail,
+                         double* __restrict__ grad) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_chunks) return;
+  const int start = b * VJP_CHUNK;
+  const int last = min(start + VJP_CHUNK, n_seg) - 1;
+  const int c = skeys[last];
+  if (last + 1 >= n_seg || skeys[last + 1] != c) return;   // the last run ends in this chunk
+  if (skeys[start] == c && b > 0 && skeys[start - 1] == c) return;   // it started earlier
+  double sum = tail[b];
+  for (int j = b + 1; j < n_chunks; ++j) {
+    sum += head[j];
+    const int jl = min((j + 1) * VJP_CHUNK, n_seg) - 1;
+    if (skeys[jl] != c || jl + 1 >= n_seg || skeys[jl + 1] != c) break;   // the run ends in chunk j
   }
-  grad[c] = acc;
+  if (c >= 0 && c < n_cells) grad[c] = sum;
 }
 
 // The remaining-rock derivatives all belong to cell 0 (the remainder is
@@ -546,19 +601,29 @@ int dp_csda_cube_vjp_dev(dp_ctx c, const dp_paths* p, const double* jac, const d
                o_h = take(4 * dps::sort_hist_ints(ne));
   const int n_part = (nr + VJP_REM_THREADS * VJP_REM_PER - 1) / (VJP_REM_THREADS * VJP_REM_PER);
   const size_t o_part = take(8 * (size_t)n_part);
+  const int n_chunks = (ns + VJP_CHUNK - 1) / VJP_CHUNK;
+  const size_t o_hd = take(8 * (size_t)n_chunks + 8), o_tl = take(8 * (size_t)n_chunks + 8);
   int rc = dph::ensure(c->scratch2, o);
   if (rc) return rc;
   char* b = (char*)c->scratch2.ptr;
   int32_t *k = (int32_t*)(b + o_k), *v = (int32_t*)(b + o_v), *ro = (int32_t*)(b + o_ro);
-  hipLaunchKernelGGL(vjp_keys, dim3((nr + 255) / 256), dim3(256), 0, c->stream, nr, ns,
-                     p->row_ptr, p->cell, k, v, ro);
+  hipLaunchKernelGGL(vjp_keys, dim3(nr), dim3(64), 0, c->stream, nr, ns, p->row_ptr, p->cell, k, v,
+                     ro);
   DP_HIP(hipGetLastError());
   const int32_t *sk = k, *sv = v;
   if (ns > 0)
     dps::radix_sort_pairs(k, v, ns, bits, (int32_t*)(b + o_p0), (int32_t*)(b + o_p1),
                           (int32_t*)(b + o_h), c->stream, &sk, &sv);
-  hipLaunchKernelGGL(vjp_gather, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
-                     ns, sk, sv, ro, jac, fbar, grad);
+  hipLaunchKernelGGL(vjp_zero, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
+                     grad);
+  if (ns > 0) {
+    double* head = (double*)(b + o_hd);
+    double* tail = (double*)(b + o_tl);
+    hipLaunchKernelGGL(vjp_chunk, dim3(n_chunks), dim3(VJP_CHUNK), 0, c->stream, ns, n_cells, sk,
+                       sv, ro, jac, fbar, grad, head, tail);
+    hipLaunchKernelGGL(vjp_join, dim3((n_chunks + 255) / 256), dim3(256), 0, c->stream, ns,
+                       n_chunks, n_cells, sk, head, tail, grad);
+  }
   if (jac_rem) {
     double* part = (double*)(b + o_part);
     hipLaunchKernelGGL(vjp_rem_partial, dim3(n_part), dim3(VJP_REM_THREADS), 0, c->stream, nr,
