@@ -7,7 +7,7 @@
 // column j in ascending order (Gustavson): C[i, j] += J[r, i] * (W[r] J[r, j]).
 // The build does the same on the device:
 //
-//   1. one lane per row tags each entry with its active column (sentinel
+//   1. one 64-lane block per row tags each entry with its active column (sentinel
 //      n_active for dropped entries: zero, non-finite, inactive column) and
 //      records the entry's row;
 //   2. a stable radix sort of (active column, entry) pairs (dp_sort.hpp: no
@@ -19,7 +19,8 @@
 //      the lanes never collide); the block then writes its tile of column j
 //      of the column-major result, zeros included (row bounds are fetched 64
 //      rows at a time and the next row's entries are loaded ahead);
-//   4. one lane per active column sums the gradient in the same order.
+//   4. one lane per active column sums the gradient in the same order
+//      (operands of 8 entries loaded ahead of their in-order additions).
 //
 // Every output is accumulated in the reference's order with separate
 // multiply and add (no FP contraction), so results are deterministic and do
@@ -53,13 +54,19 @@ __global__ void ne_scatter_index(int n_active, int n_cols, const int32_t* __rest
 
 __device__ __forceinline__ bool ne_keep(double v) { return isfinite(v) && v != 0.0; }
 
-__global__ void ne_keys(int n_rows, int n_cols, int n_active, const int32_t* __restrict__ row_ptr,
-                        const int32_t* __restrict__ col, const double* __restrict__ val,
-                        const int32_t* __restrict__ act_index, int32_t* __restrict__ key,
-                        int32_t* __restrict__ ent, int32_t* __restrict__ row_of) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// One 64-lane block per row: the row's entries are handled by consecutive
+// lanes (coalesced reads and writes).
+__global__ __launch_bounds__(64) void ne_keys(int n_rows, int n_cols, int n_active,
+                                              const int32_t* __restrict__ row_ptr,
+                                              const int32_t* __restrict__ col,
+                                              const double* __restrict__ val,
+                                              const int32_t* __restrict__ act_index,
+                                              int32_t* __restrict__ key, int32_t* __restrict__ ent,
+                                              int32_t* __restrict__ row_of) {
+  const int r = blockIdx.x;
   if (r >= n_rows) return;
-  for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+  const int e1 = row_ptr[r + 1];
+  for (int e = row_ptr[r] + (int)threadIdx.x; e < e1; e += 64) {
     const int c = col[e];
     int a = (c >= 0 && c < n_cols) ? act_index[c] : -1;
     if (a < 0 || !ne_keep(val[e])) a = n_active;
@@ -154,8 +161,28 @@ __global__ void ne_grad_kernel(int n_active, const double* __restrict__ val,
                                const int32_t* __restrict__ col_ptr, double* __restrict__ grad) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_active) return;
+  const int end = col_ptr[j + 1];
   double g = 0.0;
-  for (int s = col_ptr[j]; s < col_ptr[j + 1]; ++s) {
+  int s = col_ptr[j];
+  // Operands of 8 entries are loaded before they are added, so the dependent
+  // loads of a long column overlap; the additions keep ascending-row order.
+  constexpr int B = 8;
+  for (; s + B <= end; s += B) {
+    int e[B], r[B];
+    double v[B], wr[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) e[q] = sent[s + q];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      r[q] = row_of[e[q]];
+      v[q] = val[e[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) wr[q] = weights ? weights[r[q]] * residual[r[q]] : residual[r[q]];
+#pragma unroll
+    for (int q = 0; q < B; ++q) g += v[q] * wr[q];
+  }
+  for (; s < end; ++s) {
     const int e = sent[s];
     const int r = row_of[e];
     const double wr = weights ? weights[r] * residual[r] : residual[r];   // (W .* r)[r]
@@ -202,8 +229,8 @@ int run_normal(dp_ctx c, const dp_csr* J, const double* weights, const double* r
   hipLaunchKernelGGL(ne_scatter_index, dim3((na + 255) / 256), dim3(256), 0, st, na, nc, active,
                      ai);
   if (nr > 0)
-    hipLaunchKernelGGL(ne_keys, dim3((nr + 63) / 64), dim3(64), 0, st, nr, nc, na, J->row_ptr,
-                       J->col, J->val, ai, k, e, ro);
+    hipLaunchKernelGGL(ne_keys, dim3(nr), dim3(64), 0, st, nr, nc, na, J->row_ptr, J->col, J->val,
+                       ai, k, e, ro);
   DP_HIP(hipGetLastError());
   if (nnz > 0)
     dps::radix_sort_pairs(k, e, nnz, bits, (int32_t*)(b + o_p0), (int32_t*)(b + o_p1),
