@@ -402,12 +402,43 @@ __global__ void bucket_offsets(int* cnt) {
     acc += c;
   }
 }
+// Each block takes rounds of 256 x SCATTER_ITEMS events: ranks inside the
+// round come from an LDS histogram, and one global atomic per bucket and
+// round reserves the round's range in that bucket (the order inside a bucket
+// is arbitrary; results do not depend on it).
+constexpr int SCATTER_ITEMS = 8;
 __global__ __launch_bounds__(256) void bucket_scatter(uint64_t seed, int64_t n, int* cur,
                                                       uint32_t* order) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-    const int pos = atomicAdd(&cur[energy_bucket(seed, e)], 1);
-    if (pos >= 0 && pos < n) order[pos] = (uint32_t)e;
+  __shared__ int h[MC_BUCKETS];
+  __shared__ int base[MC_BUCKETS];
+  const int tid = threadIdx.x;
+  const int64_t per = (int64_t)256 * SCATTER_ITEMS;
+  for (int64_t b0 = (int64_t)blockIdx.x * per; b0 < n; b0 += (int64_t)gridDim.x * per) {
+    for (int i = tid; i < MC_BUCKETS; i += 256) h[i] = 0;
+    __syncthreads();
+    int bk[SCATTER_ITEMS], rk[SCATTER_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SCATTER_ITEMS; ++k) {
+      const int64_t e = b0 + (int64_t)k * 256 + tid;
+      bk[k] = 0;
+      rk[k] = -1;
+      if (e < n) {
+        bk[k] = energy_bucket(seed, e);
+        rk[k] = atomicAdd(&h[bk[k]], 1);
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < MC_BUCKETS; i += 256) base[i] = h[i] ? atomicAdd(&cur[i], h[i]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCATTER_ITEMS; ++k) {
+      const int64_t e = b0 + (int64_t)k * 256 + tid;
+      if (e < n && rk[k] >= 0) {
+        const int pos = base[bk[k]] + rk[k];
+        if (pos >= 0 && pos < n) order[pos] = (uint32_t)e;
+      }
+    }
+    __syncthreads();   // h and base are reused by the next round
   }
 }
 
