@@ -855,7 +855,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(rays and events shard trivially; see DESIGN.md).", file=sys.stderr)
+              "(events, rays and bins are independent and would shard by range, one "
+              "process per GPU).", file=sys.stderr)
         sys.exit(2)
     import torch
     import diffpumas_hip as dp

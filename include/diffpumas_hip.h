@@ -4,7 +4,8 @@
  *
  * The reference (grinisrit/diffpumas.jl) is pure Julia with no FFI seam; each
  * entry point below replaces the Julia function named in its comment and is
- * what a Julia `ccall` binding binds (see INTEGRATION.md for the stubs).
+ * what a Julia `ccall` binding binds (one `ccall` per entry point, with the
+ * argument order and types declared here).
  *
  * Conventions
  *   - Every function returns int: 0 = DP_OK, negative = error class.  The
