@@ -12,6 +12,10 @@ int fail(int code, const std::string& msg) {
 }
 int ensure(DevBuf& b, size_t bytes) {
   if (bytes <= b.bytes && b.ptr) return DP_OK;
+  if (b.frozen && *b.frozen)
+    return fail(DP_E_ARGUMENT, "a device buffer would have to grow during a graph capture: call "
+                               "the entry point once with the same sizes before capturing");
+  if (b.generation) ++*b.generation;
   if (b.ptr) {
     hipError_t e = hipFree(b.ptr);
     b.ptr = nullptr;
@@ -57,6 +61,20 @@ int dp_create(dp_ctx* out, int device) {
     return dph::fail(DP_E_HIP, "dp_create: stream/event creation failed");
   }
   c->stream = c->own_stream;
+  // scratch, scratch2 and counter are what _dev entry points (and so captured
+  // graphs) use; staging only serves the host entry points
+  for (dph::DevBuf* b : {&c->scratch, &c->scratch2, &c->counter}) b->generation = &c->generation;
+  for (dph::DevBuf* b : {&c->scratch, &c->scratch2, &c->counter, &c->staging}) b->frozen = &c->capturing;
+  // the counter buffer exists from the start with a zero sticky error word
+  if (dph::ensure(c->counter, 256) != DP_OK || hipMemset(c->counter.ptr, 0, 256) != hipSuccess) {
+    const std::string msg = dp_last_error();
+    dph::release(c->counter);
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return dph::fail(DP_E_HIP, "dp_create: counter buffer: " + msg);
+  }
   *out = c;
   return DP_OK;
 }
@@ -88,10 +106,27 @@ int dp_synchronize(dp_ctx c) {
   return DP_OK;
 }
 
+int dp_check_errors(dp_ctx c) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_check_errors: null context");
+  if (c->capturing) return dph::fail(DP_E_ARGUMENT, "dp_check_errors: a capture is open");
+  DP_HIP(hipSetDevice(c->device));
+  DP_HIP(hipStreamSynchronize(c->stream));
+  int* w = (int*)c->counter.ptr + dph::COUNTER_STICKY;
+  int bits = 0;
+  DP_HIP(hipMemcpy(&bits, w, sizeof(int), hipMemcpyDeviceToHost));
+  if (!bits) return DP_OK;
+  DP_HIP(hipMemset(w, 0, sizeof(int)));
+  if (bits & dph::KERR_CELL)
+    return dph::fail(DP_E_ARGUMENT, "a _dev call met a cell index outside the field");
+  return dph::fail(DP_E_CAPACITY, "a _dev call met a path longer than its max_segments");
+}
+
 // ------------------------------------------------------------ graph capture
 struct dp_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  dp_ctx ctx = nullptr;     // context whose buffers the graph's nodes point into
+  uint64_t generation = 0;  // that context's buffer generation at capture
 };
 
 int dp_capture_begin(dp_ctx c) {
@@ -126,6 +161,8 @@ int dp_capture_end(dp_ctx c, dp_graph* out) {
   dp_graph h = new dp_graph_s;
   h->graph = g;
   h->exec = x;
+  h->ctx = c;
+  h->generation = c->generation;
   *out = h;
   return DP_OK;
 }
@@ -133,6 +170,10 @@ int dp_capture_end(dp_ctx c, dp_graph* out) {
 int dp_graph_launch(dp_ctx c, dp_graph g) {
   if (!c || !g) return dph::fail(DP_E_ARGUMENT, "dp_graph_launch: null argument");
   if (c->capturing) return dph::fail(DP_E_ARGUMENT, "dp_graph_launch: a capture is open");
+  if (g->ctx != c) return dph::fail(DP_E_ARGUMENT, "dp_graph_launch: the graph was captured on another context");
+  if (g->generation != c->generation)
+    return dph::fail(DP_E_ARGUMENT, "dp_graph_launch: a later call reallocated the context's device "
+                                    "buffers that the graph uses; capture it again");
   DP_HIP(hipSetDevice(c->device));
   DP_HIP(hipGraphLaunch(g->exec, c->stream));
   return DP_OK;

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
     const int ns = a.row_ptr[r + 1] - b0;
     if (ns < 0 || ns > a.max_seg) {
       if (tid == 0) {
-        atomicOr(a.err, 1);
+        atomicOr(a.err, dph::KERR_CAPACITY);
         a.flux[r] = __builtin_nan("");
         if (a.jac_rem) a.jac_rem[r] = __builtin_nan("");
       }
@@ -297,9 +297,13 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
   }
 }
 
-__global__ void reset_counters(int* q, int n) {
-  int i = threadIdx.x;
-  if (i < n) q[i] = 0;
+// Clears the work queue and, for the host entry point, its error word (the
+// _dev entry points accumulate errors in the context's sticky word).
+__global__ void reset_counters(int* queue, int* err, int reset_err) {
+  if (threadIdx.x == 0) {
+    *queue = 0;
+    if (reset_err) *err = 0;
+  }
 }
 
 // ------------------------------------------------------------------ VJP
@@ -456,7 +460,8 @@ size_t lds_bytes(int K, int nw, int max_seg) {
 }
 
 int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
-                const double* dens, double* flux, double* jac, double* jac_rem, int* qerr) {
+                const double* dens, double* flux, double* jac, double* jac_rem, int* queue,
+                int* err, bool reset_err) {
   if (p->n_rays == 0) return DP_OK;
   const int bd = nd->n_nodes >= 256 ? 256 : ((nd->n_nodes + 63) / 64) * 64;
   const int nw = bd / 64;
@@ -484,8 +489,8 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
   a.n_nodes = nd->n_nodes; a.ne = nd->node_energy; a.nw = nd->node_weight; a.nq = nd->node_charge;
   a.dens = dens; a.flux = flux; a.jac = jac; a.jac_rem = jac_rem;
   a.scratch = (double*)c->scratch.ptr;
-  a.queue = qerr; a.err = qerr + 1;
-  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, qerr, 2);
+  a.queue = queue; a.err = err;
+  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, queue, err, reset_err ? 1 : 0);
   dph::record(c, c->ev0);
   hipLaunchKernelGGL(csda_cube_kernel, dim3(grid), dim3(bd), lds, c->stream, a);
   dph::record(c, c->ev1);
@@ -504,9 +509,9 @@ int dp_csda_cube_dev(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nod
   int rc = check_paths(p, nd, n_cells, t);
   if (rc) return rc;
   DP_HIP(hipSetDevice(c->device));
-  rc = dph::ensure(c->counter, 256);
-  if (rc) return rc;
-  return launch_csda(c, t, p, nd, dens, flux, jac, jac_rem, (int*)c->counter.ptr);
+  int* q = (int*)c->counter.ptr;
+  return launch_csda(c, t, p, nd, dens, flux, jac, jac_rem, q + dph::COUNTER_QUEUE,
+                     q + dph::COUNTER_STICKY, false);
 }
 
 int dp_csda_cube(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
@@ -567,7 +572,8 @@ int dp_csda_cube(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* 
   dn.node_energy = (double*)(b + o_ne); dn.node_weight = (double*)(b + o_nw); dn.node_charge = (double*)(b + o_nq);
   int* qerr = (int*)(b + o_q);
   rc = launch_csda(c, t, &dp, &dn, (double*)(b + o_den), (double*)(b + o_flux),
-                   jac ? (double*)(b + o_jac) : nullptr, jac_rem ? (double*)(b + o_jr) : nullptr, qerr);
+                   jac ? (double*)(b + o_jac) : nullptr, jac_rem ? (double*)(b + o_jr) : nullptr, qerr,
+                   qerr + 1, true);
   if (rc) return rc;
   int herr[2] = {0, 0};
   DP_HIP(hipMemcpyAsync(flux, b + o_flux, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));

@@ -19,10 +19,15 @@ int fail(int code, const std::string& msg);
       return dph::fail(DP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Growable device scratch buffer owned by a context.
+// Growable device scratch buffer owned by a context.  Every reallocation
+// bumps the owning context's generation (captured graphs hold raw pointers
+// into these buffers and are refused once any of them moved), and a buffer
+// cannot grow while its context captures a graph.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  uint64_t* generation = nullptr;  // the owning context's counter
+  const bool* frozen = nullptr;    // the owning context's capture flag
 };
 int ensure(DevBuf& b, size_t bytes);
 void release(DevBuf& b);
@@ -69,11 +74,12 @@ struct dp_ctx_s {
   float last_ms = 0.f;
   dph::DevBuf scratch;       // kernel scratch (partials, queues)
   dph::DevBuf scratch2;      // second scratch (sort buffers)
-  dph::DevBuf counter;       // small: work-queue counters
+  dph::DevBuf counter;       // small: work-queue counters and the sticky error word
   dph::DevBuf staging;       // host-API staging of inputs/outputs
   dp_csda_correction corr{};            // w-model corrections (off)
   dp_csda_material_correction mcorr{};
   bool capturing = false;               // between dp_capture_begin and dp_capture_end
+  uint64_t generation = 0;              // bumped by every scratch reallocation
 };
 
 namespace dph {
@@ -84,6 +90,16 @@ inline int pow2_for(int n) {
   return n - 2 >= 1 ? p : 0;
 }
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+// Int slots of the context's counter buffer: [COUNTER_QUEUE] the work queue
+// of the persistent kernels, [COUNTER_STICKY] the error bits the _dev entry
+// points accumulate until dp_check_errors reads and clears them.
+constexpr int COUNTER_QUEUE = 0;
+constexpr int COUNTER_STICKY = 16;
+// Error bits of the deterministic kernels (both the per-call and the sticky
+// word): a path longer than the scratch sized for it, a cell index outside
+// the field.
+constexpr int KERR_CAPACITY = 1;
+constexpr int KERR_CELL = 2;
 // Timing event on the context's stream; not recorded while the stream is
 // being captured into a graph.
 inline void record(dp_ctx c, hipEvent_t e) {

@@ -37,7 +37,7 @@ namespace {
 
 constexpr double MAX_REL_GAIN = 0.02;  // CSDA_MAX_RELATIVE_GAIN, src/Tomography.jl:90
 constexpr double MAX_STEP_M = 60.0;    // CSDA_MAX_STEP_M, src/Tomography.jl:91
-constexpr int ERR_CAP = 1, ERR_CELL = 2;
+constexpr int ERR_CAP = dph::KERR_CAPACITY, ERR_CELL = dph::KERR_CELL;
 
 // _csda_min / _csda_max (src/Tomography.jl:421-434): branch on primal values,
 // the left operand wins ties.
@@ -656,9 +656,13 @@ __global__ __launch_bounds__(256, (JAC && CORR) ? 2 : 1) void wcsda_kernel(WArgs
   }
 }
 
-__global__ void reset_counters_w(int* q, int n) {
-  int i = threadIdx.x;
-  if (i < n) q[i] = 0;
+// Clears the work queue and, for the host entry point, its error word (the
+// _dev entry points accumulate errors in the context's sticky word).
+__global__ void reset_counters_w(int* queue, int* err, int reset_err) {
+  if (threadIdx.x == 0) {
+    *queue = 0;
+    if (reset_err) *err = 0;
+  }
 }
 
 size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr, int slots) {
@@ -697,7 +701,7 @@ bool same_grid(dp_tables t, int a, int b) {
 int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_config* site,
              const dp_directional_paths* p, int32_t n_cells, const double* w,
              const uint8_t* shallow, const dp_energy_samples* es, double* flux, double* jac,
-             int* qerr) {
+             int* queue, int* err, bool reset_err) {
   if (p->n_rays == 0) return DP_OK;
   const int bd = es->n >= 256 ? 256 : ((es->n + 63) / 64) * 64;
   const bool PK = es->n <= 32;   // two bins per 64-lane block
@@ -749,8 +753,8 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
   a.w = w; a.shallow = shallow;
   a.n_nodes = es->n; a.ne = es->energy; a.nw = es->weight; a.nq = es->charge;
   a.flux = flux; a.jac = jac;
-  a.queue = qerr; a.err = qerr + 1;
-  hipLaunchKernelGGL(reset_counters_w, dim3(1), dim3(64), 0, c->stream, qerr, 2);
+  a.queue = queue; a.err = err;
+  hipLaunchKernelGGL(reset_counters_w, dim3(1), dim3(64), 0, c->stream, queue, err, reset_err ? 1 : 0);
   dph::record(c, c->ev0);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(bd), lds, c->stream, a);
   dph::record(c, c->ev1);
@@ -784,9 +788,9 @@ int dp_csda_w_flux_jacobian_dev(dp_ctx c, dp_tables t, const dp_material_config*
   int rc = check_w("dp_csda_w_flux_jacobian_dev", t, m, site, p, n_cells, w, shallow, es, flux);
   if (rc) return rc;
   DP_HIP(hipSetDevice(c->device));
-  rc = dph::ensure(c->counter, 256);
-  if (rc) return rc;
-  return launch_w(c, t, m, site, p, n_cells, w, shallow, es, flux, jac, (int*)c->counter.ptr);
+  int* q = (int*)c->counter.ptr;
+  return launch_w(c, t, m, site, p, n_cells, w, shallow, es, flux, jac, q + dph::COUNTER_QUEUE,
+                  q + dph::COUNTER_STICKY, false);
 }
 
 int dp_csda_w_flux_jacobian(dp_ctx c, dp_tables t, const dp_material_config* m,
@@ -852,7 +856,7 @@ int dp_csda_w_flux_jacobian(dp_ctx c, dp_tables t, const dp_material_config* m,
   de.energy = (double*)(b + o_ne); de.weight = (double*)(b + o_nw); de.charge = (double*)(b + o_nq);
   int* qerr = (int*)(b + o_q);
   rc = launch_w(c, t, m, site, &dp, n_cells, (double*)(b + o_w), (uint8_t*)(b + o_sh), &de,
-                (double*)(b + o_flux), jac ? (double*)(b + o_jac) : nullptr, qerr);
+                (double*)(b + o_flux), jac ? (double*)(b + o_jac) : nullptr, qerr, qerr + 1, true);
   if (rc) return rc;
   int herr[2] = {0, 0};
   DP_HIP(hipMemcpyAsync(flux, b + o_flux, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));

@@ -179,6 +179,7 @@ _SIGS = {
     "dp_last_error": (C.c_char_p, []),
     "dp_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dp_synchronize": (C.c_int, [C.c_void_p]),
+    "dp_check_errors": (C.c_int, [C.c_void_p]),
     "dp_device_info": (C.c_int, [C.c_void_p, i32_p, i32_p]),
     "dp_capture_begin": (C.c_int, [C.c_void_p]),
     "dp_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -321,6 +322,13 @@ class Context:
 
     def synchronize(self):
         check(self.lib.dp_synchronize(self.h))
+
+    def check_errors(self):
+        """Raise what the kernels of earlier _dev calls (or graph replays)
+        found: CapacityError for a path longer than its max_segments,
+        ArgumentError for a cell index outside the field.  Clears the
+        context's sticky error word."""
+        check(self.lib.dp_check_errors(self.h))
 
     def last_kernel_ms(self):
         return float(self.lib.dp_last_kernel_ms(self.h))

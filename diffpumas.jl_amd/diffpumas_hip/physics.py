@@ -261,8 +261,9 @@ class PhysicsTables:
 
 
 def default_data_dir():
-    here = os.path.dirname(os.path.abspath(__file__))
-    return os.path.join(here, "..", "..", "tests", "golden")
+    """The package's material data: PUMAS dE/dx tables (pumas-materials
+    dedx/muon) and the element recipes materials.xml."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
 def create_physics_csda(materials=("StandardRock", "Air"), data_dir=None, mdf=None):

@@ -78,12 +78,28 @@ const char* dp_last_error(void);
  * the context's own stream. */
 int dp_set_stream(dp_ctx ctx, void* stream);
 int dp_synchronize(dp_ctx ctx);
+/* Device-side errors of the _dev entry points (no counterpart in the
+ * reference, where the same conditions throw at once).  A _dev call cannot
+ * report what its kernels find, so they set bits in a sticky word of the
+ * context instead: a path longer than the max_segments it was sized with
+ * (that ray's flux is NaN), or a cell index outside the field (the segment
+ * is skipped).  dp_check_errors synchronises the context's stream, reads and
+ * clears the word, and returns DP_E_ARGUMENT (cell index) or DP_E_CAPACITY
+ * (path length) if any bit was set since the last check, else DP_OK.  Call
+ * it after a chain of _dev calls or graph launches; host entry points check
+ * their own kernels and leave the word alone.                             */
+int dp_check_errors(dp_ctx ctx);
 /* HIP graphs of device work (no counterpart in the reference: Julia re-runs
  * the calls).  Between dp_capture_begin and dp_capture_end the context's
  * stream is captured instead of run; only _dev entry points may be called
  * in between, each already called once with the same sizes so that no
- * buffer grows during the capture (a growth or a host entry point makes the
- * capture fail, reported by dp_capture_end).  dp_graph_launch replays the
+ * buffer grows during the capture (a growth returns DP_E_ARGUMENT at once; a
+ * host entry point makes the capture fail, reported by dp_capture_end).  A
+ * graph holds the addresses of the context's device buffers: once a later
+ * call on the context reallocates one of them (a larger problem than at
+ * capture), dp_graph_launch refuses the graph with DP_E_ARGUMENT, and it
+ * must be captured again; a graph is launched on the context that captured
+ * it.  dp_graph_launch replays the
  * captured work on the context's stream with the device buffers' current
  * contents (e.g. the per-iteration forward model + Jacobian + normal
  * equations of a Gauss-Newton loop, gauss_newton_reconstruct,

@@ -90,3 +90,84 @@ def test_gauss_newton_step_graph(ctx):
         assert np.array_equal(d_N.numpy().reshape(na, na).T, N_ref)
         assert np.array_equal(d_g.numpy(), g_ref)
     graph.close()
+
+
+def _w_dev_setup(ctx, ph, n_dirs, seed, max_seg=None):
+    """Device copies of a traced w-model problem and a closure that enqueues
+    dp_csda_w_flux_jacobian_dev on them (optionally with a max_segments
+    smaller than the longest path)."""
+    paths, n_cells = _traced(ctx, n_dirs, seed)
+    s = dp.csda_energy_quadrature(32, 1.0, 1e4)
+    m = MaterialConfig(0, 1, 2, -1, ph.scalars[0, 0], ph.scalars[1, 0])
+    nr, ns = paths.n_rays, paths.n_segments
+    keep = [D(paths.row_ptr.astype(np.int32)), D(paths.cells.astype(np.int32)), D(paths.lengths),
+            D(paths.cos_theta), D(paths.remaining_rock), D(paths.exit_z),
+            D(paths.valid.astype(np.uint8)), D(np.full(n_cells, 0.1)), D(np.zeros(n_cells, np.uint8)),
+            D(s.energy), D(s.weight), D(s.charge), D(shape=(nr,)), D(shape=(ns,))]
+    rp, cell, ln, cs, rem, ez, val, w, sh, e, wt, q, flux, jac = keep
+    cast = lambda x, t: C.cast(x.ptr, t)
+    cp = L.DirectionalPaths(nr, ns, max_seg or paths.max_segments(), cast(rp, L.i32_p),
+                            cast(cell, L.i32_p), cast(ln, L.dbl_p), cast(cs, L.dbl_p),
+                            cast(rem, L.dbl_p), cast(ez, L.dbl_p),
+                            C.cast(val.ptr, C.POINTER(C.c_uint8)))
+    es = L.EnergySamples(len(s.energy), cast(e, L.dbl_p), cast(wt, L.dbl_p), cast(q, L.dbl_p))
+    mc, sc = m.c(), L.SiteConfig(600.0, 3e4)
+    h = ph.device_handle(ctx)
+
+    def work():
+        return ctx.lib.dp_csda_w_flux_jacobian_dev(ctx.h, h, C.byref(mc), C.byref(sc), C.byref(cp),
+                                                   n_cells, w.ptr, sh.ptr, C.byref(es), flux.ptr,
+                                                   jac.ptr)
+    return work, keep, paths
+
+
+def test_graph_refused_after_reallocation():
+    """A graph points into the context's scratch: once a larger call moves
+    that buffer, the graph is refused (not replayed into freed memory)."""
+    ph = dp.create_physics_csda(("StandardRock", "Water", "Air"))
+    c2 = dp.Context(0)            # a fresh context: its scratch starts small
+    work, keep, _ = _w_dev_setup(c2, ph, 4, 3)
+    L.check(work())
+    c2.synchronize()
+    g = c2.capture(lambda: L.check(work()))
+    g.launch()
+    c2.synchronize()
+    _say("graph: replay before reallocation ok")
+    big, keep2, paths2 = _w_dev_setup(c2, ph, 256, 4)   # more rays, longer paths: scratch grows
+    _say(f"graph: larger call with {paths2.n_rays} bins, {paths2.n_segments} segments")
+    L.check(big())
+    c2.synchronize()
+    with pytest.raises(dp.ArgumentError):
+        g.launch()
+    g.close()
+    g2 = c2.capture(lambda: L.check(work()))   # a new capture uses the new buffers
+    g2.launch()
+    c2.synchronize()
+    g2.close()
+    c2.check_errors()
+    c2.close()
+
+
+def test_check_errors_reports_device_capacity():
+    """A _dev call whose paths exceed the max_segments it was given marks
+    those bins NaN and sets the sticky word; dp_check_errors raises it once
+    and clears it."""
+    ph = dp.create_physics_csda(("StandardRock", "Water", "Air"))
+    c2 = dp.Context(0)
+    ok, keep, paths = _w_dev_setup(c2, ph, 16, 5)
+    L.check(ok())
+    c2.check_errors()                          # nothing to report
+    short = int(np.diff(paths.row_ptr).max()) - 1
+    assert short >= 1
+    bad, keep2, _ = _w_dev_setup(c2, ph, 16, 5, max_seg=short)
+    L.check(bad())                             # enqueued without error
+    flux = keep2[12].numpy()
+    over = np.diff(paths.row_ptr) > short
+    _say(f"check_errors: {int(over.sum())} bins over max_segments {short}")
+    assert np.all(np.isnan(flux[over]))
+    with pytest.raises(dp.CapacityError):
+        c2.check_errors()
+    c2.check_errors()                          # cleared by the first check
+    L.check(ok())                              # a good call after the bad one leaves it clear
+    c2.check_errors()
+    c2.close()

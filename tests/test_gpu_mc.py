@@ -1,11 +1,13 @@
 """GPU parity of the stochastic backward MC (straggled + mixed stepping,
 Coulomb scattering, DEL/EHS hard events) through the C ABI.
 
-The GPU draws the same Philox stream per event as the CPU oracle, so most
-events agree to rounding; transcendental rounding can flip a rejection test,
-after which an event's history legitimately diverges.  Bars: per-event
-agreement for the large majority of events, flux within 3 sigma of the
-oracle and of the reference's recorded Julia runs, bitwise determinism.
+The GPU draws the same Philox stream per event as the CPU oracle, so events
+agree to rounding; only a last-ulp difference of a transcendental can flip a
+rejection test, after which an event's history legitimately diverges.  Bars:
+at least 99 % of events agree to 1e-9 (every run so far: 100 %), no event is
+zero on one side only (a miscompiled or skipped branch), flux within 3 sigma
+of the oracle and of the reference's recorded Julia runs, bitwise
+determinism.
 """
 import csv
 import math
@@ -34,10 +36,17 @@ def _ids(ph):
     return [ph.get_material_index(n) for n in ("StandardRock", "Water", "Air")]
 
 
-def _compare(gpu_ev, orc_ev):
+def _compare(gpu_ev, orc_ev, label):
+    """Per-event agreement; prints the fraction and any diverging events."""
     rel = np.abs(gpu_ev - orc_ev) / np.maximum(np.abs(orc_ev), 1e-300)
     same = (rel <= 1e-9) | ((gpu_ev == 0) & (orc_ev == 0))
-    return same.mean()
+    one_zero = (gpu_ev == 0) != (orc_ev == 0)
+    bad = np.flatnonzero(~same)
+    _say(f"  {label}: per-event agreement {same.mean():.6f} ({same.sum()}/{same.size}), "
+         f"zero on one side only: {int(one_zero.sum())}")
+    for i in bad[:10]:
+        _say(f"    event {i}: gpu {gpu_ev[i]!r} oracle {orc_ev[i]!r}")
+    return same.mean(), int(one_zero.sum())
 
 
 @pytest.mark.parametrize("thick,elev,emin,emax", [(100.0, 45.0, 1.0, 1e3), (20.0, 80.0, 0.1, 50.0),
@@ -51,9 +60,9 @@ def test_events_match_oracle(ph3, ctx, thick, elev, emin, emax):
                            straggling=True, scattering=True, return_events=True, ctx=ctx)
     f, s, ev = O.mc_flat(ph3, [rock], [1.0], 2650.0, thick, air, elev, emin, emax, n, 5,
                          per_event=True)
-    frac = _compare(r["events"], ev)
-    _say(f"  per-event agreement {frac:.4f}; gpu {r['flux']:.6g} +- {r['sigma']:.3g} oracle {f:.6g}")
-    assert frac > 0.9
+    frac, one_zero = _compare(r["events"], ev, f"thick={thick} elev={elev}")
+    _say(f"  gpu {r['flux']:.6g} +- {r['sigma']:.3g} oracle {f:.6g}")
+    assert frac >= 0.99 and one_zero == 0
     assert abs(r["flux"] - f) <= 3 * math.hypot(r["sigma"], s) + 1e-12 * abs(f)
 
 
@@ -67,10 +76,33 @@ def test_mixture_column_matches_oracle(ph3, ctx):
                            scattering=True, rock_mixture=mix, return_events=True, ctx=ctx)
     f, s, ev = O.mc_flat(ph3, [rock, water, air], [0.85, 0.14, 0.01], 2400.0, 300.0, air, 30.0,
                          1.0, 1e4, n, 2, per_event=True)
-    frac = _compare(r["events"], ev)
-    _say(f"  mixture per-event agreement {frac:.4f}; gpu {r['flux']:.6g} oracle {f:.6g}")
-    assert frac > 0.9
+    frac, one_zero = _compare(r["events"], ev, "mixture column")
+    _say(f"  gpu {r['flux']:.6g} oracle {f:.6g}")
+    assert frac >= 0.99 and one_zero == 0
     assert abs(r["flux"] - f) <= 3 * math.hypot(r["sigma"], s)
+
+
+def test_porous_top_layer_matches_oracle(ph3, ctx):
+    """TwoLayerGeometry with its porous top layer (src/Geometry.jl:58-78,
+    stepping :595-620): 40 m of water at 1000 kg/m3 on 160 m of rock,
+    straggled + scattering per event, and deterministic CSDA to 1e-10."""
+    rock, water, air = _ids(ph3)
+    n = 3000
+    kw = dict(rock_thickness=200.0, rock_density=2650.0, elevation=55.0, energy_min=1.0,
+              energy_max=2e3, n_samples=n, seed=8, porous_material=water, porous_density=1000.0,
+              porous_thickness=40.0, ctx=ctx)
+    r = dp.run_backward_mc(ph3, straggling=True, scattering=True, return_events=True, **kw)
+    f, s, ev = O.mc_flat(ph3, [rock], [1.0], 2650.0, 200.0, air, 55.0, 1.0, 2e3, n, 8,
+                         porous=water, porous_rho=1000.0, porous_thick=40.0, per_event=True)
+    frac, one_zero = _compare(r["events"], ev, "porous top layer")
+    assert frac >= 0.99 and one_zero == 0
+    assert abs(r["flux"] - f) <= 3 * math.hypot(r["sigma"], s)
+    c = dp.run_backward_mc(ph3, straggling=False, scattering=False, **kw)
+    o = O.mc_flat_csda(ph3, [200.0, 2650.0, 1000.0, 40.0, 3e4, 0, 0, 0], [rock, air, water],
+                       1.0, 2e3, math.cos(math.radians(35.0)), n, 8, gradient=False)
+    _say(f"  porous CSDA gpu {c['flux']:.12g} oracle {o[0]:.12g}")
+    assert c["flux"] == pytest.approx(o[0], rel=1e-10)
+    assert c["sigma"] == pytest.approx(o[1], rel=1e-8)
 
 
 def test_full102_against_reference_runs(ph3, ctx):

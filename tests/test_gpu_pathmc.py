@@ -63,8 +63,12 @@ def test_w_field_events_match_oracle(ph3, ctx):
                              (s.energy, s.weight, s.charge), 11, per_event=True)
     evo = evo.reshape(ev.shape)
     same = (np.abs(ev - evo) <= 1e-9 * np.maximum(np.abs(evo), 1e-300)) | ((ev == 0) & (evo == 0))
-    _say(f"  per-event agreement {same.mean():.4f}")
-    assert same.mean() > 0.9
+    one_zero = int(((ev == 0) != (evo == 0)).sum())
+    _say(f"  per-event agreement {same.mean():.6f} ({same.sum()}/{same.size}), "
+         f"zero on one side only: {one_zero}")
+    for i in np.argwhere(~same)[:10]:
+        _say(f"    path/event {tuple(i)}: gpu {ev[tuple(i)]!r} oracle {evo[tuple(i)]!r}")
+    assert same.mean() >= 0.99 and one_zero == 0
     assert np.all(np.abs(f - fo) <= 3 * np.hypot(sg, so) + 1e-12 * np.abs(fo))
 
 
