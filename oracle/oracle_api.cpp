@@ -34,6 +34,43 @@ void orc_philox_uniforms(uint64_t seed, uint64_t event, int n, double* out) {
 }
 
 double orc_flux_gccly(double c, double k, double q) { return flux_gccly(c, k, q); }
+
+// ---------------------------------------------------------------- DCS models
+void orc_dcs(int p, double Z, double A, double I, double m, double K, int n, const double* q,
+             double* out) {
+  for (int i = 0; i < n; ++i) out[i] = dcs::dcs(p, Z, A, I, m, K, q[i]);
+}
+double orc_dcs_integral(int p, double Z, double A, double I, double m, double K, double xlow,
+                        double xhigh, int mode, int npoints) {
+  return dcs::integral(p, Z, A, I, m, K, xlow, xhigh, mode, npoints);
+}
+void orc_dilog(int n, const double* x, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = dcs::dilog(x[i]);
+}
+void orc_gauss(int n, double* x, double* w) {
+  const dcs::GaussRule g(n);
+  for (int i = 0; i < n; ++i) { x[i] = g.x[i]; w[i] = g.w[i]; }
+}
+
+// ----------------------------------------------------------------- samplers
+// rotate_direction with the azimuth draw given: n directions, polar mu[i],
+// uniform u[i] for the azimuth.
+void orc_rotate(int n, const double* d_in, const double* mu, const double* u, double* d_out) {
+  for (int i = 0; i < n; ++i) {
+    double d[3] = {d_in[3 * i], d_in[3 * i + 1], d_in[3 * i + 2]};
+    Philox4x32 rng(0, 0);
+    rng.script = u + i; rng.script_n = 1;
+    rotate(d, mu[i], rng);
+    for (int a = 0; a < 3; ++a) d_out[3 * i + a] = d[a];
+  }
+}
+// truncated_randn from n seeded streams (seed, i)
+void orc_truncated_randn(unsigned long long seed, int n, double* out) {
+  for (int i = 0; i < n; ++i) {
+    Philox4x32 rng(seed, (uint64_t)i);
+    out[i] = truncated_randn(rng);
+  }
+}
 double orc_flux_gaisser(double c, double k, double q) { return flux_gaisser(c, k, q); }
 double orc_dflux_gccly_dk(double c, double k, double q) {
   return flux_gccly(c, Dual(k, 1.0), q).d;
@@ -796,6 +833,85 @@ int orc_reverse_del(int n, const double* e, const double* stop, const double* si
 
 // sample_ehs_event, backward (src/Straggling.jl:373-434, :1284-1334).
 // out = accepted, X_ehs, k_ehs.  Returns the number of draws consumed.
+// fluctuate_energy_loss (backward) on a material or mixture: n seeded streams
+// (seed, i), or, with n_script > 0, one call on the scripted draws (the
+// return value is then the number of draws consumed).
+int orc_fluctuate(int M, int K, const double* data, const double* scal, double mass, double ctau,
+                  int n_mix, const int32_t* mats, const double* fracs, double ki, double dX,
+                  double Xtot, unsigned long long seed, int n, const double* script, int n_script,
+                  double* kout, double* ratio) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Mixture m = mixture_from(T, n_mix, mats, fracs);
+  if (n_script > 0) {
+    Philox4x32 rng(0, 0);
+    rng.script = script; rng.script_n = n_script;
+    fluctuate(T, m, ki, dX, Xtot, rng, kout[0], ratio[0]);
+    return rng.script_pos;
+  }
+  for (int i = 0; i < n; ++i) {
+    Philox4x32 rng(seed, (uint64_t)i);
+    fluctuate(T, m, ki, dX, Xtot, rng, kout[i], ratio[i]);
+  }
+  return 0;
+}
+
+// sample_soft_scattering on n seeded streams (seed, i).
+void orc_soft_scattering(int M, int K, const double* data, const double* scal, double mass,
+                         double ctau, int n_mix, const int32_t* mats, const double* fracs,
+                         double k0, double k1, double step, double density,
+                         unsigned long long seed, int n, double* out) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Mixture m = mixture_from(T, n_mix, mats, fracs);
+  for (int i = 0; i < n; ++i) {
+    Philox4x32 rng(seed, (uint64_t)i);
+    out[i] = soft_scattering(T, m, k0, k1, step, density, rng);
+  }
+}
+
+// sample_scattering_angle at energy k: mode 0 = n lab-frame samples from
+// seeded streams; mode 1 = the centre-of-mass draws alone (n samples); mode
+// 2 = the lab transform of the n centre-of-mass values in mu1 (no draws).
+// kin[5] receives (mu0, A, fspin, gCM, tau).
+void orc_scattering_angle(int M, int K, const double* data, const double* scal, double mass,
+                          double ctau, int n_mix, const int32_t* mats, const double* fracs,
+                          double k, int mode, unsigned long long seed, int n, const double* mu1,
+                          double* out, double* kin) {
+  Tables T = make_tables(M, K, data, scal, mass, ctau);
+  Mixture m = mixture_from(T, n_mix, mats, fracs);
+  const WentzelKin w = wentzel_kinematics(T, m, k);
+  if (kin) { kin[0] = w.mu0; kin[1] = w.A; kin[2] = w.fspin; kin[3] = w.gCM; kin[4] = w.tau; }
+  for (int i = 0; i < n; ++i) {
+    Philox4x32 rng(seed, (uint64_t)i);
+    if (mode == 0) out[i] = scattering_angle(T, m, k, rng);
+    else if (mode == 1) out[i] = wentzel_cm(w, rng);
+    else out[i] = wentzel_lab(w, mu1[i]);
+  }
+}
+
+// DEL polar angles (src/Straggling.jl:746-914): proc 1/2 bremsstrahlung and
+// pair production (target charge Z), 3 photonuclear, 4 ionisation; ki is the
+// energy before the event, kf after it.  n seeded streams, or one scripted
+// call (returns the draws consumed).
+int orc_polar(int proc, double m, double ki, double kf, double Z, unsigned long long seed, int n,
+              const double* script, int n_script, double* out) {
+  auto one = [&](Philox4x32& rng) {
+    if (proc == 1 || proc == 2) return polar_brems(m, ki, kf, Z, rng);
+    if (proc == 3) return polar_photo(m, ki, kf, rng);
+    return polar_ionisation(m, ki, kf);
+  };
+  if (n_script > 0) {
+    Philox4x32 rng(0, 0);
+    rng.script = script; rng.script_n = n_script;
+    out[0] = one(rng);
+    return rng.script_pos;
+  }
+  for (int i = 0; i < n; ++i) {
+    Philox4x32 rng(seed, (uint64_t)i);
+    out[i] = one(rng);
+  }
+  return 0;
+}
+
 int orc_ehs_sample(int M, int K, const double* data, const double* scal, double mass, double ctau,
                    int n_mix, const int32_t* mats, const double* fracs, double ki, double kf,
                    double dX, double Xtot, double ratio, double er, const double* script,

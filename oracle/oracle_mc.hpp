@@ -350,11 +350,17 @@ inline bool sample_ehs(const Tables& T, const Mixture& m, double ki, double kf, 
   return true;
 }
 
-// sample_scattering_angle, src/Straggling.jl:446-517, :1344-1405
-inline double scattering_angle(const Tables& T, const Mixture& m, double k, Philox4x32& rng) {
+// sample_scattering_angle, src/Straggling.jl:446-517, :1344-1405, in two
+// parts: the Wentzel draw with spin rejection in the centre-of-mass frame,
+// then the transformation of that angle to the laboratory frame.
+struct WentzelKin {
+  double mu0, A, fspin, gCM, tau;
+};
+inline WentzelKin wentzel_kinematics(const Tables& T, const Mixture& m, double k) {
   const double mass = T.mass;
   const double p2 = k * (k + 2.0 * mass);
-  const double mu0 = mix_screening(T, m, k);
+  WentzelKin w;
+  w.mu0 = mix_screening(T, m, k);
   const double zoa = mix_zoa(T, m);
   const double At = zoa > 0 ? 1.0 / zoa : 22.0;
   const double Mt = At * NEUTRON_MASS_;
@@ -364,10 +370,14 @@ inline double scattering_angle(const Tables& T, const Mixture& m, double k, Phil
   const double pcm = std::sqrt(std::max(k0 * (k0 + 2.0 * mass), 0.0));
   const double Elab = k + mass;
   const double plab = std::sqrt(std::max(p2, 0.0));
-  const double gCM = (plab > 0 && pcm > 0) ? (Elab * Ecm + plab * pcm) / (mass * std::sqrt(s)) : 1.0;
-  const double tau = (plab > 0 && pcm > 0) ? (Elab * pcm - Ecm * plab) / (plab * pcm) : 0.0;
-  const double A = mu0 / 4.0;
-  const double fspin = mix_prop_sum(T, T_SPIN_FACTOR, m, k);
+  w.gCM = (plab > 0 && pcm > 0) ? (Elab * Ecm + plab * pcm) / (mass * std::sqrt(s)) : 1.0;
+  w.tau = (plab > 0 && pcm > 0) ? (Elab * pcm - Ecm * plab) / (plab * pcm) : 0.0;
+  w.A = w.mu0 / 4.0;
+  w.fspin = mix_prop_sum(T, T_SPIN_FACTOR, m, k);
+  return w;
+}
+inline double wentzel_cm(const WentzelKin& w, Philox4x32& rng) {
+  const double A = w.A, mu0 = w.mu0;
   const double Apm = A + mu0, Ap1 = A + 1.0, omm = 1.0 - mu0;
   double mu1 = mu0;
   for (int t = 0; t < 100; ++t) {
@@ -376,18 +386,25 @@ inline double scattering_angle(const Tables& T, const Mixture& m, double k, Phil
     if (tmp <= 0) continue;
     double trial = Apm * Ap1 / tmp - A;
     trial = std::min(std::max(trial, mu0), 1.0);
-    if (rng.uniform() <= 1.0 - fspin * trial) { mu1 = trial; break; }
+    if (rng.uniform() <= 1.0 - w.fspin * trial) { mu1 = trial; break; }
   }
+  return mu1;
+}
+inline double wentzel_lab(const WentzelKin& w, double mu1) {
   double mu;
   if (mu1 > 1e-6) {
-    const double a = gCM * (tau + 1.0 - 2.0 * mu1);
+    const double a = w.gCM * (w.tau + 1.0 - 2.0 * mu1);
     const double cth = a / std::sqrt(4.0 * mu1 * (1.0 - mu1) + a * a);
     mu = 0.5 * (1.0 - cth);
   } else {
-    const double d = gCM * (1.0 + tau);
+    const double d = w.gCM * (1.0 + w.tau);
     mu = mu1 / (d * d);
   }
   return std::min(std::max(mu, 0.0), 1.0);
+}
+inline double scattering_angle(const Tables& T, const Mixture& m, double k, Philox4x32& rng) {
+  const WentzelKin w = wentzel_kinematics(T, m, k);
+  return wentzel_lab(w, wentzel_cm(w, rng));
 }
 
 // sample_soft_scattering, src/Straggling.jl:539-626, :1204-1250

@@ -649,3 +649,105 @@ def trace_tetra(points, tets, nbr, origins, dirs, surface):
     lib().orc_trace_tetra(*args, P(counts, C.c_int), P(rp, C.c_int), P(cells, C.c_int), P(lens),
                           P(tr), P(start, C.c_int))
     return rp, cells[:rp[-1]], lens[:rp[-1]], tr, start
+
+
+# ------------------------------------------------ DCS models and samplers
+_TAB = [C.c_int, C.c_int, d_p, d_p, C.c_double, C.c_double]
+_SIGS["orc_dcs"] = (None, [C.c_int] + [C.c_double] * 5 + [C.c_int, d_p, d_p])
+_SIGS["orc_dcs_integral"] = (C.c_double, [C.c_int] + [C.c_double] * 7 + [C.c_int, C.c_int])
+_SIGS["orc_dilog"] = (None, [C.c_int, d_p, d_p])
+_SIGS["orc_gauss"] = (None, [C.c_int, d_p, d_p])
+_SIGS["orc_rotate"] = (None, [C.c_int, d_p, d_p, d_p, d_p])
+_SIGS["orc_truncated_randn"] = (None, [C.c_ulonglong, C.c_int, d_p])
+_SIGS["orc_fluctuate"] = (C.c_int, _TAB + [C.c_int, i_p, d_p, C.c_double, C.c_double, C.c_double,
+                                           C.c_ulonglong, C.c_int, d_p, C.c_int, d_p, d_p])
+_SIGS["orc_soft_scattering"] = (None, _TAB + [C.c_int, i_p, d_p, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, C.c_ulonglong, C.c_int,
+                                              d_p])
+_SIGS["orc_scattering_angle"] = (None, _TAB + [C.c_int, i_p, d_p, C.c_double, C.c_int,
+                                               C.c_ulonglong, C.c_int, d_p, d_p, d_p])
+_SIGS["orc_polar"] = (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_ulonglong, C.c_int, d_p, C.c_int, d_p])
+_lib = None
+
+
+def dcs(p, Z, A, I, m, K, q):
+    q = f64(np.atleast_1d(q))
+    out = np.zeros(len(q))
+    lib().orc_dcs(p, Z, A, I, m, K, len(q), P(q), P(out))
+    return out
+
+
+def dcs_integral(p, Z, A, I, m, K, xlow, xhigh, mode, npoints=180):
+    return lib().orc_dcs_integral(p, Z, A, I, m, K, xlow, xhigh, mode, npoints)
+
+
+def dilog(x):
+    x = f64(np.atleast_1d(x))
+    out = np.zeros(len(x))
+    lib().orc_dilog(len(x), P(x), P(out))
+    return out
+
+
+def gauss_rule(n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_gauss(n, P(x), P(w))
+    return x, w
+
+
+def rotate(d, mu, u):
+    d = f64(np.atleast_2d(d))
+    mu, u = f64(np.broadcast_to(mu, len(d))), f64(np.broadcast_to(u, len(d)))
+    out = np.zeros_like(d)
+    lib().orc_rotate(len(d), P(d), P(mu), P(u), P(out))
+    return out
+
+
+def truncated_randn(seed, n):
+    out = np.zeros(n)
+    lib().orc_truncated_randn(seed, n, P(out))
+    return out
+
+
+def fluctuate(ph, mats, fracs, ki, dX, Xtot, seed=0, n=1, script=None):
+    """(kout, ratio) arrays from n seeded streams, or (kout, ratio, draws
+    consumed) of one scripted call."""
+    m, f = _mix(mats, fracs)
+    sc = None if script is None else f64(script)
+    nn = 1 if sc is not None else n
+    k, r = np.zeros(nn), np.zeros(nn)
+    used = lib().orc_fluctuate(*tabargs(ph), ph.mass, ph.ctau, len(m), P(m, C.c_int32), P(f), ki,
+                               dX, Xtot, seed, nn, P(sc), 0 if sc is None else len(sc), P(k), P(r))
+    return (k[0], r[0], used) if sc is not None else (k, r)
+
+
+def soft_scattering(ph, mats, fracs, k0, k1, step, density, seed, n):
+    m, f = _mix(mats, fracs)
+    out = np.zeros(n)
+    lib().orc_soft_scattering(*tabargs(ph), ph.mass, ph.ctau, len(m), P(m, C.c_int32), P(f), k0,
+                              k1, step, density, seed, n, P(out))
+    return out
+
+
+def scattering_angle(ph, mats, fracs, k, mode=0, seed=0, n=1, mu1=None, data=None):
+    """mode 0: lab-frame samples; 1: centre-of-mass draws; 2: lab transform
+    of the given mu1.  Returns (values, (mu0, A, fspin, gCM, tau)).  `data`
+    replaces the table array (e.g. with the spin factor zeroed)."""
+    m, f = _mix(mats, fracs)
+    mu1 = None if mu1 is None else f64(mu1)
+    nn = len(mu1) if mode == 2 else n
+    out, kin = np.zeros(nn), np.zeros(5)
+    d = f64(ph.data if data is None else data)
+    lib().orc_scattering_angle(ph.n_materials, ph.n_knots, P(d), P(ph.scalars), ph.mass, ph.ctau,
+                               len(m), P(m, C.c_int32), P(f), k, mode, seed, nn, P(mu1), P(out),
+                               P(kin))
+    return out, kin
+
+
+def polar(proc, m, ki, kf, Z=11.0, seed=0, n=1, script=None):
+    sc = None if script is None else f64(script)
+    nn = 1 if sc is not None else n
+    out = np.zeros(nn)
+    used = lib().orc_polar(proc, m, ki, kf, Z, seed, nn, P(sc), 0 if sc is None else len(sc),
+                           P(out))
+    return (out[0], used) if sc is not None else out
