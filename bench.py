@@ -182,7 +182,66 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
         "valu": valu_roofline("csda_cube_kernel", units, kern_ms),
         "flux_sum": float(flux_h.sum()), "grad_norm": float(grad.norm().item()),
         "_problem": (geo, paths, rho, ne, nw, nq),
+        "_device": dict(rp=rp, cell=cell, length=length, cos_t=cos_t, z_exit=z_exit, rho=d_rho,
+                        nodes=nodes, h=h, flux=flux.clone(), jac=jac.clone(), jrem=jrem.clone()),
     }
+
+
+def run_adjoint_per_ray(dp, L, ctx, torch, adj):
+    """The plain per-ray GPU launch loop the batched adjoint is measured
+    against: the same forward + per-segment Jacobian, one dp_csda_cube_dev
+    launch per ray (device-resident per-ray views of the same arrays), the
+    shape of the reference's loop over rays (src/Tomography.jl:773-820).
+    Every ray's flux and Jacobian must equal the batched launch bit for bit."""
+    import ctypes as C
+    geo, paths = adj["_problem"][0], adj["_problem"][1]
+    d = adj["_device"]
+    dev = torch.device("cuda:0")
+    nr = paths.n_rays
+    rp_h = paths.row_ptr.astype("int64")
+    lens = rp_h[1:] - rp_h[:-1]
+    rp1 = torch.zeros((nr, 2), dtype=torch.int32, device=dev)
+    rp1[:, 1] = torch.as_tensor(lens, dtype=torch.int32, device=dev)
+    flux = torch.zeros(nr, dtype=torch.float64, device=dev)
+    jac = torch.zeros(max(paths.n_segments, 1), dtype=torch.float64, device=dev)
+    jrem = torch.zeros(nr, dtype=torch.float64, device=dev)
+    views = []
+    for r in range(nr):
+        a, n = int(rp_h[r]), int(lens[r])
+        cp = L.Paths(1, n, max(n, 1), C.cast(rp1.data_ptr() + 8 * r, L.i32_p),
+                     C.cast(d["cell"].data_ptr() + 4 * a, L.i32_p),
+                     C.cast(d["length"].data_ptr() + 8 * a, L.dbl_p),
+                     C.cast(d["cos_t"].data_ptr() + 8 * r, L.dbl_p),
+                     C.cast(d["z_exit"].data_ptr() + 8 * r, L.dbl_p))
+        views.append((cp, flux.data_ptr() + 8 * r, jac.data_ptr() + 8 * a, jrem.data_ptr() + 8 * r))
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream.cuda_stream)
+    lib = ctx.lib
+
+    def loop():
+        for cp, fp, jp, jr in views:
+            L.check(lib.dp_csda_cube_dev(ctx.h, d["h"], C.byref(cp), C.byref(d["nodes"]),
+                                         d["rho"].data_ptr(), geo.n_cells, fp, jp, jr))
+
+    with torch.cuda.stream(stream):
+        loop()   # warm-up
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        loop()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3
+    ctx.set_stream(None)
+    dev_ms = e0.elapsed_time(e1)
+    same = bool(torch.equal(flux, d["flux"]) and torch.equal(jac, d["jac"]) and
+                torch.equal(jrem, d["jrem"]))
+    return {"rays": nr, "launches": nr, "device_ms": dev_ms, "wall_ms": wall,
+            "value": paths.n_segments / (dev_ms * 1e-3), "unit": "rays*voxels/s",
+            "bitwise_equal_to_batched": same,
+            "speedup_of_batched": dev_ms / adj["kernel_ms"]}
 
 
 def mc_kwargs(physics, n_events, ctx):
@@ -761,6 +820,32 @@ def run_pathmc(dp, physics, ctx, n_rays=4096, n_events=10000, steps=2, warmup=1)
             "_problem": (paths, rho, s, m, site)}
 
 
+def run_pathmc_per_ray(dp, physics, ctx, pm, n_sub=64):
+    """The plain per-ray GPU launch loop for the fixed-path MC: one
+    compute_directional_flux_mc launch per ray with its 1e4 events (the
+    reference's loop over bins), on n_sub rays spread over the 4096; device
+    time of the launches (staging excluded), scaled to all rays."""
+    import numpy as np
+    paths, rho, s, m, site = pm["_problem"]
+    idx = np.linspace(0, paths.n_rays - 1, n_sub).astype(int)
+    kms = 0.0
+    t0 = time.perf_counter()
+    for r in idx:
+        a, b = paths.row_ptr[r], paths.row_ptr[r + 1]
+        one = dp.DirectionalPaths([0, b - a], paths.cells[a:b], paths.lengths[a:b],
+                                  paths.zenith[r:r + 1], paths.azimuth[r:r + 1],
+                                  paths.remaining_rock[r:r + 1], paths.exit_z[r:r + 1])
+        dp.compute_directional_flux_mc(physics, m, site, one, [((m.rock_material,), (1.0,))],
+                                       np.zeros(len(rho), np.int32), rho, s, 1, ctx=ctx)
+        kms += ctx.last_kernel_ms()
+    wall = (time.perf_counter() - t0) * 1e3
+    scale = paths.n_rays / n_sub
+    return {"rays_timed": n_sub, "rays": paths.n_rays, "events_per_ray": len(s.energy),
+            "device_ms_all_rays": kms * scale, "wall_ms_all_rays_incl_staging": wall * scale,
+            "value": paths.n_rays * len(s.energy) / (kms * scale * 1e-3), "unit": "events/s",
+            "speedup_of_batched": kms * scale / pm["kernel_ms"]}
+
+
 def cpu_wmodel_baseline(physics, problem, budget_s=10.0, corr=None):
     """The oracle's w-model flux + reverse Jacobian on a bounded sample of bins
     (corrections on when corr is given)."""
@@ -878,12 +963,21 @@ def main():
     adj = run_adjoint(dp, L, physics, ctx, torch, a.steps, a.warmup)
     log(f"adjoint: {adj['value']:.4g} rays*voxels/s, {adj['ms_per_step']:.3f} ms/step, "
         f"kernel {adj['kernel_ms']:.3f} ms")
+    per_ray_adj = run_adjoint_per_ray(dp, L, ctx, torch, adj)
+    log(f"adjoint per-ray loop: {per_ray_adj['device_ms']:.1f} ms device for "
+        f"{per_ray_adj['launches']} launches, batched is {per_ray_adj['speedup_of_batched']:.0f}x "
+        f"faster, bitwise equal {per_ray_adj['bitwise_equal_to_batched']}")
+    adj.pop("_device")
     problem = adj.pop("_problem")
     wm = run_wmodel(dp, L, physics, ctx, torch, a.steps, a.warmup)
     log(f"wmodel: {wm['value']:.4g} rays*voxels/s, {wm['ms_per_step']:.3f} ms/step")
     wprob = wm.pop("_problem")
     pm = run_pathmc(dp, physics, ctx)
     log(f"pathmc: {pm['value']:.4g} events/s, kernel {pm['kernel_ms']:.3f} ms")
+    per_ray_pm = run_pathmc_per_ray(dp, physics, ctx, pm)
+    log(f"pathmc per-ray loop: {per_ray_pm['device_ms_all_rays']:.1f} ms device (scaled), "
+        f"batched is {per_ray_pm['speedup_of_batched']:.1f}x faster")
+    pm["per_ray_loop"] = per_ray_pm
     pprob = pm.pop("_problem")
     dr = run_direct(dp, L, physics, ctx, torch, a.steps, a.warmup)
     log(f"direct: {dr['value']:.4g} samples/s, kernel {dr['kernel_ms']:.3f} ms, "
@@ -936,7 +1030,8 @@ def main():
                                 "16384 rays (8 det x 2048 dirs), 256 nodes 1-1e4 GeV",
                     "rays": adj["rays"], "segments": adj["segments"], "nodes": adj["nodes"],
                     "ms_per_step": adj["ms_per_step"], "kernel_ms": adj["kernel_ms"],
-                    "roofline": adj["roofline"], "valu": adj["valu"], "cpu_baseline": cpu_adj},
+                    "roofline": adj["roofline"], "valu": adj["valu"], "cpu_baseline": cpu_adj,
+                    "per_ray_loop": per_ray_adj},
         "wmodel": dict(wm, workload="water-fraction CSDA flux + sparse Jacobian, 64x64x32 "
                                     "voxels of 10 m, w U[0, 0.3], 4 detectors x 1024 "
                                     "directions, 32 nodes 1-1e4 GeV"),
