@@ -75,6 +75,15 @@ int dp_create(dp_ctx* out, int device) {
     delete c;
     return dph::fail(DP_E_HIP, "dp_create: counter buffer: " + msg);
   }
+  if (dph::noalloc_scratch(device, &c->noalloc_scratch) != DP_OK) {
+    const std::string msg = dp_last_error();
+    dph::release(c->counter);
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return dph::fail(DP_E_HIP, "dp_create: " + msg);
+  }
   *out = c;
   return DP_OK;
 }

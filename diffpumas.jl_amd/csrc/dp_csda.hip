@@ -18,6 +18,8 @@
 // clamped), i.e. what Zygote returns on the reference.  Per-segment sums over
 // nodes use a fixed wave/LDS tree, so results are deterministic.
 #include "dp_internal.hpp"
+#include <map>
+#include <mutex>
 #include "dp_device.hpp"
 #include "dp_sort.hpp"
 #include <climits>
@@ -498,7 +500,101 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
   return DP_OK;
 }
 
+// Byte offsets of the VJP's scratch: cell-sorted keys/values and their sort
+// buffers, sort histograms, remaining-rock partials, chunk head/tail sums.
+struct VjpLayout {
+  size_t k, v, ro, p0, p1, h, part, hd, tl, total;
+};
+VjpLayout vjp_layout(int nr, int ns) {
+  VjpLayout L;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
+  const size_t ne = (size_t)(ns > 0 ? ns : 0);
+  L.k = take(4 * ne); L.v = take(4 * ne); L.ro = take(4 * ne);
+  L.p0 = take(8 * ne); L.p1 = take(8 * ne);
+  L.h = take(4 * dps::sort_hist_ints(ns > 0 ? ns : 0));
+  const int n_part = (nr + VJP_REM_THREADS * VJP_REM_PER - 1) / (VJP_REM_THREADS * VJP_REM_PER);
+  L.part = take(8 * (size_t)(n_part > 0 ? n_part : 0));
+  const int n_chunks = (ns + VJP_CHUNK - 1) / VJP_CHUNK;
+  L.hd = take(8 * (size_t)(n_chunks > 0 ? n_chunks : 0) + 8);
+  L.tl = take(8 * (size_t)(n_chunks > 0 ? n_chunks : 0) + 8);
+  L.total = o;
+  return L;
+}
+size_t vjp_scratch_bytes(int nr, int ns) { return vjp_layout(nr, ns).total; }
+
+// The VJP's kernels on the scratch at b (vjp_scratch_bytes(nr, ns) bytes; no
+// part of it needs to start at zero: every kernel writes what it later reads).
+int vjp_launch(dp_ctx c, const dp_paths* p, const double* jac, const double* jac_rem,
+               const double* fbar, int32_t n_cells, double* grad, char* b) {
+  const int nr = p->n_rays, ns = p->n_segments;
+  if (nr == 0) {   // a fill kernel, not a memset node, so the call can sit in a captured graph
+    hipLaunchKernelGGL(vjp_zero, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
+                       grad);
+    DP_HIP(hipGetLastError());
+    return DP_OK;
+  }
+  int bits = 1;
+  while ((1ll << bits) < (long long)n_cells) ++bits;
+  const VjpLayout Lo = vjp_layout(nr, ns);
+  const int n_part = (nr + VJP_REM_THREADS * VJP_REM_PER - 1) / (VJP_REM_THREADS * VJP_REM_PER);
+  const int n_chunks = (ns + VJP_CHUNK - 1) / VJP_CHUNK;
+  int32_t *k = (int32_t*)(b + Lo.k), *v = (int32_t*)(b + Lo.v), *ro = (int32_t*)(b + Lo.ro);
+  hipLaunchKernelGGL(vjp_keys, dim3(nr), dim3(64), 0, c->stream, nr, ns, p->row_ptr, p->cell, k, v,
+                     ro);
+  DP_HIP(hipGetLastError());
+  const int32_t *sk = k, *sv = v;
+  if (ns > 0)
+    dps::radix_sort_pairs(k, v, ns, bits, (int32_t*)(b + Lo.p0), (int32_t*)(b + Lo.p1),
+                          (int32_t*)(b + Lo.h), c->stream, &sk, &sv);
+  hipLaunchKernelGGL(vjp_zero, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
+                     grad);
+  if (ns > 0) {
+    double* head = (double*)(b + Lo.hd);
+    double* tail = (double*)(b + Lo.tl);
+    hipLaunchKernelGGL(vjp_chunk, dim3(n_chunks), dim3(VJP_CHUNK), 0, c->stream, ns, n_cells, sk,
+                       sv, ro, jac, fbar, grad, head, tail);
+    hipLaunchKernelGGL(vjp_join, dim3((n_chunks + 255) / 256), dim3(256), 0, c->stream, ns,
+                       n_chunks, n_cells, sk, head, tail, grad);
+  }
+  if (jac_rem) {
+    double* part = (double*)(b + Lo.part);
+    hipLaunchKernelGGL(vjp_rem_partial, dim3(n_part), dim3(VJP_REM_THREADS), 0, c->stream, nr,
+                       jac_rem, fbar, part);
+    hipLaunchKernelGGL(vjp_rem_final, dim3(1), dim3(VJP_REM_THREADS), 0, c->stream, n_part, part,
+                       grad);
+  }
+  DP_HIP(hipGetLastError());
+  return DP_OK;
+}
+
+// Scratch of dp_csda_cube_vjp_dev_noalloc: one copy per GPU, for the whole
+// process.  Host code never uses this name as an address: it uses the
+// per-device address that hipGetSymbolAddress returns (dph::noalloc_scratch).
+__device__ __align__(256) char scratch[DP_VJP_NOALLOC_SCRATCH_BYTES];
+
 }  // namespace
+
+namespace dph {
+// The device address of `scratch` on `device`, looked up once per device
+// (with that device current) and kept for the process.
+int noalloc_scratch(int device, void** out) {
+  static std::mutex mu;
+  static std::map<int, void*> addr;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(device);
+  if (it == addr.end()) {
+    DP_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(DP_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = addr.emplace(device, p).first;
+  }
+  *out = it->second;
+  return DP_OK;
+}
+}  // namespace dph
 
 extern "C" {
 
@@ -591,54 +687,31 @@ int dp_csda_cube_vjp_dev(dp_ctx c, const dp_paths* p, const double* jac, const d
   if (!c || !p || !jac || !fbar || !grad) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: null argument");
   if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp: need at least one cell");
   DP_HIP(hipSetDevice(c->device));
-  const int nr = p->n_rays, ns = p->n_segments, ne = ns;
-  if (nr == 0) {   // a fill kernel, not a memset node, so the call can sit in a captured graph
-    hipLaunchKernelGGL(vjp_zero, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
-                       grad);
-    DP_HIP(hipGetLastError());
-    return DP_OK;
-  }
-  int bits = 1;
-  while ((1ll << bits) < (long long)n_cells) ++bits;
-  size_t o = 0;
-  auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
-  const size_t o_k = take(4 * (size_t)ne), o_v = take(4 * (size_t)ne), o_ro = take(4 * (size_t)ne),
-               o_p0 = take(8 * (size_t)ne), o_p1 = take(8 * (size_t)ne),
-               o_h = take(4 * dps::sort_hist_ints(ne));
-  const int n_part = (nr + VJP_REM_THREADS * VJP_REM_PER - 1) / (VJP_REM_THREADS * VJP_REM_PER);
-  const size_t o_part = take(8 * (size_t)n_part);
-  const int n_chunks = (ns + VJP_CHUNK - 1) / VJP_CHUNK;
-  const size_t o_hd = take(8 * (size_t)n_chunks + 8), o_tl = take(8 * (size_t)n_chunks + 8);
-  int rc = dph::ensure(c->scratch2, o);
+  const size_t need = vjp_scratch_bytes(p->n_rays, p->n_segments);
+  int rc = dph::ensure(c->scratch2, need);
   if (rc) return rc;
-  char* b = (char*)c->scratch2.ptr;
-  int32_t *k = (int32_t*)(b + o_k), *v = (int32_t*)(b + o_v), *ro = (int32_t*)(b + o_ro);
-  hipLaunchKernelGGL(vjp_keys, dim3(nr), dim3(64), 0, c->stream, nr, ns, p->row_ptr, p->cell, k, v,
-                     ro);
-  DP_HIP(hipGetLastError());
-  const int32_t *sk = k, *sv = v;
-  if (ns > 0)
-    dps::radix_sort_pairs(k, v, ns, bits, (int32_t*)(b + o_p0), (int32_t*)(b + o_p1),
-                          (int32_t*)(b + o_h), c->stream, &sk, &sv);
-  hipLaunchKernelGGL(vjp_zero, dim3((n_cells + 255) / 256), dim3(256), 0, c->stream, n_cells,
-                     grad);
-  if (ns > 0) {
-    double* head = (double*)(b + o_hd);
-    double* tail = (double*)(b + o_tl);
-    hipLaunchKernelGGL(vjp_chunk, dim3(n_chunks), dim3(VJP_CHUNK), 0, c->stream, ns, n_cells, sk,
-                       sv, ro, jac, fbar, grad, head, tail);
-    hipLaunchKernelGGL(vjp_join, dim3((n_chunks + 255) / 256), dim3(256), 0, c->stream, ns,
-                       n_chunks, n_cells, sk, head, tail, grad);
-  }
-  if (jac_rem) {
-    double* part = (double*)(b + o_part);
-    hipLaunchKernelGGL(vjp_rem_partial, dim3(n_part), dim3(VJP_REM_THREADS), 0, c->stream, nr,
-                       jac_rem, fbar, part);
-    hipLaunchKernelGGL(vjp_rem_final, dim3(1), dim3(VJP_REM_THREADS), 0, c->stream, n_part, part,
-                       grad);
-  }
-  DP_HIP(hipGetLastError());
-  return DP_OK;
+  return vjp_launch(c, p, jac, jac_rem, fbar, n_cells, grad, (char*)c->scratch2.ptr);
+}
+
+int dp_csda_cube_vjp_dev_noalloc(dp_ctx c, const dp_paths* p, const double* jac,
+                                 const double* jac_rem, const double* fbar, int32_t n_cells,
+                                 double* grad) {
+  if (!c || !p || !jac || !fbar || !grad)
+    return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp_dev_noalloc: null argument");
+  if (n_cells <= 0) return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp_dev_noalloc: need at least one cell");
+  if (p->n_rays < 0 || p->n_segments < 0)
+    return dph::fail(DP_E_ARGUMENT, "dp_csda_cube_vjp_dev_noalloc: negative sizes");
+  const size_t need = vjp_scratch_bytes(p->n_rays, p->n_segments);
+  if (need > DP_VJP_NOALLOC_SCRATCH_BYTES)
+    return dph::fail(DP_E_CAPACITY, "dp_csda_cube_vjp_dev_noalloc: " + std::to_string(p->n_segments) +
+                                        " segments and " + std::to_string(p->n_rays) + " rays need " +
+                                        std::to_string(need) + " bytes of scratch; the static buffer holds " +
+                                        std::to_string((size_t)DP_VJP_NOALLOC_SCRATCH_BYTES) +
+                                        " (about 2.2e6 segments)");
+  if (!c->noalloc_scratch)
+    return dph::fail(DP_E_HIP, "dp_csda_cube_vjp_dev_noalloc: the context has no static scratch address");
+  DP_HIP(hipSetDevice(c->device));
+  return vjp_launch(c, p, jac, jac_rem, fbar, n_cells, grad, (char*)c->noalloc_scratch);
 }
 
 int dp_csda_cube_vjp(dp_ctx c, const dp_paths* p, const double* jac, const double* jac_rem,

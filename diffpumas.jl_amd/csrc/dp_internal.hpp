@@ -80,6 +80,7 @@ struct dp_ctx_s {
   dp_csda_material_correction mcorr{};
   bool capturing = false;               // between dp_capture_begin and dp_capture_end
   uint64_t generation = 0;              // bumped by every scratch reallocation
+  void* noalloc_scratch = nullptr;      // this device's static scratch (no-allocation VJP)
 };
 
 namespace dph {
@@ -100,6 +101,8 @@ constexpr int COUNTER_STICKY = 16;
 // the field.
 constexpr int KERR_CAPACITY = 1;
 constexpr int KERR_CELL = 2;
+// Device address of the no-allocation VJP's static scratch on `device`.
+int noalloc_scratch(int device, void** out);
 // Timing event on the context's stream; not recorded while the stream is
 // being captured into a graph.
 inline void record(dp_ctx c, hipEvent_t e) {

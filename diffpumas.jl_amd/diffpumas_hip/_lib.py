@@ -204,6 +204,8 @@ _SIGS = {
                                    C.c_int32, dbl_p]),
     "dp_csda_cube_vjp_dev": (C.c_int, [C.c_void_p, C.POINTER(Paths), C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_int32, C.c_void_p]),
+    "dp_csda_cube_vjp_dev_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Paths), C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int32, C.c_void_p]),
     "dp_mc_flat_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),
                                       C.POINTER(McConfig), C.POINTER(McResult)]),
     "dp_mc_flat_backward_events": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FlatGeometry),

@@ -249,6 +249,23 @@ int dp_csda_cube_vjp_dev(dp_ctx ctx, const dp_paths* paths, const double* jac,
                          const double* jac_rem, const double* flux_bar, int32_t n_cells,
                          double* grad);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dp_csda_cube_vjp_dev with the same arguments, results
+ * and kernels, whose scratch (the cell-sorted transpose and its sort buffers,
+ * chunk partial sums) is a static device array of
+ * DP_VJP_NOALLOC_SCRATCH_BYTES held by every GPU the library runs on until the
+ * process ends, instead of a context buffer.  It calls no hipMalloc, hipFree
+ * or allocating library routine.  Problems needing more scratch (about 29
+ * bytes per segment plus 1 per ray: up to about 2.2e6 segments) return
+ * DP_E_CAPACITY before anything is launched.  All calls on one GPU share that
+ * GPU's array, so calls that overlap on one GPU (several streams or threads)
+ * race; serialise them.                                                     */
+#define DP_VJP_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int dp_csda_cube_vjp_dev_noalloc(dp_ctx ctx, const dp_paths* paths, const double* jac,
+                                 const double* jac_rem, const double* flux_bar, int32_t n_cells,
+                                 double* grad);
+
 /* -------------------------------------------- flat backward Monte Carlo
  * TwoLayerGeometry (src/Geometry.jl:58-78), materials 0-based, porous < 0
  * disables the porous layer.                                               */
