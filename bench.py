@@ -182,8 +182,10 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
         "valu": valu_roofline("csda_cube_kernel", units, kern_ms),
         "flux_sum": float(flux_h.sum()), "grad_norm": float(grad.norm().item()),
         "_problem": (geo, paths, rho, ne, nw, nq),
+        # the node arrays stay referenced: `nodes` holds their device addresses
         "_device": dict(rp=rp, cell=cell, length=length, cos_t=cos_t, z_exit=z_exit, rho=d_rho,
-                        nodes=nodes, h=h, flux=flux.clone(), jac=jac.clone(), jrem=jrem.clone()),
+                        ne=d_ne, nw=d_nw, nq=d_nq, nodes=nodes, h=h, flux=flux.clone(),
+                        jac=jac.clone(), jrem=jrem.clone()),
     }
 
 

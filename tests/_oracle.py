@@ -751,3 +751,23 @@ def polar(proc, m, ki, kf, Z=11.0, seed=0, n=1, script=None):
     used = lib().orc_polar(proc, m, ki, kf, Z, seed, nn, P(sc), 0 if sc is None else len(sc),
                            P(out))
     return (out[0], used) if sc is not None else out
+
+
+_SIGS["orc_cube_fd_ld"] = (None, _TAB + [C.c_int, C.c_int, C.c_int, i_p, i_p, d_p, d_p, d_p,
+                                         C.c_double, C.c_double, C.c_int, d_p, C.c_int, d_p, d_p,
+                                         d_p, C.c_double, C.c_int, C.c_double, C.c_int, d_p])
+_lib = None
+
+
+def cube_fd_ld(ph, geo, rp, cells, lens, cos_t, z_exit, dens, ne, nw, nq, voxel, h, scale=1.0,
+               palt=3e4, threads=0):
+    """Central difference of the summed flux of the given rays with respect
+    to the density of `voxel`, evaluated in long double.  Returns (d, F)."""
+    out = np.zeros(2)
+    rp, cells, lens = i32(rp), i32(cells), f64(lens)
+    lib().orc_cube_fd_ld(*tabargs(ph), ph.mass, ph.ctau, geo.rock_material, geo.air_material,
+                         len(rp) - 1, P(rp, C.c_int32), P(cells, C.c_int32), P(lens),
+                         P(f64(cos_t)), P(f64(z_exit)), geo.z_max, palt, len(dens), P(f64(dens)),
+                         len(ne), P(f64(ne)), P(f64(nw)), P(f64(nq)), scale, int(voxel), h,
+                         threads, P(out))
+    return out[0], out[1]

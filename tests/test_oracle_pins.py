@@ -144,3 +144,26 @@ def test_flux_model_identities():
         h = E * 1e-6
         fd = (O.flux_gccly(0.6, E + h, 1.0) - O.flux_gccly(0.6, E - h, 1.0)) / (2 * h)
         assert O.lib().orc_dflux_gccly_dk(0.6, E, 1.0) == pytest.approx(fd, rel=1e-8)
+
+
+def test_long_double_central_difference_equals_forward_mode():
+    """orc_cube_fd_ld (the extended-precision central difference the full-size
+    adjoint test uses) against the oracle's forward-mode derivative of the
+    same program: agreement to 1e-10, while the FP64 difference at the same h
+    is only good to ~1e-8."""
+    from workloads import cube_rays, energy_nodes, uniform_densities
+    ph = dp.create_physics_csda(("StandardRock", "Air"))
+    geo = dp.CubeGeometry(80.0, 12, 10, 8, ph.get_material_index("StandardRock"),
+                          ph.get_material_index("Air"), width=120.0)
+    sxy, dirs, cos_t = cube_rays(geo, 2, 8, seed=4)
+    rp, cells, lens, ze = O.trace_cube(geo, sxy, dirs)
+    rho = uniform_densities(geo.n_cells, 3)
+    ne, nw, nq = energy_nodes(128, 1.0, 1e4)
+    flux, grads = O.cube_csda(ph, geo, rp, cells, lens, cos_t, ze, rho, ne, nw, nq)
+    for v in sorted(set(int(c) for c in cells[::7]))[:6]:
+        if v == 0:
+            continue
+        G = sum(g.get(v, 0.0) for g in grads)
+        d, F = O.cube_fd_ld(ph, geo, rp, cells, lens, cos_t, ze, rho, ne, nw, nq, v, 1e-3)
+        assert F == pytest.approx(flux.sum(), rel=1e-14)
+        assert d == pytest.approx(G, rel=1e-10)
