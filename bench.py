@@ -21,8 +21,10 @@ Legs
             model) over 4096 bins of a 64x64x32 grid, 32 energy nodes.
             Unit: rays*voxels/s.
   pathmc  : fixed-path straggled MC over all 4096 bins x 1e4 events of a
-            lognormal density field with a cavity (4.1e7 events).  Unit:
-            events/s (kernel time).
+            lognormal density field with a cavity (4.1e7 events), plus each
+            bin's deterministic CSDA flux over 256 charge-summed nodes, all in
+            one persistent launch.  Unit: MC events/s (kernel time of the
+            launch, CSDA items included).
   direct  : direct CSDA flux + density tangent for 2^24 samples.  Unit: samples/s.
   wmodel_corrected : the wmodel leg with the MC-calibrated corrections on.
   normal  : Gauss-Newton normal equations of the wmodel Jacobian over its
@@ -799,13 +801,14 @@ def run_pathmc(dp, physics, ctx, n_rays=4096, n_events=10000, steps=2, warmup=1)
                           physics.get_material_index("Water"),
                           physics.get_material_index("Air"), -1, 2650.0, 1000.0)
     site = dp.SiteConfig(1000.0, 3e4)
+    nodes = dp.csda_energy_quadrature(256, 1.0, 1e4)
     args = (physics, m, site, paths, [((m.rock_material,), (1.0,))],
-            np.zeros(len(rho), np.int32), rho, s, 1)
+            np.zeros(len(rho), np.int32), rho, s, nodes, 1)
     for _ in range(warmup):
-        dp.compute_directional_flux_mc(*args, ctx=ctx)
+        dp.compute_directional_flux_mc_and_csda(*args, ctx=ctx)
     kms, t0 = 0.0, time.perf_counter()
     for _ in range(steps):
-        f, sg = dp.compute_directional_flux_mc(*args, ctx=ctx)
+        f, sg, cf = dp.compute_directional_flux_mc_and_csda(*args, ctx=ctx)
         kms += ctx.last_kernel_ms()
     t1 = time.perf_counter()
     kern = kms / steps
@@ -814,6 +817,7 @@ def run_pathmc(dp, physics, ctx, n_rays=4096, n_events=10000, steps=2, warmup=1)
             "wall_ms_incl_staging": (t1 - t0) * 1e3 / steps, "rays": n_rays,
             "events_per_ray": n_events, "steps": steps,
             "segments": paths.n_segments, "mean_flux": float(np.nanmean(f)),
+            "csda_nodes_per_ray": len(nodes.energy), "mean_csda_flux": float(np.nanmean(cf)),
             "roofline": {"bound": "hbm", "achieved": (16.0 * ev + 12.0 * paths.n_segments) /
                          (kern * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": (16.0 * ev + 12.0 * paths.n_segments) / (kern * 1e-3) / 1e9 /
@@ -823,12 +827,13 @@ def run_pathmc(dp, physics, ctx, n_rays=4096, n_events=10000, steps=2, warmup=1)
 
 
 def run_pathmc_per_ray(dp, physics, ctx, pm, n_sub=64):
-    """The plain per-ray GPU launch loop for the fixed-path MC: one
-    compute_directional_flux_mc launch per ray with its 1e4 events (the
-    reference's loop over bins), on n_sub rays spread over the 4096; device
-    time of the launches (staging excluded), scaled to all rays."""
+    """The plain per-ray GPU launch loop for the fixed-path MC: one launch per
+    ray with its 1e4 events and 256 CSDA nodes (the reference's loop over
+    bins), on n_sub rays spread over the 4096; device time of the launches
+    (staging excluded), scaled to all rays."""
     import numpy as np
     paths, rho, s, m, site = pm["_problem"]
+    nodes = dp.csda_energy_quadrature(256, 1.0, 1e4)
     idx = np.linspace(0, paths.n_rays - 1, n_sub).astype(int)
     kms = 0.0
     t0 = time.perf_counter()
@@ -837,8 +842,10 @@ def run_pathmc_per_ray(dp, physics, ctx, pm, n_sub=64):
         one = dp.DirectionalPaths([0, b - a], paths.cells[a:b], paths.lengths[a:b],
                                   paths.zenith[r:r + 1], paths.azimuth[r:r + 1],
                                   paths.remaining_rock[r:r + 1], paths.exit_z[r:r + 1])
-        dp.compute_directional_flux_mc(physics, m, site, one, [((m.rock_material,), (1.0,))],
-                                       np.zeros(len(rho), np.int32), rho, s, 1, ctx=ctx)
+        dp.compute_directional_flux_mc_and_csda(physics, m, site, one,
+                                                [((m.rock_material,), (1.0,))],
+                                                np.zeros(len(rho), np.int32), rho, s, nodes, 1,
+                                                ctx=ctx)
         kms += ctx.last_kernel_ms()
     wall = (time.perf_counter() - t0) * 1e3
     scale = paths.n_rays / n_sub
@@ -1038,7 +1045,8 @@ def main():
                                     "voxels of 10 m, w U[0, 0.3], 4 detectors x 1024 "
                                     "directions, 32 nodes 1-1e4 GeV"),
         "pathmc": dict(pm, workload="fixed-path straggled MC, 64x64x32 lognormal density "
-                                    "field with a 1800 kg/m3 cavity, 4096 bins x 1e4 events"),
+                                    "field with a 1800 kg/m3 cavity, 4096 bins x (1e4 events + the "
+                                    "CSDA flux over 256 nodes) in one persistent launch"),
         "direct": dict(dr, workload="direct CSDA flux + d/d rho (compute_flux_gradient_csda), "
                                     "2^24 samples: rock 10-800 m at 2000-3000 kg/m3, "
                                     "elevation 10-90 deg, E 1-1e4 GeV"),

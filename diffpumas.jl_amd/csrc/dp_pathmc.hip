@@ -56,7 +56,40 @@ struct PathArgs {
   int* queue;
   int* err;
   unsigned long long* bad;
+  // Second sample set of the same launch: deterministic CSDA nodes (charge
+  // -summed allowed), items after the MC items.  n_nodes = 0: none.
+  int n_nodes;
+  const double* ne;
+  const double* nw;
+  const double* nq;
+  const uint32_t* order_n;
+  int chunks_n;
+  int64_t n_items_mc;  // items [0, n_items_mc) of this launch are MC chunks
+  double* part_n;      // 2 per CSDA item (indexed from path r0)
+  double* w_local_n;   // per CSDA node of this launch
 };
+
+// The sample set an item of this launch draws from.
+struct ItemSet {
+  bool csda;           // deterministic CSDA steps (no straggling, no draws)
+  int n, chunks;
+  const double *e, *w, *q;
+  const uint32_t* order;
+  double* w_local;
+  int64_t it;          // item index within its set
+};
+__device__ __forceinline__ ItemSet item_set(const PathArgs& a, int64_t item) {
+  ItemSet S;
+  S.csda = item >= a.n_items_mc;
+  if (S.csda) {
+    S.n = a.n_nodes; S.chunks = a.chunks_n; S.e = a.ne; S.w = a.nw; S.q = a.nq;
+    S.order = a.order_n; S.w_local = a.w_local_n; S.it = item - a.n_items_mc;
+  } else {
+    S.n = a.n_samples; S.chunks = a.chunks; S.e = a.se; S.w = a.sw; S.q = a.sq;
+    S.order = a.order; S.w_local = a.w_local; S.it = item;
+  }
+  return S;
+}
 
 // propagate_mc_uniform_segment / propagate_mc_air_segment, src/Tomography.jl:
 // 876-969 (scattering off).  _mc_segment_step (:873-874) bounds the step by
@@ -65,7 +98,7 @@ struct PathArgs {
 template <bool FAST>
 __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, double density,
                              bool air, double cos_t, double distance, Philox& rng, int& err,
-                             int& g, DelLog* dlog) {
+                             int& g, DelLog* dlog, bool straggling) {
   double remaining = distance;
   const double elimit = a.elimit;
   int it = 0;
@@ -93,17 +126,17 @@ __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, d
     const double step = fmin(remaining, fmax(1e-6, fmin(remaining + 1e-7, max_step)));
     const double oldL = st.L, oldE = st.E;
     if (FAST) {
-      if (st.E < a.thr_low - EPS64 && a.straggling)
+      if (st.E < a.thr_low - EPS64 && straggling)
         straggled_step_f(a.T, md, st, L, 0.0, dens, step, false, cur, rng, err, &g, dlog);
-      else if (a.straggling)
+      else if (straggling)
         mixed_step_f(a.T, md, st, L, dens, step, cur, rng, err, dlog);
       else
         csda_mix_step_f(a.T, md, st, L, dens, step, &g);
     } else if (st.E < a.thr_low - EPS64) {
-      if (a.straggling) straggled_step(a.T, md, st, dens, step, false, cur, rng, err);
+      if (straggling) straggled_step(a.T, md, st, dens, step, false, cur, rng, err);
       else csda_mix_step(a.T, md, st, dens, step);
     } else {
-      if (a.straggling) mixed_step(a.T, md, st, dens, step, cur, rng, err);
+      if (straggling) mixed_step(a.T, md, st, dens, step, cur, rng, err);
       else csda_mix_step(a.T, md, st, dens, step);
     }
     if (err) return;
@@ -133,53 +166,56 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
     __syncthreads();
     const int64_t item = s_item;
     if (item >= a.n_items) break;
-    const int r = a.r0 + (int)(item / a.chunks);
-    const int sl = (int)(item % a.chunks) * PM_THREADS + tid;
+    const ItemSet S = item_set(a, item);
+    const bool strag = a.straggling && !S.csda;
+    const int r = a.r0 + (int)(S.it / S.chunks);
+    const int sl = (int)(S.it % S.chunks) * PM_THREADS + tid;
     const double ct = a.cos_t[r];
     const bool live = (a.valid ? a.valid[r] != 0 : true) && ct > 0.0;
-    const bool ran = live && sl < a.n_samples;
-    if (!live && sl < a.n_samples) {
-      const int64_t el = (int64_t)(r - a.r0) * a.n_samples + (int)a.order[sl];
-      a.w_local[el] = 0.0;
+    const bool ran = live && sl < S.n;
+    if (!live && sl < S.n) {
+      const int64_t el = (int64_t)(r - a.r0) * S.n + (int)S.order[sl];
+      S.w_local[el] = 0.0;
     }
     double wi = 0.0;
     int err = 0;
     int64_t e = 0, el = 0;
     DelLog dlg{pool, &pool_n, PM_POOL, tid, 0, 0, nullptr, a.media, a.air_medium + 1};
     if (ran) {
-      const int s = (int)a.order[sl];
-      e = (int64_t)r * a.n_samples + s;
-      el = (int64_t)(r - a.r0) * a.n_samples + s;  // event index in this launch
+      const int s = (int)S.order[sl];
+      e = (int64_t)r * S.n + s;
+      el = (int64_t)(r - a.r0) * S.n + s;  // sample index in this launch's set
       Philox rng;
       rng.init(a.seed, (uint64_t)e);
       McState st;
-      st.E = a.se[s]; st.w = 1.0; st.t = 0.0; st.L = 0.0;
+      st.E = S.e[s]; st.w = 1.0; st.t = 0.0; st.L = 0.0;
       st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
       st.dir[0] = -sqrt(fmax(0.0, 1.0 - ct * ct)); st.dir[1] = 0.0; st.dir[2] = -ct;
       int g = a.T.K / 2;
-      DelLog* dl = (FAST && a.straggling) ? &dlg : nullptr;
+      DelLog* dl = (FAST && strag) ? &dlg : nullptr;
       const int b0 = a.row_ptr[r], ns = a.row_ptr[r + 1] - b0;
       for (int k = 0; k < ns; ++k) {
         const int c = a.cell[b0 + k];
         dlg.medium = a.cell_medium[c];
         path_segment<FAST>(a, a.media[dlg.medium], st, a.cell_density[c], false, ct,
-                           a.len[b0 + k], rng, err, g, dl);
+                           a.len[b0 + k], rng, err, g, dl, strag);
         if (err || st.w <= 0.0 || !isfinite(st.w)) break;
       }
       if (!err && st.w > 0.0 && isfinite(st.w) && a.remaining[r] > 1e-8) {
         dlg.medium = a.rock_medium;
         path_segment<FAST>(a, a.media[a.rock_medium], st, a.rho_rock, false, ct, a.remaining[r],
-                           rng, err, g, dl);
+                           rng, err, g, dl, strag);
       }
       if (!err && st.w > 0.0 && isfinite(st.w)) {
         const double air_d = fmax(0.0, a.palt - a.exit_z[r]) / fmax(ct, 1e-3);
         dlg.medium = a.air_medium;
-        path_segment<FAST>(a, a.media[a.air_medium], st, 0.0, true, ct, air_d, rng, err, g, dl);
+        path_segment<FAST>(a, a.media[a.air_medium], st, 0.0, true, ct, air_d, rng, err, g, dl,
+                           strag);
       }
       double f = 0.0;
       if (!err && st.w > 0.0 && isfinite(st.w) && st.E < a.elimit)
-        f = st.w * flux_gccly(ct, st.E, a.sq[s]);
-      wi = err ? 0.0 : 2.0 * a.sw[s] * f;
+        f = st.w * flux_gccly(ct, st.E, S.q[s]);
+      wi = err ? 0.0 : 2.0 * S.w[s] * f;
     }
     int m = 0;
     if (FAST) {
@@ -196,7 +232,7 @@ __global__ __launch_bounds__(PM_THREADS, 4) void path_mc_kernel(PathArgs a) {
         atomicMin(a.bad, (unsigned long long)e);
         wi = 0.0;
       }
-      a.w_local[el] = wi;
+      S.w_local[el] = wi;
     }
     __syncthreads();  // every thread has read s_item and the pool before thread 0 resets them
   }
@@ -208,14 +244,15 @@ __global__ __launch_bounds__(PM_THREADS) void path_items_kernel(PathArgs a) {
   __shared__ double s_sum[2][PM_THREADS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int64_t item = blockIdx.x; item < a.n_items; item += gridDim.x) {
-    const int r = a.r0 + (int)(item / a.chunks);
-    const int sl = (int)(item % a.chunks) * PM_THREADS + tid;
+    const ItemSet S = item_set(a, item);
+    const int r = a.r0 + (int)(S.it / S.chunks);
+    const int sl = (int)(S.it % S.chunks) * PM_THREADS + tid;
     double wi = 0.0;
-    if (sl < a.n_samples) {
-      const int s = (int)a.order[sl];
-      const int64_t el = (int64_t)(r - a.r0) * a.n_samples + s;
-      wi = a.w_local[el];
-      if (a.events) a.events[(int64_t)r * a.n_samples + s] = wi;
+    if (sl < S.n) {
+      const int s = (int)S.order[sl];
+      const int64_t el = (int64_t)(r - a.r0) * S.n + s;
+      wi = S.w_local[el];
+      if (a.events && !S.csda) a.events[(int64_t)r * S.n + s] = wi;
     }
     const double s1 = wave_sum(wi), s2 = wave_sum(wi * wi);
     if (lane == 0) { s_sum[0][wave] = s1; s_sum[1][wave] = s2; }
@@ -223,9 +260,10 @@ __global__ __launch_bounds__(PM_THREADS) void path_items_kernel(PathArgs a) {
     if (tid == 0) {
       double t1 = 0.0, t2 = 0.0;
       for (int j = 0; j < PM_THREADS / 64; ++j) { t1 += s_sum[0][j]; t2 += s_sum[1][j]; }
-      const int64_t gi = (int64_t)a.r0 * a.chunks + item;
-      a.part[2 * gi] = t1;
-      a.part[2 * gi + 1] = t2;
+      const int64_t gi = (int64_t)a.r0 * S.chunks + S.it;
+      double* part = S.csda ? a.part_n : a.part;
+      part[2 * gi] = t1;
+      part[2 * gi + 1] = t2;
     }
     __syncthreads();
   }
@@ -262,21 +300,26 @@ __global__ void reset_queue(int* q) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
-                      const dp_site_config* site, const dp_directional_paths* p, int32_t n_cells,
-                      int32_t n_media, const dp_mixture* media, const int32_t* cell_medium,
-                      const double* cell_density, const dp_energy_samples* es, uint64_t seed,
-                      int32_t straggling, double thr_low, double* flux, double* sigma,
-                      double* event_weights) {
-  const std::string fn = "dp_mc_fixed_paths";
+// The fixed-path MC of every path for the samples `es` and, when `nodes` is
+// given, the deterministic CSDA flux of every path over those nodes, in one
+// persistent launch per batch of paths (MC items first, then CSDA items).
+int run_paths(const std::string& fn, dp_ctx c, dp_tables t, const dp_material_config* mat,
+              const dp_site_config* site, const dp_directional_paths* p, int32_t n_cells,
+              int32_t n_media, const dp_mixture* media, const int32_t* cell_medium,
+              const double* cell_density, const dp_energy_samples* es,
+              const dp_energy_samples* nodes, uint64_t seed, int32_t straggling, double thr_low,
+              double* flux, double* sigma, double* event_weights, double* csda_flux) {
   if (!c || !t || !mat || !site || !p || !media || !cell_medium || !cell_density || !es || !flux ||
-      !sigma)
+      !sigma || (nodes && !csda_flux))
     return dph::fail(DP_E_ARGUMENT, fn + ": null argument");
   const int M = t->n_materials, nr = p->n_rays, ns = p->n_segments, nn = es->n;
-  if (nr < 0 || ns < 0 || n_cells <= 0 || n_media <= 0 || nn <= 0)
+  const int nq = nodes ? nodes->n : 0;
+  if (nr < 0 || ns < 0 || n_cells <= 0 || n_media <= 0 || nn <= 0 || (nodes && nq <= 0))
     return dph::fail(DP_E_ARGUMENT, fn + ": sizes must be positive");
+  for (int i = 0; i < nq; ++i)
+    if (!(nodes->energy[i] > 0)) return dph::fail(DP_E_ARGUMENT, fn + ": node energies must be positive");
   if (mat->rock_material < 0 || mat->rock_material >= M || mat->air_material < 0 ||
       mat->air_material >= M || !(mat->rock_density > 0))
     return dph::fail(DP_E_ARGUMENT, fn + ": invalid rock/air material or rock density");
@@ -318,6 +361,7 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   if ((rc = dph::make_medium(t, {mat->rock_material}, {1.0}, hm[n_media], kern))) return rc;
   if ((rc = dph::make_medium(t, {mat->air_material}, {1.0}, hm[n_media + 1], kern))) return rc;
   const int chunks = (nn + PM_THREADS - 1) / PM_THREADS;
+  const int chunks_n = (nq + PM_THREADS - 1) / PM_THREADS;
   const int64_t n_items = (int64_t)nr * chunks;
   const int64_t n_ev = (int64_t)nr * nn;
   size_t o = 0;
@@ -329,6 +373,9 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
                o_zex = take(8 * (size_t)nr), o_val = take((size_t)nr),
                o_se = take(8 * (size_t)nn), o_sw = take(8 * (size_t)nn), o_sq = take(8 * (size_t)nn),
                o_ord = take(4 * (size_t)nn),
+               o_ne = take(8 * (size_t)nq), o_nw = take(8 * (size_t)nq), o_nq = take(8 * (size_t)nq),
+               o_ordn = take(4 * (size_t)nq), o_partn = take(16 * (size_t)nr * chunks_n),
+               o_fluxn = take(8 * (size_t)nr), o_sign = take(8 * (size_t)nr),
                o_part = take(16 * (size_t)n_items), o_flux = take(8 * (size_t)nr),
                o_sig = take(8 * (size_t)nr), o_ev = take(event_weights ? 8 * (size_t)n_ev : 8),
                o_q = take(256);
@@ -357,6 +404,16 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   std::stable_sort(ord.begin(), ord.end(),
                    [&](uint32_t x, uint32_t y) { return es->energy[x] < es->energy[y]; });
   DP_HIP(hipMemcpyAsync(b + o_ord, ord.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, st));
+  std::vector<uint32_t> ordn(nq);
+  if (nq) {
+    DP_HIP(hipMemcpyAsync(b + o_ne, nodes->energy, 8 * (size_t)nq, hipMemcpyHostToDevice, st));
+    DP_HIP(hipMemcpyAsync(b + o_nw, nodes->weight, 8 * (size_t)nq, hipMemcpyHostToDevice, st));
+    DP_HIP(hipMemcpyAsync(b + o_nq, nodes->charge, 8 * (size_t)nq, hipMemcpyHostToDevice, st));
+    for (int i = 0; i < nq; ++i) ordn[i] = (uint32_t)i;
+    std::stable_sort(ordn.begin(), ordn.end(),
+                     [&](uint32_t x, uint32_t y) { return nodes->energy[x] < nodes->energy[y]; });
+    DP_HIP(hipMemcpyAsync(b + o_ordn, ordn.data(), 4 * (size_t)nq, hipMemcpyHostToDevice, st));
+  }
   double supported = t->last_energy[0];
   for (double e : t->last_energy) supported = e < supported ? e : supported;
   PathArgs a;
@@ -377,6 +434,10 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   a.sq = (const double*)(b + o_sq);
   a.order = (const uint32_t*)(b + o_ord);
   a.seed = seed; a.chunks = chunks; a.n_items = n_items; a.r0 = 0;
+  a.n_nodes = nq; a.chunks_n = chunks_n;
+  a.ne = (const double*)(b + o_ne); a.nw = (const double*)(b + o_nw); a.nq = (const double*)(b + o_nq);
+  a.order_n = (const uint32_t*)(b + o_ordn);
+  a.part_n = (double*)(b + o_partn);
   a.part = (double*)(b + o_part);
   a.events = event_weights ? (double*)(b + o_ev) : nullptr;
   int* q = (int*)(b + o_q);
@@ -391,17 +452,20 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   // Paths are processed in batches of at most 2^22 events, which bounds the
   // per-event weight buffer.
   const int64_t batch_ev = (int64_t)1 << 22;
-  int rays_per_batch = (int)std::max<int64_t>(1, batch_ev / nn);
+  int rays_per_batch = (int)std::max<int64_t>(1, batch_ev / (nn + nq));
   if (rays_per_batch > nr) rays_per_batch = nr;
   const int64_t ev_batch = (int64_t)rays_per_batch * nn;
-  rc = dph::ensure(c->scratch, 8 * (size_t)ev_batch + 256);
+  const size_t o_wn = dph::align_up(8 * (size_t)ev_batch, 256);
+  rc = dph::ensure(c->scratch, o_wn + 8 * (size_t)rays_per_batch * nq + 256);
   if (rc) return rc;
   a.w_local = (double*)c->scratch.ptr;
+  a.w_local_n = (double*)((char*)c->scratch.ptr + o_wn);
   dph::record(c, c->ev0);
   for (int r0 = 0; r0 < nr; r0 += rays_per_batch) {
     const int r1 = std::min(nr, r0 + rays_per_batch);
     a.r0 = r0;
-    a.n_items = (int64_t)(r1 - r0) * chunks;
+    a.n_items_mc = (int64_t)(r1 - r0) * chunks;
+    a.n_items = a.n_items_mc + (int64_t)(r1 - r0) * chunks_n;
     int64_t grid = (int64_t)c->n_cu * occ;
     if (grid > a.n_items) grid = a.n_items;
     hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, st, q);
@@ -415,11 +479,16 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
   dph::record(c, c->ev1);
   hipLaunchKernelGGL(path_reduce, dim3((nr + 255) / 256), dim3(256), 0, st, a.part, nr, chunks,
                      nn, a.cos_t, a.valid, (double*)(b + o_flux), (double*)(b + o_sig));
+  if (nq)
+    hipLaunchKernelGGL(path_reduce, dim3((nr + 255) / 256), dim3(256), 0, st, a.part_n, nr,
+                       chunks_n, nq, a.cos_t, a.valid, (double*)(b + o_fluxn),
+                       (double*)(b + o_sign));
   DP_HIP(hipGetLastError());
   int herr[2] = {0, 0};
   unsigned long long hb = 0;
   DP_HIP(hipMemcpyAsync(flux, b + o_flux, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));
   DP_HIP(hipMemcpyAsync(sigma, b + o_sig, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));
+  if (nq) DP_HIP(hipMemcpyAsync(csda_flux, b + o_fluxn, 8 * (size_t)nr, hipMemcpyDeviceToHost, st));
   if (event_weights)
     DP_HIP(hipMemcpyAsync(event_weights, b + o_ev, 8 * (size_t)n_ev, hipMemcpyDeviceToHost, st));
   DP_HIP(hipMemcpyAsync(herr, a.err, 8, hipMemcpyDeviceToHost, st));
@@ -433,6 +502,34 @@ int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
     return dph::fail(DP_E_EXHAUSTED, fn + ": MC segment made no spatial progress or hit a cap (first failing event " +
                                          std::to_string(hb) + ")");
   return DP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dp_mc_fixed_paths(dp_ctx c, dp_tables t, const dp_material_config* mat,
+                      const dp_site_config* site, const dp_directional_paths* p, int32_t n_cells,
+                      int32_t n_media, const dp_mixture* media, const int32_t* cell_medium,
+                      const double* cell_density, const dp_energy_samples* es, uint64_t seed,
+                      int32_t straggling, double thr_low, double* flux, double* sigma,
+                      double* event_weights) {
+  return run_paths("dp_mc_fixed_paths", c, t, mat, site, p, n_cells, n_media, media, cell_medium,
+                   cell_density, es, nullptr, seed, straggling, thr_low, flux, sigma,
+                   event_weights, nullptr);
+}
+
+int dp_mc_fixed_paths_csda(dp_ctx c, dp_tables t, const dp_material_config* mat,
+                           const dp_site_config* site, const dp_directional_paths* p,
+                           int32_t n_cells, int32_t n_media, const dp_mixture* media,
+                           const int32_t* cell_medium, const double* cell_density,
+                           const dp_energy_samples* samples, const dp_energy_samples* nodes,
+                           uint64_t seed, double thr_low, double* flux, double* sigma,
+                           double* csda_flux) {
+  if (!nodes) return dph::fail(DP_E_ARGUMENT, "dp_mc_fixed_paths_csda: null nodes");
+  return run_paths("dp_mc_fixed_paths_csda", c, t, mat, site, p, n_cells, n_media, media,
+                   cell_medium, cell_density, samples, nodes, seed, 1, thr_low, flux, sigma,
+                   nullptr, csda_flux);
 }
 
 }  // extern "C"

@@ -22,7 +22,8 @@ from .tomography import (DirectionalPaths, EnergySamples, MaterialConfig, SiteCo
                          direction_vector, directional_flux_and_grad_csda,
                          flux_and_segment_jacobian, sample_energy_set, trace_voxel_paths,
                          mc_cell_mixture_and_density, build_cell_properties_for_mc,
-                         compute_directional_flux_mc, CsdaCorrection, CsdaMaterialCorrection,
+                         compute_directional_flux_mc, compute_directional_flux_mc_and_csda,
+                         CsdaCorrection, CsdaMaterialCorrection,
                          set_csda_correction, get_csda_correction,
                          set_csda_material_correction, get_csda_material_correction)
 
@@ -40,6 +41,7 @@ __all__ = [
     "set_csda_material_correction", "get_csda_material_correction",
     "flux_and_segment_jacobian", "sample_energy_set", "trace_voxel_paths",
     "mc_cell_mixture_and_density", "build_cell_properties_for_mc", "compute_directional_flux_mc",
+    "compute_directional_flux_mc_and_csda",
     "make_csda_operator", "normal_equations", "calibrate_csda_correction", "path_rock_length",
     "TetraVolume", "build_tetra_neighbors", "trace_tetra_paths", "trace_tetra_rays",
 ]

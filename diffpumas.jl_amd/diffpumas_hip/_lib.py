@@ -216,6 +216,11 @@ _SIGS = {
                                     C.c_int32, C.POINTER(Mixture), i32_p, dbl_p,
                                     C.POINTER(EnergySamples), C.c_uint64, C.c_int32, C.c_double,
                                     dbl_p, dbl_p, dbl_p]),
+    "dp_mc_fixed_paths_csda": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(MaterialConfig),
+                                         C.POINTER(SiteConfig), C.POINTER(DirectionalPaths),
+                                         C.c_int32, C.c_int32, C.POINTER(Mixture), i32_p, dbl_p,
+                                         C.POINTER(EnergySamples), C.POINTER(EnergySamples),
+                                         C.c_uint64, C.c_double, dbl_p, dbl_p, dbl_p]),
     "dp_ray_surface_exit": (C.c_int, [C.c_void_p, C.POINTER(ElevationMap), C.c_double,
                                       C.c_double, C.c_int32, C.POINTER(C.c_double), C.c_double,
                                       C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint8),

@@ -389,3 +389,42 @@ def compute_directional_flux_mc(physics, matcfg, site, paths, media, cell_medium
     if return_events:
         return flux, sigma, ev.reshape(paths.n_rays, len(ne))
     return flux, sigma
+
+
+def compute_directional_flux_mc_and_csda(physics, matcfg, site, paths, media, cell_medium,
+                                         cell_density, samples, nodes, seed,
+                                         energy_threshold_low=100.0, ctx=None):
+    """In one persistent launch: the straggled fixed-path MC of every path for
+    `samples` (compute_directional_flux_mc, src/Tomography.jl:979-1041) and
+    the deterministic CSDA flux of every path over the quadrature `nodes`
+    (the same stepping with straggling off; charge-summed nodes allowed).
+    Returns (mc_flux, mc_sigma, csda_flux), each n_rays long."""
+    ctx = ctx or _lib.default_context()
+    h = physics.device_handle(ctx)
+    mix = (_lib.Mixture * len(media))()
+    for j, (mats, fr) in enumerate(media):
+        if len(mats) > 4:
+            raise ArgumentError("at most 4 components per medium")
+        mix[j].n_components = len(mats)
+        for k, (m, f) in enumerate(zip(mats, fr)):
+            mix[j].components[k] = int(m)
+            mix[j].fractions[k] = float(f)
+    cm = i32(cell_medium)
+    cd = f64(cell_density)
+    se, sw, sq = f64(samples.energy), f64(samples.weight), f64(samples.charge)
+    ne, nw, nq = f64(nodes.energy), f64(nodes.weight), f64(nodes.charge)
+    es = _lib.EnergySamples(len(se), ptr(se), ptr(sw), ptr(sq))
+    en = _lib.EnergySamples(len(ne), ptr(ne), ptr(nw), ptr(nq))
+    mc = matcfg.c()
+    sc = _lib.SiteConfig(float(site.detector_elevation), float(site.primary_altitude))
+    cp = paths.c_paths()
+    flux = np.zeros(paths.n_rays)
+    sigma = np.zeros(paths.n_rays)
+    cflux = np.zeros(paths.n_rays)
+    _lib.check(ctx.lib.dp_mc_fixed_paths_csda(ctx.h, h, C.byref(mc), C.byref(sc), C.byref(cp),
+                                              len(cm), len(media), mix, ptr(cm, C.c_int32),
+                                              ptr(cd), C.byref(es), C.byref(en),
+                                              int(seed) & (2**64 - 1),
+                                              float(energy_threshold_low), ptr(flux),
+                                              ptr(sigma), ptr(cflux)))
+    return flux, sigma, cflux

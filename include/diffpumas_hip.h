@@ -481,6 +481,23 @@ int dp_mc_fixed_paths(dp_ctx ctx, dp_tables tables, const dp_material_config* ma
                       double energy_threshold_low, double* flux, double* sigma,
                       double* event_weights);
 
+/* Replaces, in one persistent launch: compute_directional_flux_mc
+ * (src/Tomography.jl:979-1041) of every path for the MC samples, as
+ * dp_mc_fixed_paths with straggling on (flux, sigma: the same values bit for
+ * bit), and the deterministic CSDA flux of every path over the quadrature
+ * `nodes` (charge-summed nodes allowed): the same fixed-path stepping with
+ * straggling off, csda_flux[r] = sum_n 2 w_n flux_n / n_nodes (NaN for an
+ * invalid path).  The work queue holds both kinds of items (a path's chunks
+ * of samples, then of nodes), so uneven path lengths and the two workloads
+ * balance across the chip together.                                         */
+int dp_mc_fixed_paths_csda(dp_ctx ctx, dp_tables tables, const dp_material_config* mat,
+                           const dp_site_config* site, const dp_directional_paths* paths,
+                           int32_t n_cells, int32_t n_media, const dp_mixture* media,
+                           const int32_t* cell_medium, const double* cell_density,
+                           const dp_energy_samples* samples, const dp_energy_samples* nodes,
+                           uint64_t seed, double thr_low, double* flux, double* sigma,
+                           double* csda_flux);
+
 /* ---------------------------------------------- tetrahedral traversal
  * TetraVolume (examples/lvd_tomography.jl:172-179) as plain arrays, 0-based.
  * Face f of a tet is the triangle without its vertex 3 - f, i.e. the faces

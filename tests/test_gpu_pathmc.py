@@ -114,3 +114,29 @@ def test_rejects_quadrature_nodes_and_scattering(ph3, ctx):
         dp.compute_directional_flux_mc(ph3, m, SiteConfig(1000.0, 12000.0), paths, [((0,), (1.0,))],
                                        [0], [2650.0], dp.sample_energy_set(4, 1.0, 10.0, 1), 1,
                                        scattering=True, ctx=ctx)
+
+
+def test_mc_and_csda_in_one_launch(ph3, ctx):
+    """dp_mc_fixed_paths_csda: a path's MC samples and its CSDA quadrature
+    nodes share one persistent launch.  The MC part equals a separate
+    dp_mc_fixed_paths run bit for bit; the CSDA part (charge-summed nodes,
+    straggling off) equals the oracle's deterministic walk to 1e-10."""
+    paths, n_cells = _traced(ctx, 24, 21)
+    paths.valid[3] = 0
+    m, media, cm, cd = _setup(ph3, n_cells, 22)
+    s = dp.sample_energy_set(700, 1.0, 1e4, 23)
+    nodes = dp.csda_energy_quadrature(96, 1.0, 1e4)
+    site = SiteConfig(600.0, 3e4)
+    _say(f"combined launch: {paths.n_rays} paths, {len(s.energy)} samples + "
+         f"{len(nodes.energy)} nodes per path")
+    f, sg, cf = dp.compute_directional_flux_mc_and_csda(ph3, m, site, paths, media, cm, cd, s,
+                                                        nodes, 31, ctx=ctx)
+    f2, sg2 = dp.compute_directional_flux_mc(ph3, m, site, paths, media, cm, cd, s, 31, ctx=ctx)
+    assert np.array_equal(f, f2, equal_nan=True) and np.array_equal(sg, sg2, equal_nan=True)
+    fo, _ = O.mc_paths(ph3, media, cm, cd, 0, 2, 2650.0, (600.0, 3e4), _oracle_paths(paths),
+                       (nodes.energy, nodes.weight, nodes.charge), 1, straggling=False)
+    ok = paths.valid.astype(bool)
+    assert np.all(np.isnan(cf[~ok]))
+    np.testing.assert_allclose(cf[ok], fo[ok], rtol=1e-10)
+    _say(f"  csda flux range {cf[ok].min():.4g}..{cf[ok].max():.4g}; mc/csda mean ratio "
+         f"{np.nanmean(f / cf):.4f}")
