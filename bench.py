@@ -965,6 +965,9 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     ctx = dp.Context(0)
+    # The first (warm-up) call of each tuned kernel times its candidate launch
+    # configurations; the timed steps run the fastest.
+    ctx.set_autotune(True)
     physics = dp.create_physics(("StandardRock", "Water", "Air"))
     mc = run_mc(dp, physics, ctx, torch, a.mc_events, a.steps, a.warmup)
     log(f"mc(straggled): {mc['value']:.4g} events/s, kernel {mc['kernel_ms']:.3f} ms, "
@@ -972,10 +975,12 @@ def main():
     adj = run_adjoint(dp, L, physics, ctx, torch, a.steps, a.warmup)
     log(f"adjoint: {adj['value']:.4g} rays*voxels/s, {adj['ms_per_step']:.3f} ms/step, "
         f"kernel {adj['kernel_ms']:.3f} ms")
+    ctx.set_autotune(False)   # the per-ray comparators run the default launch shape
     per_ray_adj = run_adjoint_per_ray(dp, L, ctx, torch, adj)
     log(f"adjoint per-ray loop: {per_ray_adj['device_ms']:.1f} ms device for "
         f"{per_ray_adj['launches']} launches, batched is {per_ray_adj['speedup_of_batched']:.0f}x "
         f"faster, bitwise equal {per_ray_adj['bitwise_equal_to_batched']}")
+    ctx.set_autotune(True)
     adj.pop("_device")
     problem = adj.pop("_problem")
     wm = run_wmodel(dp, L, physics, ctx, torch, a.steps, a.warmup)
@@ -983,7 +988,9 @@ def main():
     wprob = wm.pop("_problem")
     pm = run_pathmc(dp, physics, ctx)
     log(f"pathmc: {pm['value']:.4g} events/s, kernel {pm['kernel_ms']:.3f} ms")
+    ctx.set_autotune(False)
     per_ray_pm = run_pathmc_per_ray(dp, physics, ctx, pm)
+    ctx.set_autotune(True)
     log(f"pathmc per-ray loop: {per_ray_pm['device_ms_all_rays']:.1f} ms device (scaled), "
         f"batched is {per_ray_pm['speedup_of_batched']:.1f}x faster")
     pm["per_ray_loop"] = per_ray_pm
@@ -1057,6 +1064,9 @@ def main():
         "dem": dm,
         "normal": ne,
         "tetra": tt,
+        "autotune": {k: {"best": list(v["best"]), "ms": v["ms"],
+                         "tried": [[list(c), m] for c, m in v["tried"]]}
+                     for k, v in ctx.tuning().items()},
     }
     if a.calib_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)

@@ -1,5 +1,6 @@
 // Context lifecycle, error reporting, table upload.
 #include "dp_internal.hpp"
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -35,6 +36,40 @@ void release(DevBuf& b) {
   if (b.ptr) (void)hipFree(b.ptr);
   b.ptr = nullptr;
   b.bytes = 0;
+}
+
+int tuned(dp_ctx c, const std::string& key, const std::vector<TuneCfg>& cands,
+          const std::function<int(const TuneCfg&)>& run, TuneCfg* out) {
+  auto it = c->tune.find(key);
+  if (it != c->tune.end()) { *out = it->second.best; return DP_OK; }
+  *out = cands.front();
+  if (!c->autotune || c->capturing || cands.size() < 2) return DP_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(DP_E_HIP, "autotune: hipEventCreate");
+  }
+  TuneEntry E;
+  E.best_ms = 3.0e38f;
+  int rc = DP_OK;
+  for (const TuneCfg& cfg : cands) {
+    if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(DP_E_HIP, "autotune: hipEventRecord"); break; }
+    if ((rc = run(cfg)) != DP_OK) break;
+    if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(DP_E_HIP, "autotune: timing the candidate failed");
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    E.tried.push_back({cfg, ms});
+    if (ms < E.best_ms) { E.best_ms = ms; E.best = cfg; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  c->tune[key] = E;
+  *out = E.best;
+  return DP_OK;
 }
 }  // namespace dph
 
@@ -113,6 +148,46 @@ int dp_synchronize(dp_ctx c) {
   if (!c) return dph::fail(DP_E_ARGUMENT, "dp_synchronize: null context");
   DP_HIP(hipStreamSynchronize(c->stream));
   return DP_OK;
+}
+
+int dp_set_autotune(dp_ctx c, int32_t on) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_set_autotune: null context");
+  c->autotune = on != 0;
+  return DP_OK;
+}
+
+int dp_set_tuning(dp_ctx c, const char* key, int32_t k0, int32_t k1, int32_t k2) {
+  if (!c || !key) return dph::fail(DP_E_ARGUMENT, "dp_set_tuning: null argument");
+  dph::TuneEntry E;
+  E.best = {k0, k1, k2};
+  c->tune[key] = E;
+  return DP_OK;
+}
+
+int dp_clear_tuning(dp_ctx c) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_clear_tuning: null context");
+  c->tune.clear();
+  return DP_OK;
+}
+
+int64_t dp_tuning_report(dp_ctx c, char* buf, int64_t size) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_tuning_report: null context");
+  std::string r;
+  for (const auto& kv : c->tune) {
+    const dph::TuneEntry& E = kv.second;
+    r += kv.first + " | best " + std::to_string(E.best.k0) + "," + std::to_string(E.best.k1) + "," +
+         std::to_string(E.best.k2) + " " + std::to_string(E.best_ms) + " ms | tried";
+    for (const auto& t : E.tried)
+      r += " " + std::to_string(t.first.k0) + "," + std::to_string(t.first.k1) + "," +
+           std::to_string(t.first.k2) + ":" + std::to_string(t.second);
+    r += "\n";
+  }
+  if (buf && size > 0) {
+    const size_t n = std::min<size_t>(r.size(), (size_t)size - 1);
+    std::memcpy(buf, r.data(), n);
+    buf[n] = 0;
+  }
+  return (int64_t)r.size();
 }
 
 int dp_check_errors(dp_ctx c) {

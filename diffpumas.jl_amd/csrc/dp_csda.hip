@@ -50,6 +50,7 @@ struct CsdaArgs {
   double* jac_rem;
   double* scratch;
   int* queue;
+  int per_pop;     // rays a block takes from the queue at a time (tuned)
   int* err;
 };
 
@@ -151,30 +152,37 @@ __device__ __forceinline__ StepOut csda_segment_g(const double* sE, const double
 
 // 4 waves per SIMD (128 VGPRs, a 216-byte spill): measured 16 % faster than
 // the unbounded 198 VGPRs at 2 waves per SIMD, and 4 % faster than 3 waves.
+// TLDS: the rock and air tables are staged in LDS (8 K doubles); otherwise
+// they are read through L1/L2 from the table block.  Both give the same
+// values; the autotuner picks one per workload.
+template <bool TLDS>
 __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
   extern __shared__ double lds[];
   const int K = a.K;
   const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = bd >> 6;
-  double* rE = lds;
-  double* rLE = lds + K;
-  double* rR = lds + 2 * K;
-  double* rS = lds + 3 * K;
-  double* aE = lds + 4 * K;
-  double* aLE = lds + 5 * K;
-  double* aR = lds + 6 * K;
-  double* aS = lds + 7 * K;
-  double* wsum = lds + 8 * K;               // [nw][max_seg]
+  const double* tr = a.tab + (size_t)a.rock * 19 * K;
+  const double* ta = a.tab + (size_t)a.air * 19 * K;
+  double* tl = TLDS ? lds : nullptr;
+  const double* rE = TLDS ? tl : tr;
+  const double* rLE = TLDS ? tl + K : tr + K;
+  const double* rR = TLDS ? tl + 2 * K : tr + 3 * K;
+  const double* rS = TLDS ? tl + 3 * K : tr + 2 * K;
+  const double* aE = TLDS ? tl + 4 * K : ta;
+  const double* aLE = TLDS ? tl + 5 * K : ta + K;
+  const double* aR = TLDS ? tl + 6 * K : ta + 3 * K;
+  const double* aS = TLDS ? tl + 7 * K : ta + 2 * K;
+  double* wsum = lds + (TLDS ? 8 * K : 0);       // [nw][max_seg]
   double* wrem = wsum + (size_t)nw * a.max_seg;  // [nw]
-  double* wflux = wrem + nw;                // [nw]
+  double* wflux = wrem + nw;                     // [nw]
   __shared__ int s_ray;
 
-  {
-    const double* tr = a.tab + (size_t)a.rock * 19 * K;
-    const double* ta = a.tab + (size_t)a.air * 19 * K;
+  if (TLDS) {
     for (int i = tid; i < K; i += bd) {
-      rE[i] = tr[0 * K + i]; rLE[i] = tr[1 * K + i]; rR[i] = tr[3 * K + i]; rS[i] = tr[2 * K + i];
-      aE[i] = ta[0 * K + i]; aLE[i] = ta[1 * K + i]; aR[i] = ta[3 * K + i]; aS[i] = ta[2 * K + i];
+      tl[i] = tr[0 * K + i]; tl[K + i] = tr[1 * K + i]; tl[2 * K + i] = tr[3 * K + i];
+      tl[3 * K + i] = tr[2 * K + i];
+      tl[4 * K + i] = ta[0 * K + i]; tl[5 * K + i] = ta[1 * K + i]; tl[6 * K + i] = ta[3 * K + i];
+      tl[7 * K + i] = ta[2 * K + i];
     }
   }
   double* sc = a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd;
@@ -182,120 +190,124 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
   const double inv_mass = 1.0 / a.mass;
 
   for (;;) {
-    if (tid == 0) s_ray = atomicAdd(a.queue, 1);
+    if (tid == 0) s_ray = atomicAdd(a.queue, a.per_pop);
     __syncthreads();
-    const int r = s_ray;
-    if (r >= a.n_rays) break;
-    const int b0 = a.row_ptr[r];
-    const int ns = a.row_ptr[r + 1] - b0;
-    if (ns < 0 || ns > a.max_seg) {
-      if (tid == 0) {
-        atomicOr(a.err, dph::KERR_CAPACITY);
-        a.flux[r] = __builtin_nan("");
-        if (a.jac_rem) a.jac_rem[r] = __builtin_nan("");
+    const int r0 = s_ray;
+    __syncthreads();   // s_ray is read before thread 0 writes it again
+    if (r0 >= a.n_rays) break;
+    const int r_end = r0 + a.per_pop < a.n_rays ? r0 + a.per_pop : a.n_rays;
+    for (int r = r0; r < r_end; ++r) {
+      const int b0 = a.row_ptr[r];
+      const int ns = a.row_ptr[r + 1] - b0;
+      if (ns < 0 || ns > a.max_seg) {
+        if (tid == 0) {
+          atomicOr(a.err, dph::KERR_CAPACITY);
+          a.flux[r] = __builtin_nan("");
+          if (a.jac_rem) a.jac_rem[r] = __builtin_nan("");
+        }
+        __syncthreads();
+        continue;
       }
-      __syncthreads();
-      continue;
-    }
-    for (int s = lane; s < ns; s += 64) wsum[wave * a.max_seg + s] = 0.0;
-    if (lane == 0) { wrem[wave] = 0.0; wflux[wave] = 0.0; }
+      for (int s = lane; s < ns; s += 64) wsum[wave * a.max_seg + s] = 0.0;
+      if (lane == 0) { wrem[wave] = 0.0; wflux[wave] = 0.0; }
 
-    const double cosr = a.cos_theta[r];
-    const double zex = a.z_exit[r];
-    const double cs = fmax(fabs(cosr), 0.01);
-    const bool has_rem = zex < a.z_max - EPS64;
-    const double rem = (a.z_max - zex) / cs;
-    const double Xair = air_grammage(a.z_max, a.palt, cosr);
-    const double path_air = (a.palt - a.z_max) / cs;
+      const double cosr = a.cos_theta[r];
+      const double zex = a.z_exit[r];
+      const double cs = fmax(fabs(cosr), 0.01);
+      const bool has_rem = zex < a.z_max - EPS64;
+      const double rem = (a.z_max - zex) / cs;
+      const double Xair = air_grammage(a.z_max, a.palt, cosr);
+      const double path_air = (a.palt - a.z_max) / cs;
 
-    for (int n0 = 0; n0 < a.n_nodes; n0 += bd) {
-      const int n = n0 + tid;
-      const bool active = n < a.n_nodes;
-      double E = active ? a.ne[n] : a.ne[0];
-      double lE = log(E);
-      double W = 1.0;
-      bool ok = true;
-      int ie = -1, jr = K / 2;   // interval guesses (rock tables)
-      // ------------------------------------------------ forward sweep
-      for (int s = 0; s < ns; ++s) {
-        const int cell = a.cell[b0 + s];
-        const double d = a.len[b0 + s];
-        const double rho = a.dens[cell];
-        StepOut o = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, inv_mass,
-                                   a.ctau, ie, jr);
-        ok = ok && o.ok;
-        W = W * o.ratio;
-        W = W * o.dec;
-        double* p = sc + (size_t)s * 4 * bd + tid;
-        p[0] = o.c;
-        p[bd] = o.b;
-        p[2 * bd] = o.e;
-        p[3 * bd] = o.k * d;
-        E = o.E;
-        lE = o.lE;
-      }
-      StepOut orem;
-      orem.c = orem.b = orem.e = orem.k = 0.0;
-      if (has_rem) {
-        orem = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass,
-                              inv_mass, a.ctau, ie, jr);
-        ok = ok && orem.ok;
-        W = W * orem.ratio;
-        W = W * orem.dec;
-        E = orem.E;
-        lE = orem.lE;
-      }
-      int ie_air = -1, jr_air = jr;   // air tables: a fresh entry search
-      StepOut oair = csda_segment_g(aE, aLE, aR, aS, K, a.pow2, E, lE, Xair, path_air, a.mass,
-                                    inv_mass, a.ctau, ie_air, jr_air);
-      ok = ok && oair.ok;
-      W = W * oair.ratio;
-      W = W * oair.dec;
-      Dd phi = flux_gccly_d(cosr, oair.E, active ? a.nq[n] : 0.0);
-      const double fl = (active && ok) ? 2.0 * a.nw[n] * (W * phi.v) : 0.0;
-      // d log(tail)/dE at the exit of the cell list
-      double T = 0.0;
-      if (fl != 0.0) {
-        double as = phi.d / phi.v + oair.e;
-        T = as * oair.c + oair.b;
-      }
-      double fs = wave_sum(fl);
-      if (lane == 0) wflux[wave] += fs;
-      if (has_rem) {
-        double as = T + orem.e;
-        double g = fl != 0.0 ? fl * as * (orem.k * rem) : 0.0;
-        T = as * orem.c + orem.b;
-        double gs = wave_sum(g);
-        if (lane == 0) wrem[wave] += gs;
-      }
-      if (a.jac) {
-        // ---------------------------------------------- reverse sweep
-        for (int s = ns - 1; s >= 0; --s) {
-          const double* p = sc + (size_t)s * 4 * bd + tid;
-          double c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
-          double as = T + e;
-          double g = fl != 0.0 ? fl * as * rr : 0.0;
-          T = as * c + bb;
+      for (int n0 = 0; n0 < a.n_nodes; n0 += bd) {
+        const int n = n0 + tid;
+        const bool active = n < a.n_nodes;
+        double E = active ? a.ne[n] : a.ne[0];
+        double lE = log(E);
+        double W = 1.0;
+        bool ok = true;
+        int ie = -1, jr = K / 2;   // interval guesses (rock tables)
+        // ------------------------------------------------ forward sweep
+        for (int s = 0; s < ns; ++s) {
+          const int cell = a.cell[b0 + s];
+          const double d = a.len[b0 + s];
+          const double rho = a.dens[cell];
+          StepOut o = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, inv_mass,
+                                     a.ctau, ie, jr);
+          ok = ok && o.ok;
+          W = W * o.ratio;
+          W = W * o.dec;
+          double* p = sc + (size_t)s * 4 * bd + tid;
+          p[0] = o.c;
+          p[bd] = o.b;
+          p[2 * bd] = o.e;
+          p[3 * bd] = o.k * d;
+          E = o.E;
+          lE = o.lE;
+        }
+        StepOut orem;
+        orem.c = orem.b = orem.e = orem.k = 0.0;
+        if (has_rem) {
+          orem = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass,
+                                inv_mass, a.ctau, ie, jr);
+          ok = ok && orem.ok;
+          W = W * orem.ratio;
+          W = W * orem.dec;
+          E = orem.E;
+          lE = orem.lE;
+        }
+        int ie_air = -1, jr_air = jr;   // air tables: a fresh entry search
+        StepOut oair = csda_segment_g(aE, aLE, aR, aS, K, a.pow2, E, lE, Xair, path_air, a.mass,
+                                      inv_mass, a.ctau, ie_air, jr_air);
+        ok = ok && oair.ok;
+        W = W * oair.ratio;
+        W = W * oair.dec;
+        Dd phi = flux_gccly_d(cosr, oair.E, active ? a.nq[n] : 0.0);
+        const double fl = (active && ok) ? 2.0 * a.nw[n] * (W * phi.v) : 0.0;
+        // d log(tail)/dE at the exit of the cell list
+        double T = 0.0;
+        if (fl != 0.0) {
+          double as = phi.d / phi.v + oair.e;
+          T = as * oair.c + oair.b;
+        }
+        double fs = wave_sum(fl);
+        if (lane == 0) wflux[wave] += fs;
+        if (has_rem) {
+          double as = T + orem.e;
+          double g = fl != 0.0 ? fl * as * (orem.k * rem) : 0.0;
+          T = as * orem.c + orem.b;
           double gs = wave_sum(g);
-          if (lane == 0) wsum[wave * a.max_seg + s] += gs;
+          if (lane == 0) wrem[wave] += gs;
+        }
+        if (a.jac) {
+          // ---------------------------------------------- reverse sweep
+          for (int s = ns - 1; s >= 0; --s) {
+            const double* p = sc + (size_t)s * 4 * bd + tid;
+            double c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
+            double as = T + e;
+            double g = fl != 0.0 ? fl * as * rr : 0.0;
+            T = as * c + bb;
+            double gs = wave_sum(g);
+            if (lane == 0) wsum[wave * a.max_seg + s] += gs;
+          }
         }
       }
-    }
-    __syncthreads();
-    if (a.jac) {
-      for (int s = tid; s < ns; s += bd) {
-        double v = 0.0;
-        for (int w = 0; w < nw; ++w) v += wsum[w * a.max_seg + s];
-        a.jac[b0 + s] = a.scale * v;
+      __syncthreads();
+      if (a.jac) {
+        for (int s = tid; s < ns; s += bd) {
+          double v = 0.0;
+          for (int w = 0; w < nw; ++w) v += wsum[w * a.max_seg + s];
+          a.jac[b0 + s] = a.scale * v;
+        }
       }
-    }
-    if (tid == 0) {
-      double f = 0.0, g = 0.0;
-      for (int w = 0; w < nw; ++w) { f += wflux[w]; g += wrem[w]; }
-      a.flux[r] = a.scale * f;
-      if (a.jac_rem) a.jac_rem[r] = a.scale * g;
-    }
-    __syncthreads();
+      if (tid == 0) {
+        double f = 0.0, g = 0.0;
+        for (int w = 0; w < nw; ++w) { f += wflux[w]; g += wrem[w]; }
+        a.flux[r] = a.scale * f;
+        if (a.jac_rem) a.jac_rem[r] = a.scale * g;
+      }
+      __syncthreads();
+    }  // rays of this pop
   }
 }
 
@@ -457,8 +469,17 @@ int check_paths(const dp_paths* p, const dp_csda_nodes* nd, int32_t n_cells, dp_
   return DP_OK;
 }
 
-size_t lds_bytes(int K, int nw, int max_seg) {
-  return ((size_t)8 * K + (size_t)nw * max_seg + 2 * nw) * sizeof(double);
+size_t lds_bytes(bool tlds, int K, int nw, int max_seg) {
+  return ((tlds ? (size_t)8 * K : 0) + (size_t)nw * max_seg + 2 * nw) * sizeof(double);
+}
+
+// Launch knobs of csda_cube_kernel: k0 = tables in LDS (1) or read through
+// L2 (0), k1 = rays per queue pop, k2 = grid multiplier over one resident
+// wave of blocks.  The first candidate is the default.
+const std::vector<dph::TuneCfg>& csda_candidates() {
+  static const std::vector<dph::TuneCfg> v = {
+      {1, 1, 1}, {0, 1, 1}, {1, 4, 1}, {0, 4, 1}, {1, 1, 2}, {0, 1, 2}};
+  return v;
 }
 
 int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
@@ -467,21 +488,37 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
   if (p->n_rays == 0) return DP_OK;
   const int bd = nd->n_nodes >= 256 ? 256 : ((nd->n_nodes + 63) / 64) * 64;
   const int nw = bd / 64;
+  const int K = t->n_knots;
   const int max_seg = p->max_segments > 0 ? p->max_segments : 1;
-  size_t lds = lds_bytes(t->n_knots, nw, max_seg);
-  if (lds > 160 * 1024)
+  if (lds_bytes(false, K, nw, max_seg) > 160 * 1024)
     return dph::fail(DP_E_CAPACITY, "dp_csda_cube: max_segments " + std::to_string(max_seg) +
                                         " exceeds the LDS budget");
-  int occ = 0;
-  DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel, bd, lds));
-  if (occ < 1) occ = 1;
-  int grid = c->n_cu * occ;
-  if (grid > p->n_rays) grid = p->n_rays;
-  size_t scratch = (size_t)grid * max_seg * 4 * bd * sizeof(double);
+  // Per candidate: LDS, grid; the scratch covers the largest grid.
+  const auto& all = csda_candidates();
+  std::vector<dph::TuneCfg> cands;
+  std::vector<int> grids;
+  int grid_max = 1;
+  for (const auto& cfg : all) {
+    const size_t lds = lds_bytes(cfg.k0 != 0, K, nw, max_seg);
+    if (lds > 160 * 1024) continue;
+    int occ = 0;
+    if (cfg.k0)
+      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<true>, bd, lds));
+    else
+      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<false>, bd, lds));
+    if (occ < 1) occ = 1;
+    long g = (long)c->n_cu * occ * cfg.k2;
+    const long pops = (p->n_rays + cfg.k1 - 1) / cfg.k1;
+    if (g > pops) g = pops;
+    cands.push_back(cfg);
+    grids.push_back((int)g);
+    if (g > grid_max) grid_max = (int)g;
+  }
+  size_t scratch = (size_t)grid_max * max_seg * 4 * bd * sizeof(double);
   int rc = dph::ensure(c->scratch, scratch);
   if (rc) return rc;
   CsdaArgs a;
-  a.tab = t->data; a.K = t->n_knots; a.pow2 = t->pow2;
+  a.tab = t->data; a.K = K; a.pow2 = t->pow2;
   a.rock = nd->rock_material; a.air = nd->air_material;
   a.mass = t->mass; a.ctau = t->ctau;
   a.n_rays = p->n_rays; a.max_seg = max_seg;
@@ -492,12 +529,33 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
   a.dens = dens; a.flux = flux; a.jac = jac; a.jac_rem = jac_rem;
   a.scratch = (double*)c->scratch.ptr;
   a.queue = queue; a.err = err;
-  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, queue, err, reset_err ? 1 : 0);
-  dph::record(c, c->ev0);
-  hipLaunchKernelGGL(csda_cube_kernel, dim3(grid), dim3(bd), lds, c->stream, a);
-  dph::record(c, c->ev1);
-  DP_HIP(hipGetLastError());
-  return DP_OK;
+  auto launch = [&](const dph::TuneCfg& cfg) -> int {
+    int i = 0;
+    while (i < (int)cands.size() && !(cands[i].k0 == cfg.k0 && cands[i].k1 == cfg.k1 &&
+                                      cands[i].k2 == cfg.k2))
+      ++i;
+    if (i == (int)cands.size()) i = 0;
+    CsdaArgs b = a;
+    b.per_pop = cands[i].k1;
+    const size_t lds = lds_bytes(cands[i].k0 != 0, K, nw, max_seg);
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, queue, err,
+                       reset_err ? 1 : 0);
+    dph::record(c, c->ev0);
+    if (cands[i].k0)
+      hipLaunchKernelGGL(csda_cube_kernel<true>, dim3(grids[i]), dim3(bd), lds, c->stream, b);
+    else
+      hipLaunchKernelGGL(csda_cube_kernel<false>, dim3(grids[i]), dim3(bd), lds, c->stream, b);
+    dph::record(c, c->ev1);
+    DP_HIP(hipGetLastError());
+    return DP_OK;
+  };
+  const std::string key = "csda_cube K=" + std::to_string(K) + " nodes=" +
+                          std::to_string(nd->n_nodes) + " rays=" + std::to_string(p->n_rays) +
+                          " max_seg=" + std::to_string(max_seg) + " jac=" + (jac ? "1" : "0");
+  dph::TuneCfg cfg;
+  rc = dph::tuned(c, key, cands, launch, &cfg);
+  if (rc) return rc;
+  return launch(cfg);
 }
 
 // Byte offsets of the VJP's scratch: cell-sorted keys/values and their sort

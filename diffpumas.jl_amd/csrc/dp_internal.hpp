@@ -34,10 +34,21 @@ void release(DevBuf& b);
 
 }  // namespace dph
 
+#include <functional>
 #include <map>
 #include <memory>
 
 namespace dph {
+// Launch knobs of one kernel configuration (their meaning is per kernel).
+struct TuneCfg {
+  int k0 = 0, k1 = 0, k2 = 0;
+};
+// What the autotuner measured for one (kernel, workload shape) key.
+struct TuneEntry {
+  TuneCfg best;
+  float best_ms = 0.f;
+  std::vector<std::pair<TuneCfg, float>> tried;
+};
 // Element of a material recipe (Z, A, I in GeV, mass fraction).
 struct Element { double Z, A, I, f; };
 // A prepared object living in device memory (mixture tables or mixed kernel).
@@ -81,6 +92,8 @@ struct dp_ctx_s {
   bool capturing = false;               // between dp_capture_begin and dp_capture_end
   uint64_t generation = 0;              // bumped by every scratch reallocation
   void* noalloc_scratch = nullptr;      // this device's static scratch (no-allocation VJP)
+  bool autotune = false;                // time launch candidates on first use
+  std::map<std::string, dph::TuneEntry> tune;  // chosen launch knobs per key
 };
 
 namespace dph {
@@ -103,6 +116,13 @@ constexpr int KERR_CAPACITY = 1;
 constexpr int KERR_CELL = 2;
 // Device address of the no-allocation VJP's static scratch on `device`.
 int noalloc_scratch(int device, void** out);
+// Launch knobs for `key`: the cached choice if there is one; otherwise, with
+// autotuning on and no capture open, the fastest of `cands` (each timed once
+// by run(cfg), which enqueues the work on the context's stream), cached; else
+// cands[0], the default.  The kernels' outputs do not depend on the knobs, so
+// the timed runs only overwrite outputs with the same values.
+int tuned(dp_ctx c, const std::string& key, const std::vector<TuneCfg>& cands,
+          const std::function<int(const TuneCfg&)>& run, TuneCfg* out);
 // Timing event on the context's stream; not recorded while the stream is
 // being captured into a graph.
 inline void record(dp_ctx c, hipEvent_t e) {

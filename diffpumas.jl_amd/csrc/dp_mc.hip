@@ -884,12 +884,23 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     if (fast) DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<true>, MC_THREADS, 0));
     else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<false>, MC_THREADS, 0));
     if (occ < 1) occ = 1;
+    // Blocks take 64-event rounds grid-stride; the tuned knob k0 is the
+    // number of blocks per resident block slot (default 4).
     const int64_t want = (n + MC_THREADS - 1) / MC_THREADS;
-    int64_t grid = (int64_t)c->n_cu * occ * 4;
-    if (grid > want) grid = want;
-    if (fast) hipLaunchKernelGGL(mc_flat_stochastic_kernel<true>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
-    else hipLaunchKernelGGL(mc_flat_stochastic_kernel<false>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
-    DP_HIP(hipGetLastError());
+    auto launch = [&](const dph::TuneCfg& cfg) -> int {
+      int64_t grid = (int64_t)c->n_cu * occ * std::max(1, cfg.k0);
+      if (grid > want) grid = want;
+      if (fast) hipLaunchKernelGGL(mc_flat_stochastic_kernel<true>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
+      else hipLaunchKernelGGL(mc_flat_stochastic_kernel<false>, dim3((unsigned)grid), dim3(MC_THREADS), 0, c->stream, s);
+      DP_HIP(hipGetLastError());
+      return DP_OK;
+    };
+    static const std::vector<dph::TuneCfg> cands = {{4, 0, 0}, {1, 0, 0}, {2, 0, 0}, {8, 0, 0}, {16, 0, 0}};
+    const std::string key = std::string("mc_flat_stochastic fast=") + (fast ? "1" : "0") +
+                            " events=" + std::to_string(n);
+    dph::TuneCfg cfg;
+    if ((rc = dph::tuned(c, key, cands, launch, &cfg))) return rc;
+    if ((rc = launch(cfg))) return rc;
     dph::record(c, c->ev1);
   }
   if (!stochastic || a.grad) {

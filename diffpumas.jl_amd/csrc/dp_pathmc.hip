@@ -460,13 +460,14 @@ int run_paths(const std::string& fn, dp_ctx c, dp_tables t, const dp_material_co
   if (rc) return rc;
   a.w_local = (double*)c->scratch.ptr;
   a.w_local_n = (double*)((char*)c->scratch.ptr + o_wn);
-  dph::record(c, c->ev0);
-  for (int r0 = 0; r0 < nr; r0 += rays_per_batch) {
+  // One batch of paths: the persistent queue kernel with k0 blocks per
+  // resident block slot (the tuned knob, default 1), then the per-item sums.
+  auto batch = [&](int r0, const dph::TuneCfg& cfg) -> int {
     const int r1 = std::min(nr, r0 + rays_per_batch);
     a.r0 = r0;
     a.n_items_mc = (int64_t)(r1 - r0) * chunks;
     a.n_items = a.n_items_mc + (int64_t)(r1 - r0) * chunks_n;
-    int64_t grid = (int64_t)c->n_cu * occ;
+    int64_t grid = (int64_t)c->n_cu * occ * std::max(1, cfg.k0);
     if (grid > a.n_items) grid = a.n_items;
     hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, st, q);
     if (fast) hipLaunchKernelGGL(path_mc_kernel<true>, dim3((unsigned)grid), dim3(PM_THREADS), 0, st, a);
@@ -475,7 +476,19 @@ int run_paths(const std::string& fn, dp_ctx c, dp_tables t, const dp_material_co
     int64_t gi = a.n_items < (int64_t)c->n_cu * 8 ? a.n_items : (int64_t)c->n_cu * 8;
     hipLaunchKernelGGL(path_items_kernel, dim3((unsigned)gi), dim3(PM_THREADS), 0, st, a);
     DP_HIP(hipGetLastError());
-  }
+    return DP_OK;
+  };
+  // Tuned on the first batch (every batch has the same shape but the last).
+  static const std::vector<dph::TuneCfg> cands = {{1, 0, 0}, {2, 0, 0}, {4, 0, 0}};
+  const std::string key = std::string("path_mc fast=") + (fast ? "1" : "0") + " rays=" +
+                          std::to_string(nr) + " events=" + std::to_string(nn) + " nodes=" +
+                          std::to_string(nq) + " segments=" + std::to_string(p->n_segments);
+  dph::TuneCfg cfg;
+  rc = dph::tuned(c, key, cands, [&](const dph::TuneCfg& k) { return batch(0, k); }, &cfg);
+  if (rc) return rc;
+  dph::record(c, c->ev0);
+  for (int r0 = 0; r0 < nr; r0 += rays_per_batch)
+    if ((rc = batch(r0, cfg))) return rc;
   dph::record(c, c->ev1);
   hipLaunchKernelGGL(path_reduce, dim3((nr + 255) / 256), dim3(256), 0, st, a.part, nr, chunks,
                      nn, a.cos_t, a.valid, (double*)(b + o_flux), (double*)(b + o_sig));

@@ -180,6 +180,10 @@ _SIGS = {
     "dp_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dp_synchronize": (C.c_int, [C.c_void_p]),
     "dp_check_errors": (C.c_int, [C.c_void_p]),
+    "dp_set_autotune": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dp_clear_tuning": (C.c_int, [C.c_void_p]),
+    "dp_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32]),
+    "dp_tuning_report": (C.c_int64, [C.c_void_p, C.c_char_p, C.c_int64]),
     "dp_device_info": (C.c_int, [C.c_void_p, i32_p, i32_p]),
     "dp_capture_begin": (C.c_int, [C.c_void_p]),
     "dp_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -336,6 +340,44 @@ class Context:
         ArgumentError for a cell index outside the field.  Clears the
         context's sticky error word."""
         check(self.lib.dp_check_errors(self.h))
+
+    def set_autotune(self, on=True):
+        """Time each candidate launch configuration of a tuned kernel on its
+        first call per workload shape and keep the fastest (results are the
+        same for every candidate)."""
+        check(self.lib.dp_set_autotune(self.h, 1 if on else 0))
+
+    def clear_tuning(self):
+        check(self.lib.dp_clear_tuning(self.h))
+
+    def set_tuning(self, key, k0, k1=0, k2=0):
+        """Pin the launch knobs of one (kernel, shape) key."""
+        check(self.lib.dp_set_tuning(self.h, key.encode(), k0, k1, k2))
+
+    def tuning(self):
+        """Kept choices as {key: {"best": (k0, k1, k2), "ms": t, "tried":
+        [((k0, k1, k2), ms), ...]}}."""
+        out = {}
+        for line in self.tuning_report().splitlines():
+            key, best, tried = (x.strip() for x in line.split(" | "))
+            _, knobs, ms, _ = (best + " x").split(" ", 3)
+            tr = []
+            for t in tried.split()[1:]:
+                k, m = t.split(":")
+                tr.append((tuple(int(x) for x in k.split(",")), float(m)))
+            out[key] = {"best": tuple(int(x) for x in knobs.split(",")), "ms": float(ms),
+                        "tried": tr}
+        return out
+
+    def tuning_report(self):
+        """Kept choices, one line per (kernel, shape): key, best knobs and
+        time, and every candidate's time."""
+        n = int(self.lib.dp_tuning_report(self.h, None, 0))
+        if n < 0:
+            check(n)
+        buf = C.create_string_buffer(n + 1)
+        self.lib.dp_tuning_report(self.h, buf, n + 1)
+        return buf.value.decode()
 
     def last_kernel_ms(self):
         return float(self.lib.dp_last_kernel_ms(self.h))

@@ -89,6 +89,22 @@ int dp_synchronize(dp_ctx ctx);
  * it after a chain of _dev calls or graph launches; host entry points check
  * their own kernels and leave the word alone.                             */
 int dp_check_errors(dp_ctx ctx);
+/* Launch autotuning (no counterpart in the reference, whose kernels are
+ * compiled by Julia per call with fixed launch shapes).  With it on, the
+ * first call of a tuned kernel for a workload shape (kernel, sizes) times
+ * each candidate launch configuration once on the context's stream and keeps
+ * the fastest for that shape in the context; later calls, and calls while a
+ * capture is open, use the kept choice (or the default if none).  Results do
+ * not depend on the choice: every candidate computes the same values
+ * bit for bit.  dp_clear_tuning forgets the choices; dp_tuning_report writes
+ * one line per shape ("key | best k0,k1,k2 ms | tried cfg:ms ...") into buf
+ * (NUL-terminated, truncated to size) and returns the full length;
+ * dp_set_tuning pins the knobs for a key as the report spells it (a
+ * configuration outside a kernel's candidate list selects its default).   */
+int dp_set_autotune(dp_ctx ctx, int32_t on);
+int dp_set_tuning(dp_ctx ctx, const char* key, int32_t k0, int32_t k1, int32_t k2);
+int dp_clear_tuning(dp_ctx ctx);
+int64_t dp_tuning_report(dp_ctx ctx, char* buf, int64_t size);
 /* HIP graphs of device work (no counterpart in the reference: Julia re-runs
  * the calls).  Between dp_capture_begin and dp_capture_end the context's
  * stream is captured instead of run; only _dev entry points may be called
