@@ -249,6 +249,28 @@ int run_normal(dp_ctx c, const dp_csr* J, const double* weights, const double* r
   return DP_OK;
 }
 
+// r = pred - obs and the data cost 0.5 * sum(W r^2) (W = 1 without weights),
+// one block: thread t sums rows t, t + 256, ... in ascending order, then a
+// fixed tree, so the cost does not depend on timing.
+constexpr int RES_THREADS = 256;
+__global__ __launch_bounds__(RES_THREADS) void ne_residual_kernel(
+    int64_t n, const double* __restrict__ pred, const double* __restrict__ obs,
+    const double* __restrict__ w, double* __restrict__ r, double* __restrict__ cost) {
+  __shared__ double part[RES_THREADS];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += RES_THREADS) {
+    const double d = pred[i] - obs[i];
+    if (r) r[i] = d;
+    acc += w ? w[i] * (d * d) : d * d;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = RES_THREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && cost) *cost = 0.5 * part[0];
+}
 }  // namespace
 
 extern "C" {
@@ -332,6 +354,19 @@ int dp_normal_equations(dp_ctx c, const dp_csr* J, const double* weights, const 
   if (grad) DP_HIP(hipMemcpyAsync(grad, b + o_g, 8 * (size_t)na, hipMemcpyDeviceToHost, st));
   DP_HIP(hipStreamSynchronize(st));
   (void)hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1);
+  return DP_OK;
+}
+
+int dp_residual_dev(dp_ctx c, int64_t n, const double* pred, const double* obs,
+                    const double* weights, double* residual, double* cost) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_residual_dev: null context");
+  if (n < 0 || (n > 0 && (!pred || !obs)))
+    return dph::fail(DP_E_ARGUMENT, "dp_residual_dev: n must be >= 0 with pred and obs given");
+  if (!residual && !cost) return DP_OK;
+  DP_HIP(hipSetDevice(c->device));
+  hipLaunchKernelGGL(ne_residual_kernel, dim3(1), dim3(RES_THREADS), 0, c->stream, n, pred, obs,
+                     weights, residual, cost);
+  DP_HIP(hipGetLastError());
   return DP_OK;
 }
 

@@ -572,6 +572,17 @@ int dp_normal_equations(dp_ctx ctx, const dp_csr* J, const double* weights,
 int dp_normal_equations_dev(dp_ctx ctx, const dp_csr* J, const double* weights,
                             const double* residual, int32_t n_active, const int32_t* active,
                             double* normal, double* grad);
+/* Residual and data cost of a Gauss-Newton iterate on the device: the lines
+ * r = pred .- obs and data_cost(r) = 0.5*sum(W .* r.^2) (0.5*dot(r, r)
+ * without weights) of gauss_newton_reconstruct (src/Tomography_inverse.jl:
+ * 478-482), so that a captured iteration (w-model flux and Jacobian, this,
+ * the normal equations) needs no host round trip; no separate function of
+ * the reference.  Device pointers: pred, obs, weights (NULL: all 1) of n
+ * rows; residual (n) and cost (one double) may each be NULL.  The cost is
+ * summed in a fixed order (deterministic); non-finite rows propagate into
+ * it, as in the reference's objective.  Returns without synchronising.    */
+int dp_residual_dev(dp_ctx ctx, int64_t n, const double* pred, const double* obs,
+                    const double* weights, double* residual, double* cost);
 
 /* Last kernel duration in ms, timed with HIP events on the context's stream
  * around the dominant kernel of the previous call (0 if none). */
