@@ -156,6 +156,12 @@ int dp_set_autotune(dp_ctx c, int32_t on) {
   return DP_OK;
 }
 
+int dp_set_fp32_stepping(dp_ctx c, int32_t on) {
+  if (!c) return dph::fail(DP_E_ARGUMENT, "dp_set_fp32_stepping: null context");
+  c->fp32_stepping = on != 0;
+  return DP_OK;
+}
+
 int dp_set_tuning(dp_ctx c, const char* key, int32_t k0, int32_t k1, int32_t k2) {
   if (!c || !key) return dph::fail(DP_E_ARGUMENT, "dp_set_tuning: null argument");
   dph::TuneEntry E;

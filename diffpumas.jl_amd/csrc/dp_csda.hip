@@ -23,6 +23,7 @@
 #include "dp_device.hpp"
 #include "dp_sort.hpp"
 #include <climits>
+#include <type_traits>
 
 using namespace dpk;
 
@@ -54,14 +55,31 @@ struct CsdaArgs {
   int* err;
 };
 
-struct StepOut {
-  double E, lE;      // exit energy and its log
-  double ratio;      // S(E_out)/S(E_in)
-  double dec;        // decay factor of the segment
-  double c, b, e;    // partials (see header)
-  double k;          // dE_out/dX (X = grammage)
+// F: the type of the segment arithmetic (double; float for the FP32 stepping
+// variant).  The log energy, the state the walk carries, is FP64 in both.
+template <typename F>
+struct StepOutT {
+  F E;          // exit energy
+  double lE;    // its log
+  F ratio;      // S(E_out)/S(E_in)
+  F dec;        // decay factor of the segment
+  F c, b, e;    // partials (see header)
+  F k;          // dE_out/dX (X = grammage)
   bool ok;
 };
+using StepOut = StepOutT<double>;
+
+// Table views of one material.  FP64: energy, log energy, range, stopping
+// power.  FP32 stepping: log energy and range in FP64 with the reciprocals of
+// their knot spacings (the FP64 spine of the walk), stopping power in float.
+struct Tab64 {
+  const double *E, *LE, *R, *S;
+};
+struct Tab32 {
+  const double *LE, *R, *iLE, *iR;
+  const float* S;
+};
+constexpr double LOG_1E12 = 27.631021115928547;   // log(1e12): the "ok" energy bound
 
 // One CSDA segment: range inversion at grammage X_in(E) + gram, then the
 // stopping-power ratio and the decay factor at the mean energy.  The interval
@@ -72,11 +90,10 @@ struct StepOut {
 // segment's entry energy on the same tables.  Quotients that share a divisor
 // use one reciprocal (FP64 division costs ~10 quarter-rate instructions on
 // gfx950): values agree with the division form to a few ulp.
-__device__ __forceinline__ StepOut csda_segment_g(const double* sE, const double* sLE,
-                                                  const double* sR, const double* sS, int K,
-                                                  int pow2, double E, double lE, double gram,
-                                                  double path, double mass, double inv_mass,
-                                                  double ctau, int& ie, int& jr) {
+__device__ __forceinline__ StepOut csda_segment_g(const Tab64& T, int K, int pow2, double E,
+                                                  double lE, double gram, double path,
+                                                  double inv_mass, double ctau, int& ie, int& jr) {
+  const double *sE = T.E, *sLE = T.LE, *sR = T.R, *sS = T.S;
   StepOut o;
   double y0, dy0, y1, dy1;
   if (E <= sE[0]) {
@@ -150,44 +167,160 @@ __device__ __forceinline__ StepOut csda_segment_g(const double* sE, const double
   return o;
 }
 
+// The same segment map with FP32 arithmetic around an FP64 spine.  The walk's
+// state is the log energy: the entry range R(E) = R_i + t (R_i+1 - R_i), the
+// exit grammage X = R(E) + gram, its interval and the exit log energy
+// LE_j + t' (LE_j+1 - LE_j) are FP64 (subtractions and products only, with
+// the knot reciprocals precomputed), so rounding does not accumulate along
+// the path; the exit log energy is carried as is (log(exp(x)) is not
+// re-rounded).  Everything else (exp to the exit energy, stopping powers,
+// decay, the partials) is float.
+__device__ __forceinline__ StepOutT<float> csda_segment_g(const Tab32& T, int K, int pow2,
+                                                         float E, double lE, double gram,
+                                                         float path, float inv_mass, float ctau,
+                                                         int& ie, int& jr) {
+  StepOutT<float> o;
+  double y0;
+  float dy0, y1, dy1;
+  if (lE <= T.LE[0]) {
+    y0 = T.R[0]; y1 = T.S[0]; dy0 = dy1 = 0.f;
+  } else if (lE >= T.LE[K - 1]) {
+    y0 = T.R[K - 1]; y1 = T.S[K - 1]; dy0 = dy1 = 0.f;
+  } else {
+    const int i = ie >= 0 ? ie : locate_idx(T.LE, K, pow2, lE, jr);
+    const double rdl = T.iLE[i];
+    const double t = (lE - T.LE[i]) * rdl;
+    const double d0 = T.R[i + 1] - T.R[i];
+    const float d1 = T.S[i + 1] - T.S[i];
+    y0 = T.R[i] + t * d0;
+    y1 = T.S[i] + (float)t * d1;
+    const float re = (float)rdl / E;
+    dy0 = (float)d0 * re;
+    dy1 = d1 * re;
+  }
+  const double Xf = y0 + gram;
+  double lEo;
+  float kq = 0.f;   // dlogE/dX at the exit
+  if (Xf <= T.R[0]) {
+    lEo = T.LE[0];
+  } else if (Xf >= T.R[K - 1]) {
+    lEo = T.LE[K - 1];
+  } else {
+    const int j = locate_idx(T.R, K, pow2, Xf, jr);
+    jr = j;
+    const double rdr = T.iR[j];
+    const double t = (Xf - T.R[j]) * rdr;
+    const double dle = T.LE[j + 1] - T.LE[j];
+    lEo = T.LE[j] + t * dle;
+    kq = (float)(dle * rdr);
+  }
+  const float Eo = expf((float)lEo);
+  const float kdy = Eo * kq;
+  o.E = Eo;
+  o.lE = lEo;
+  o.ok = !(lEo >= LOG_1E12) && isfinite(lEo);
+  float so_y, so_dy;
+  if (lEo <= T.LE[0]) {
+    so_y = T.S[0]; so_dy = 0.f; ie = -1;
+  } else if (lEo >= T.LE[K - 1]) {
+    so_y = T.S[K - 1]; so_dy = 0.f; ie = -1;
+  } else {
+    const int i = locate_idx(T.LE, K, pow2, lEo, jr);
+    ie = i;
+    const double rdl = T.iLE[i];
+    const float t = (float)((lEo - T.LE[i]) * rdl);
+    const float dyy = T.S[i + 1] - T.S[i];
+    so_y = T.S[i] + t * dyy;
+    so_dy = dyy * ((float)rdl / Eo);
+  }
+  const float g = 1.f + 0.5f * (E + Eo) * inv_mass;
+  const float b2 = 1.f - 1.f / (g * g);
+  float beta, dbg_dg;
+  if (b2 >= 1e-12f) {
+    beta = sqrtf(b2);
+    dbg_dg = 1.f / beta;
+  } else {
+    beta = 1e-6f;
+    dbg_dg = beta;
+  }
+  const float rL = 1.f / (beta * g * ctau);
+  o.dec = expf(-path * rL);
+  const float dlog = path * rL * rL * (ctau * dbg_dg * inv_mass);
+  const float ry1 = 1.f / y1, rso = 1.f / so_y;
+  o.ratio = so_y * ry1;
+  o.k = kdy;
+  o.c = kdy * dy0;
+  o.b = -dy1 * ry1 + 0.5f * dlog;
+  o.e = so_dy * rso + 0.5f * dlog;
+  return o;
+}
+
 // 4 waves per SIMD (128 VGPRs, a 216-byte spill): measured 16 % faster than
 // the unbounded 198 VGPRs at 2 waves per SIMD, and 4 % faster than 3 waves.
-// TLDS: the rock and air tables are staged in LDS (8 K doubles); otherwise
-// they are read through L1/L2 from the table block.  Both give the same
-// values; the autotuner picks one per workload.
-template <bool TLDS>
+// TLDS: the rock and air tables are staged in LDS; otherwise they are read
+// through L1/L2 from the table block.  Both give the same values; the
+// autotuner picks one per workload.  F: the type of the segment arithmetic.
+// With F = float (the FP32 stepping variant) the segment maps, their
+// partials, the tape and the reverse sweep are float around the FP64
+// log-energy spine; the node weights, the flux model and every sum over
+// nodes stay FP64.
+template <bool TLDS, typename F>
 __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
+  constexpr bool F64 = sizeof(F) == sizeof(double);
+  static_assert(TLDS || F64, "FP32 stepping stages its tables in LDS");
+  using Tab = typename std::conditional<F64, Tab64, Tab32>::type;
   extern __shared__ double lds[];
   const int K = a.K;
   const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = bd >> 6;
   const double* tr = a.tab + (size_t)a.rock * 19 * K;
   const double* ta = a.tab + (size_t)a.air * 19 * K;
-  double* tl = TLDS ? lds : nullptr;
-  const double* rE = TLDS ? tl : tr;
-  const double* rLE = TLDS ? tl + K : tr + K;
-  const double* rR = TLDS ? tl + 2 * K : tr + 3 * K;
-  const double* rS = TLDS ? tl + 3 * K : tr + 2 * K;
-  const double* aE = TLDS ? tl + 4 * K : ta;
-  const double* aLE = TLDS ? tl + 5 * K : ta + K;
-  const double* aR = TLDS ? tl + 6 * K : ta + 3 * K;
-  const double* aS = TLDS ? tl + 7 * K : ta + 2 * K;
-  double* wsum = lds + (TLDS ? 8 * K : 0);       // [nw][max_seg]
-  double* wrem = wsum + (size_t)nw * a.max_seg;  // [nw]
-  double* wflux = wrem + nw;                     // [nw]
-  __shared__ int s_ray;
-
-  if (TLDS) {
+  Tab rock, air;
+  size_t tab_bytes = 0;
+  if constexpr (!F64) {
+    // [rock LE, R, 1/dLE, 1/dR][air LE, R, 1/dLE, 1/dR] doubles, then
+    // [rock S][air S] floats
+    double* tl = lds;
+    float* ts = (float*)(lds + 8 * K);
+    for (int i = tid; i < K; i += bd) {
+      const bool in = i + 1 < K;
+      tl[i] = tr[K + i]; tl[K + i] = tr[3 * K + i];
+      tl[2 * K + i] = in ? 1.0 / (tr[K + i + 1] - tr[K + i]) : 0.0;
+      tl[3 * K + i] = in ? 1.0 / (tr[3 * K + i + 1] - tr[3 * K + i]) : 0.0;
+      tl[4 * K + i] = ta[K + i]; tl[5 * K + i] = ta[3 * K + i];
+      tl[6 * K + i] = in ? 1.0 / (ta[K + i + 1] - ta[K + i]) : 0.0;
+      tl[7 * K + i] = in ? 1.0 / (ta[3 * K + i + 1] - ta[3 * K + i]) : 0.0;
+      ts[i] = (float)tr[2 * K + i];
+      ts[K + i] = (float)ta[2 * K + i];
+    }
+    rock = {tl, tl + K, tl + 2 * K, tl + 3 * K, ts};
+    air = {tl + 4 * K, tl + 5 * K, tl + 6 * K, tl + 7 * K, ts + K};
+    tab_bytes = (size_t)K * 72;
+  } else if constexpr (TLDS) {
+    double* tl = lds;
     for (int i = tid; i < K; i += bd) {
       tl[i] = tr[0 * K + i]; tl[K + i] = tr[1 * K + i]; tl[2 * K + i] = tr[3 * K + i];
       tl[3 * K + i] = tr[2 * K + i];
       tl[4 * K + i] = ta[0 * K + i]; tl[5 * K + i] = ta[1 * K + i]; tl[6 * K + i] = ta[3 * K + i];
       tl[7 * K + i] = ta[2 * K + i];
     }
+    rock = {tl, tl + K, tl + 2 * K, tl + 3 * K};
+    air = {tl + 4 * K, tl + 5 * K, tl + 6 * K, tl + 7 * K};
+    tab_bytes = (size_t)K * 64;
+  } else {
+    rock = {tr, tr + K, tr + 3 * K, tr + 2 * K};
+    air = {ta, ta + K, ta + 3 * K, ta + 2 * K};
   }
-  double* sc = a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd;
+  // the table bytes (a multiple of 8) keep the doubles after them aligned
+  double* wsum = (double*)((char*)lds + tab_bytes);  // [nw][max_seg]
+  double* wrem = wsum + (size_t)nw * a.max_seg;      // [nw]
+  double* wflux = wrem + nw;                         // [nw]
+  __shared__ int s_ray;
+
+  F* sc = (F*)a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd;
   const double rho0 = a.dens[0];
-  const double inv_mass = 1.0 / a.mass;
+  const F inv_mass = F(1.0 / a.mass);
+  const F ctau = F(a.ctau);
 
   for (;;) {
     if (tid == 0) s_ray = atomicAdd(a.queue, a.per_pop);
@@ -222,8 +355,9 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
       for (int n0 = 0; n0 < a.n_nodes; n0 += bd) {
         const int n = n0 + tid;
         const bool active = n < a.n_nodes;
-        double E = active ? a.ne[n] : a.ne[0];
-        double lE = log(E);
+        const double E0 = active ? a.ne[n] : a.ne[0];
+        F E = F(E0);
+        double lE = log(E0);
         double W = 1.0;
         bool ok = true;
         int ie = -1, jr = K / 2;   // interval guesses (rock tables)
@@ -232,24 +366,24 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
           const int cell = a.cell[b0 + s];
           const double d = a.len[b0 + s];
           const double rho = a.dens[cell];
-          StepOut o = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, d * rho, d, a.mass, inv_mass,
-                                     a.ctau, ie, jr);
+          StepOutT<F> o = csda_segment_g(rock, K, a.pow2, E, lE, d * rho, F(d), inv_mass, ctau,
+                                         ie, jr);
           ok = ok && o.ok;
           W = W * o.ratio;
           W = W * o.dec;
-          double* p = sc + (size_t)s * 4 * bd + tid;
+          F* p = sc + (size_t)s * 4 * bd + tid;
           p[0] = o.c;
           p[bd] = o.b;
           p[2 * bd] = o.e;
-          p[3 * bd] = o.k * d;
+          p[3 * bd] = o.k * F(d);
           E = o.E;
           lE = o.lE;
         }
-        StepOut orem;
-        orem.c = orem.b = orem.e = orem.k = 0.0;
+        StepOutT<F> orem;
+        orem.c = orem.b = orem.e = orem.k = F(0);
         if (has_rem) {
-          orem = csda_segment_g(rE, rLE, rR, rS, K, a.pow2, E, lE, rem * rho0, rem, a.mass,
-                                inv_mass, a.ctau, ie, jr);
+          orem = csda_segment_g(rock, K, a.pow2, E, lE, rem * rho0, F(rem), inv_mass, ctau, ie,
+                                jr);
           ok = ok && orem.ok;
           W = W * orem.ratio;
           W = W * orem.dec;
@@ -257,24 +391,26 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
           lE = orem.lE;
         }
         int ie_air = -1, jr_air = jr;   // air tables: a fresh entry search
-        StepOut oair = csda_segment_g(aE, aLE, aR, aS, K, a.pow2, E, lE, Xair, path_air, a.mass,
-                                      inv_mass, a.ctau, ie_air, jr_air);
+        StepOutT<F> oair = csda_segment_g(air, K, a.pow2, E, lE, Xair, F(path_air), inv_mass,
+                                          ctau, ie_air, jr_air);
         ok = ok && oair.ok;
         W = W * oair.ratio;
         W = W * oair.dec;
-        Dd phi = flux_gccly_d(cosr, oair.E, active ? a.nq[n] : 0.0);
+        double Es;   // surface energy
+        if constexpr (F64) Es = oair.E; else Es = exp(oair.lE);
+        Dd phi = flux_gccly_d(cosr, Es, active ? a.nq[n] : 0.0);
         const double fl = (active && ok) ? 2.0 * a.nw[n] * (W * phi.v) : 0.0;
         // d log(tail)/dE at the exit of the cell list
-        double T = 0.0;
+        F T = F(0);
         if (fl != 0.0) {
-          double as = phi.d / phi.v + oair.e;
+          F as = F(phi.d / phi.v) + oair.e;
           T = as * oair.c + oair.b;
         }
         double fs = wave_sum(fl);
         if (lane == 0) wflux[wave] += fs;
         if (has_rem) {
-          double as = T + orem.e;
-          double g = fl != 0.0 ? fl * as * (orem.k * rem) : 0.0;
+          F as = T + orem.e;
+          double g = fl != 0.0 ? fl * (double)as * ((double)orem.k * rem) : 0.0;
           T = as * orem.c + orem.b;
           double gs = wave_sum(g);
           if (lane == 0) wrem[wave] += gs;
@@ -282,10 +418,10 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
         if (a.jac) {
           // ---------------------------------------------- reverse sweep
           for (int s = ns - 1; s >= 0; --s) {
-            const double* p = sc + (size_t)s * 4 * bd + tid;
-            double c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
-            double as = T + e;
-            double g = fl != 0.0 ? fl * as * rr : 0.0;
+            const F* p = sc + (size_t)s * 4 * bd + tid;
+            F c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
+            F as = T + e;
+            double g = fl != 0.0 ? fl * (double)as * (double)rr : 0.0;
             T = as * c + bb;
             double gs = wave_sum(g);
             if (lane == 0) wsum[wave * a.max_seg + s] += gs;
@@ -469,17 +605,22 @@ int check_paths(const dp_paths* p, const dp_csda_nodes* nd, int32_t n_cells, dp_
   return DP_OK;
 }
 
-size_t lds_bytes(bool tlds, int K, int nw, int max_seg) {
-  return ((tlds ? (size_t)8 * K : 0) + (size_t)nw * max_seg + 2 * nw) * sizeof(double);
+// Tables: FP64 8 K doubles; FP32 stepping 8 K doubles (log energy, range and
+// their knot reciprocals) and 2 K floats (stopping powers).
+size_t lds_bytes(bool tlds, int K, int nw, int max_seg, size_t fsize = sizeof(double)) {
+  const size_t tab = !tlds ? 0 : (size_t)K * (fsize == sizeof(float) ? 72 : 64);
+  return tab + ((size_t)nw * max_seg + 2 * nw) * sizeof(double);
 }
 
 // Launch knobs of csda_cube_kernel: k0 = tables in LDS (1) or read through
 // L2 (0), k1 = rays per queue pop, k2 = grid multiplier over one resident
-// wave of blocks.  The first candidate is the default.
-const std::vector<dph::TuneCfg>& csda_candidates() {
+// wave of blocks.  The first candidate is the default.  FP32 stepping always
+// stages its (float) tables in LDS.
+const std::vector<dph::TuneCfg>& csda_candidates(bool fp32) {
   static const std::vector<dph::TuneCfg> v = {
       {1, 1, 1}, {0, 1, 1}, {1, 4, 1}, {0, 4, 1}, {1, 1, 2}, {0, 1, 2}};
-  return v;
+  static const std::vector<dph::TuneCfg> v32 = {{1, 1, 1}, {1, 4, 1}, {1, 1, 2}};
+  return fp32 ? v32 : v;
 }
 
 int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
@@ -490,22 +631,29 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
   const int nw = bd / 64;
   const int K = t->n_knots;
   const int max_seg = p->max_segments > 0 ? p->max_segments : 1;
-  if (lds_bytes(false, K, nw, max_seg) > 160 * 1024)
+  const bool f32 = c->fp32_stepping;
+  const size_t fsz = f32 ? sizeof(float) : sizeof(double);
+  if (lds_bytes(f32, K, nw, max_seg, fsz) > 160 * 1024)
     return dph::fail(DP_E_CAPACITY, "dp_csda_cube: max_segments " + std::to_string(max_seg) +
                                         " exceeds the LDS budget");
   // Per candidate: LDS, grid; the scratch covers the largest grid.
-  const auto& all = csda_candidates();
+  const auto& all = csda_candidates(f32);
   std::vector<dph::TuneCfg> cands;
   std::vector<int> grids;
   int grid_max = 1;
   for (const auto& cfg : all) {
-    const size_t lds = lds_bytes(cfg.k0 != 0, K, nw, max_seg);
+    const size_t lds = lds_bytes(cfg.k0 != 0, K, nw, max_seg, fsz);
     if (lds > 160 * 1024) continue;
     int occ = 0;
-    if (cfg.k0)
-      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<true>, bd, lds));
+    if (f32)
+      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<true, float>, bd,
+                                                          lds));
+    else if (cfg.k0)
+      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<true, double>, bd,
+                                                          lds));
     else
-      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<false>, bd, lds));
+      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<false, double>,
+                                                          bd, lds));
     if (occ < 1) occ = 1;
     long g = (long)c->n_cu * occ * cfg.k2;
     const long pops = (p->n_rays + cfg.k1 - 1) / cfg.k1;
@@ -514,7 +662,7 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
     grids.push_back((int)g);
     if (g > grid_max) grid_max = (int)g;
   }
-  size_t scratch = (size_t)grid_max * max_seg * 4 * bd * sizeof(double);
+  size_t scratch = (size_t)grid_max * max_seg * 4 * bd * fsz;
   int rc = dph::ensure(c->scratch, scratch);
   if (rc) return rc;
   CsdaArgs a;
@@ -537,19 +685,24 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
     if (i == (int)cands.size()) i = 0;
     CsdaArgs b = a;
     b.per_pop = cands[i].k1;
-    const size_t lds = lds_bytes(cands[i].k0 != 0, K, nw, max_seg);
+    const size_t lds = lds_bytes(cands[i].k0 != 0, K, nw, max_seg, fsz);
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, queue, err,
                        reset_err ? 1 : 0);
     dph::record(c, c->ev0);
-    if (cands[i].k0)
-      hipLaunchKernelGGL(csda_cube_kernel<true>, dim3(grids[i]), dim3(bd), lds, c->stream, b);
+    if (f32)
+      hipLaunchKernelGGL((csda_cube_kernel<true, float>), dim3(grids[i]), dim3(bd), lds, c->stream,
+                         b);
+    else if (cands[i].k0)
+      hipLaunchKernelGGL((csda_cube_kernel<true, double>), dim3(grids[i]), dim3(bd), lds,
+                         c->stream, b);
     else
-      hipLaunchKernelGGL(csda_cube_kernel<false>, dim3(grids[i]), dim3(bd), lds, c->stream, b);
+      hipLaunchKernelGGL((csda_cube_kernel<false, double>), dim3(grids[i]), dim3(bd), lds,
+                         c->stream, b);
     dph::record(c, c->ev1);
     DP_HIP(hipGetLastError());
     return DP_OK;
   };
-  const std::string key = "csda_cube K=" + std::to_string(K) + " nodes=" +
+  const std::string key = std::string(f32 ? "csda_cube fp32" : "csda_cube") + " K=" + std::to_string(K) + " nodes=" +
                           std::to_string(nd->n_nodes) + " rays=" + std::to_string(p->n_rays) +
                           " max_seg=" + std::to_string(max_seg) + " jac=" + (jac ? "1" : "0");
   dph::TuneCfg cfg;

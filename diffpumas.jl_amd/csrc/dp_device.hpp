@@ -44,6 +44,28 @@ __device__ __forceinline__ int locate_idx(const double* __restrict__ xs, int n, 
   return bracket(xs, n, pow2, x);
 }
 
+// The same two searches over FP32 grids (the FP32 stepping kernels).
+__device__ __forceinline__ int bracket(const float* __restrict__ a, int n, int pow2, float x) {
+  int lo = 0;
+#pragma unroll 1
+  for (int step = pow2; step > 0; step >>= 1) {
+    int c = lo + step;
+    if (c <= n - 2 && a[c] <= x) lo = c;
+  }
+  return lo;
+}
+__device__ __forceinline__ int locate_idx(const float* __restrict__ xs, int n, int pow2, float x,
+                                          int g) {
+  g = min(max(g, 0), n - 2);
+  if (xs[g] <= x) {
+    if (g == n - 2 || x < xs[g + 1]) return g;
+    if (g + 1 == n - 2 || x < xs[g + 2]) return g + 1;
+  } else if (g > 0 && xs[g - 1] <= x) {
+    return g - 1;
+  }
+  return bracket(xs, n, pow2, x);
+}
+
 // Interpolated property with slope dy/dE (0 when clamped).
 struct Lookup {
   double y, dy;

@@ -93,6 +93,7 @@ struct dp_ctx_s {
   uint64_t generation = 0;              // bumped by every scratch reallocation
   void* noalloc_scratch = nullptr;      // this device's static scratch (no-allocation VJP)
   bool autotune = false;                // time launch candidates on first use
+  bool fp32_stepping = false;           // density adjoint steps in FP32
   std::map<std::string, dph::TuneEntry> tune;  // chosen launch knobs per key
 };
 

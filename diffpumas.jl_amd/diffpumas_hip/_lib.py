@@ -181,6 +181,7 @@ _SIGS = {
     "dp_synchronize": (C.c_int, [C.c_void_p]),
     "dp_check_errors": (C.c_int, [C.c_void_p]),
     "dp_set_autotune": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dp_set_fp32_stepping": (C.c_int, [C.c_void_p, C.c_int32]),
     "dp_clear_tuning": (C.c_int, [C.c_void_p]),
     "dp_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32]),
     "dp_tuning_report": (C.c_int64, [C.c_void_p, C.c_char_p, C.c_int64]),
@@ -348,6 +349,12 @@ class Context:
         first call per workload shape and keep the fastest (results are the
         same for every candidate)."""
         check(self.lib.dp_set_autotune(self.h, 1 if on else 0))
+
+    def set_fp32_stepping(self, on=True):
+        """FP32 segment arithmetic around an FP64 log-energy spine for the
+        density CSDA flux and adjoint of this context (relative errors of a
+        few 1e-6 against the FP64 program)."""
+        check(self.lib.dp_set_fp32_stepping(self.h, 1 if on else 0))
 
     def clear_tuning(self):
         check(self.lib.dp_clear_tuning(self.h))

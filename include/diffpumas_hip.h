@@ -102,6 +102,17 @@ int dp_check_errors(dp_ctx ctx);
  * dp_set_tuning pins the knobs for a key as the report spells it (a
  * configuration outside a kernel's candidate list selects its default).   */
 int dp_set_autotune(dp_ctx ctx, int32_t on);
+/* FP32 stepping for the per-voxel-density CSDA flux and adjoint (dp_csda_cube
+ * and every entry point built on it) of this context: the segment maps'
+ * exponentials, stopping powers, decay factors, their partials, the adjoint
+ * tape and the reverse sweep in float, around an FP64 spine (the walk's log
+ * energy, the range lookup and the exit-grammage inversion, which take
+ * subtractions and products only); node weights, the flux model, every sum
+ * over nodes and the outputs stay FP64.  The reference computes in Float64
+ * only (examples/diff_flux_cube.jl:374-468); this variant trades relative
+ * errors of a few 1e-6 (bar 1e-5) against the FP64 program for speed.  Off
+ * by default.                                                             */
+int dp_set_fp32_stepping(dp_ctx ctx, int32_t on);
 int dp_set_tuning(dp_ctx ctx, const char* key, int32_t k0, int32_t k1, int32_t k2);
 int dp_clear_tuning(dp_ctx ctx);
 int64_t dp_tuning_report(dp_ctx ctx, char* buf, int64_t size);
