@@ -112,9 +112,12 @@ def adjoint_workload(dp, physics, ctx, torch):
     return geo, paths, rho, ne, nw, nq
 
 
-def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
+def run_adjoint(dp, L, physics, ctx, torch, steps, warmup, fp32=False):
+    """fp32=True runs the FP32 stepping variant (dp_set_fp32_stepping) on the
+    same workload."""
     import ctypes as C
     geo, paths, rho, ne, nw, nq = adjoint_workload(dp, physics, ctx, torch)
+    ctx.set_fp32_stepping(fp32)
     dev = torch.device("cuda:0")
     T = lambda a, dt: torch.as_tensor(a, dtype=dt).to(dev)
     rp = T(paths.row_ptr, torch.int32)
@@ -169,6 +172,8 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
     ms_step = (t1 - t0) * 1e3 / steps
     kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
     ctx.set_stream(None)
+    ctx.set_fp32_stepping(False)
+    kname = "csda_cube_kernel_fp32" if fp32 else "csda_cube_kernel"
     units = paths.n_segments
     # algorithmic bytes of the forward+Jacobian kernel: 28 B per ray*voxel
     # (cell 4, length 8, density gather 8, Jacobian write 8) + 8 B per ray (flux)
@@ -180,8 +185,9 @@ def run_adjoint(dp, L, physics, ctx, torch, steps, warmup):
         "rays": paths.n_rays, "segments": units, "nodes": len(ne),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_BYTES_PER_UNIT["csda_cube_kernel"] * units},
-        "valu": valu_roofline("csda_cube_kernel", units, kern_ms),
+                     "traffic": (MEASURED_BYTES_PER_UNIT[kname] * units
+                                 if kname in MEASURED_BYTES_PER_UNIT else None)},
+        "valu": valu_roofline(kname, units, kern_ms),
         "flux_sum": float(flux_h.sum()), "grad_norm": float(grad.norm().item()),
         "_problem": (geo, paths, rho, ne, nw, nq),
         # the node arrays stay referenced: `nodes` holds their device addresses
@@ -977,6 +983,18 @@ def main():
         f"kernel {adj['kernel_ms']:.3f} ms")
     ctx.set_autotune(False)   # the per-ray comparators run the default launch shape
     per_ray_adj = run_adjoint_per_ray(dp, L, ctx, torch, adj)
+    ctx.set_autotune(True)
+    a32 = run_adjoint(dp, L, physics, ctx, torch, a.steps, a.warmup, fp32=True)
+    f64d, f32d = adj["_device"], a32.pop("_device")
+    a32.pop("_problem")
+    fe = ((f32d["flux"] - f64d["flux"]).abs() / f64d["flux"].abs()).max().item()
+    ge = ((f32d["jac"] - f64d["jac"]).abs().max() / f64d["jac"].abs().max()).item()
+    adj32 = {k: a32[k] for k in ("value", "ms_per_step", "kernel_ms", "roofline", "valu")}
+    adj32.update(max_rel_err_flux_vs_fp64=fe, max_err_jacobian_vs_fp64_rel_to_max=ge,
+                 speedup_vs_fp64=adj["kernel_ms"] / a32["kernel_ms"])
+    log(f"adjoint fp32 stepping: {a32['value']:.4g} rays*voxels/s, kernel {a32['kernel_ms']:.3f} ms, "
+        f"flux err vs fp64 {fe:.2g}")
+    ctx.set_autotune(False)
     log(f"adjoint per-ray loop: {per_ray_adj['device_ms']:.1f} ms device for "
         f"{per_ray_adj['launches']} launches, batched is {per_ray_adj['speedup_of_batched']:.0f}x "
         f"faster, bitwise equal {per_ray_adj['bitwise_equal_to_batched']}")
@@ -1047,7 +1065,8 @@ def main():
                     "rays": adj["rays"], "segments": adj["segments"], "nodes": adj["nodes"],
                     "ms_per_step": adj["ms_per_step"], "kernel_ms": adj["kernel_ms"],
                     "roofline": adj["roofline"], "valu": adj["valu"], "cpu_baseline": cpu_adj,
-                    "per_ray_loop": per_ray_adj},
+                    "per_ray_loop": per_ray_adj,
+                    "fp32_stepping": adj32},
         "wmodel": dict(wm, workload="water-fraction CSDA flux + sparse Jacobian, 64x64x32 "
                                     "voxels of 10 m, w U[0, 0.3], 4 detectors x 1024 "
                                     "directions, 32 nodes 1-1e4 GeV"),

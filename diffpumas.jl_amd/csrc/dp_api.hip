@@ -51,7 +51,13 @@ int tuned(dp_ctx c, const std::string& key, const std::vector<TuneCfg>& cands,
   }
   TuneEntry E;
   E.best_ms = 3.0e38f;
-  int rc = DP_OK;
+  // one untimed run first, so no candidate is timed on cold caches and pages
+  int rc = run(cands.front());
+  if (rc) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  }
   for (const TuneCfg& cfg : cands) {
     if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(DP_E_HIP, "autotune: hipEventRecord"); break; }
     if ((rc = run(cfg)) != DP_OK) break;
