@@ -62,10 +62,19 @@ struct StepOutT {
   F E;          // exit energy
   double lE;    // its log
   F ratio;      // S(E_out)/S(E_in)
-  F dec;        // decay factor of the segment
+  double dexp;  // decay exponent of the segment: decay factor exp(-dexp)
   F c, b, e;    // partials (see header)
   F k;          // dE_out/dX (X = grammage)
   bool ok;
+};
+
+// Entry-side lookups of a segment: range and stopping power at the entry
+// energy with their slopes, and 1/S.  A rock segment's exit computes them
+// for the next rock segment (same tables, same energy), which then skips
+// its own entry lookup.
+struct Entry64 {
+  double y0, y1, dy0, dy1, ry1;
+  bool valid;
 };
 using StepOut = StepOutT<double>;
 
@@ -74,6 +83,7 @@ using StepOut = StepOutT<double>;
 // their knot spacings (the FP64 spine of the walk), stopping power in float.
 struct Tab64 {
   const double *E, *LE, *R, *S;
+  const double *iLE, *iR;   // knot-spacing reciprocals (staged tables), or null
 };
 struct Tab32 {
   const double *LE, *R, *iLE, *iR;
@@ -90,80 +100,99 @@ constexpr double LOG_1E12 = 27.631021115928547;   // log(1e12): the "ok" energy 
 // segment's entry energy on the same tables.  Quotients that share a divisor
 // use one reciprocal (FP64 division costs ~10 quarter-rate instructions on
 // gfx950): values agree with the division form to a few ulp.
+__device__ __forceinline__ double inv_step(const double* inv, const double* x, int i) {
+  return inv ? inv[i] : 1.0 / (x[i + 1] - x[i]);
+}
+
+// `en` carries the entry lookups from the previous rock segment (en.valid)
+// and returns this segment's exit lookups for the next one.  The exit log
+// energy is the interpolated value whose exponential is the exit energy
+// (not re-rounded through log).  Reciprocals that a quantity shares are
+// taken once; the decay factor is returned as its exponent so the walk
+// multiplies one exponential per node.
 __device__ __forceinline__ StepOut csda_segment_g(const Tab64& T, int K, int pow2, double E,
                                                   double lE, double gram, double path,
-                                                  double inv_mass, double ctau, int& ie, int& jr) {
+                                                  double inv_mass, double ctau, double inv_ctau,
+                                                  int& ie, int& jr, Entry64& en) {
   const double *sE = T.E, *sLE = T.LE, *sR = T.R, *sS = T.S;
   StepOut o;
-  double y0, dy0, y1, dy1;
-  if (E <= sE[0]) {
-    y0 = sR[0]; y1 = sS[0]; dy0 = dy1 = 0.0;
-  } else if (E >= sE[K - 1]) {
-    y0 = sR[K - 1]; y1 = sS[K - 1]; dy0 = dy1 = 0.0;
-  } else {
-    const int i = ie >= 0 ? ie : locate_idx(sLE, K, pow2, lE, jr);
-    const double rdl = 1.0 / (sLE[i + 1] - sLE[i]);
-    const double t = (lE - sLE[i]) * rdl;
-    const double d0 = sR[i + 1] - sR[i], d1 = sS[i + 1] - sS[i];
-    y0 = sR[i] + t * d0;
-    y1 = sS[i] + t * d1;
-    const double re = rdl / E;
-    dy0 = d0 * re;
-    dy1 = d1 * re;
+  if (!en.valid) {
+    if (E <= sE[0]) {
+      en.y0 = sR[0]; en.y1 = sS[0]; en.dy0 = en.dy1 = 0.0;
+    } else if (E >= sE[K - 1]) {
+      en.y0 = sR[K - 1]; en.y1 = sS[K - 1]; en.dy0 = en.dy1 = 0.0;
+    } else {
+      const int i = ie >= 0 ? ie : locate_idx(sLE, K, pow2, lE, jr);
+      const double rdl = inv_step(T.iLE, sLE, i);
+      const double t = (lE - sLE[i]) * rdl;
+      const double d0 = sR[i + 1] - sR[i], d1 = sS[i + 1] - sS[i];
+      en.y0 = sR[i] + t * d0;
+      en.y1 = sS[i] + t * d1;
+      const double re = rdl / E;
+      en.dy0 = d0 * re;
+      en.dy1 = d1 * re;
+    }
+    en.ry1 = 1.0 / en.y1;
   }
+  const double y0 = en.y0, dy0 = en.dy0, dy1 = en.dy1, ry1 = en.ry1;
   const double Xf = y0 + gram;
   double Eo, kdy;
   if (Xf <= sR[0]) {
-    Eo = sE[0]; kdy = 0.0;
+    Eo = sE[0]; kdy = 0.0; o.lE = sLE[0];
   } else if (Xf >= sR[K - 1]) {
-    Eo = sE[K - 1]; kdy = 0.0;
+    Eo = sE[K - 1]; kdy = 0.0; o.lE = sLE[K - 1];
   } else {
     const int j = locate_idx(sR, K, pow2, Xf, jr);
     jr = j;
-    const double rdr = 1.0 / (sR[j + 1] - sR[j]);
+    const double rdr = inv_step(T.iR, sR, j);
     const double t = (Xf - sR[j]) * rdr;
     const double dle = sLE[j + 1] - sLE[j];
-    Eo = exp(sLE[j] + t * dle);
+    o.lE = sLE[j] + t * dle;
+    Eo = exp(o.lE);
     kdy = Eo * (dle * rdr);
   }
   o.E = Eo;
   o.ok = !(Eo >= 1e12) && isfinite(Eo);
-  o.lE = log(Eo);
-  double so_y, so_dy;
+  // exit lookups, which are also the next rock segment's entry lookups
   if (Eo <= sE[0]) {
-    so_y = sS[0]; so_dy = 0.0; ie = -1;
+    en.y0 = sR[0]; en.y1 = sS[0]; en.dy0 = en.dy1 = 0.0; ie = -1;
   } else if (Eo >= sE[K - 1]) {
-    so_y = sS[K - 1]; so_dy = 0.0; ie = -1;
+    en.y0 = sR[K - 1]; en.y1 = sS[K - 1]; en.dy0 = en.dy1 = 0.0; ie = -1;
   } else {
     const int i = locate_idx(sLE, K, pow2, o.lE, jr);
     ie = i;
-    const double rdl = 1.0 / (sLE[i + 1] - sLE[i]);
+    const double rdl = inv_step(T.iLE, sLE, i);
     const double t = (o.lE - sLE[i]) * rdl;
-    const double dyy = sS[i + 1] - sS[i];
-    so_y = sS[i] + t * dyy;
-    so_dy = dyy * (rdl / Eo);
+    const double d0 = sR[i + 1] - sR[i], d1 = sS[i + 1] - sS[i];
+    en.y0 = sR[i] + t * d0;
+    en.y1 = sS[i] + t * d1;
+    const double re = rdl / Eo;
+    en.dy0 = d0 * re;
+    en.dy1 = d1 * re;
   }
+  const double rso = 1.0 / en.y1;
+  en.ry1 = rso;
+  en.valid = true;
   // decay_weight with its log-derivative (compute_decay_weight_from_path,
   // src/Geometry.jl:1083-1107): beta^2 = 1 - 1/gamma^2 floored at 1e-12
-  const double g = 1.0 + 0.5 * (E + Eo) * inv_mass;
-  const double b2 = 1.0 - 1.0 / (g * g);
-  double beta, dbg_dg;
+  const double rg = 1.0 / (1.0 + 0.5 * (E + Eo) * inv_mass);   // 1/gamma
+  const double b2 = 1.0 - rg * rg;
+  double rbeta, dbg_dg;   // 1/beta, d(beta*gamma)/d gamma
   if (b2 >= 1e-12) {
-    beta = sqrt(b2);
-    dbg_dg = 1.0 / beta;
+    rbeta = 1.0 / sqrt(b2);
+    dbg_dg = rbeta;
   } else {
-    beta = 1e-6;
-    dbg_dg = beta;
+    rbeta = 1e6;
+    dbg_dg = 1e-6;
   }
-  const double rL = 1.0 / (beta * g * ctau);
-  o.dec = exp(-path * rL);
+  const double rL = rbeta * rg * inv_ctau;   // 1 / (beta gamma c tau)
+  o.dexp = path * rL;
   const double dlog = path * rL * rL * (ctau * dbg_dg * inv_mass);
-  const double ry1 = 1.0 / y1, rso = 1.0 / so_y;
-  o.ratio = so_y * ry1;
+  o.ratio = en.y1 * ry1;
   o.k = kdy;
   o.c = kdy * dy0;
   o.b = -dy1 * ry1 + 0.5 * dlog;
-  o.e = so_dy * rso + 0.5 * dlog;
+  o.e = en.dy1 * rso + 0.5 * dlog;
   return o;
 }
 
@@ -244,7 +273,7 @@ __device__ __forceinline__ StepOutT<float> csda_segment_g(const Tab32& T, int K,
     dbg_dg = beta;
   }
   const float rL = 1.f / (beta * g * ctau);
-  o.dec = expf(-path * rL);
+  o.dexp = (double)(path * rL);
   const float dlog = path * rL * rL * (ctau * dbg_dg * inv_mass);
   const float ry1 = 1.f / y1, rso = 1.f / so_y;
   o.ratio = so_y * ry1;
@@ -255,8 +284,7 @@ __device__ __forceinline__ StepOutT<float> csda_segment_g(const Tab32& T, int K,
   return o;
 }
 
-// 4 waves per SIMD (128 VGPRs, a 216-byte spill): measured 16 % faster than
-// the unbounded 198 VGPRs at 2 waves per SIMD, and 4 % faster than 3 waves.
+// WAVES: the occupancy bound (waves per SIMD), a tuned knob.
 // TLDS: the rock and air tables are staged in LDS; otherwise they are read
 // through L1/L2 from the table block.  Both give the same values; the
 // autotuner picks one per workload.  F: the type of the segment arithmetic.
@@ -264,8 +292,8 @@ __device__ __forceinline__ StepOutT<float> csda_segment_g(const Tab32& T, int K,
 // partials, the tape and the reverse sweep are float around the FP64
 // log-energy spine; the node weights, the flux model and every sum over
 // nodes stay FP64.
-template <bool TLDS, typename F>
-__global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
+template <bool TLDS, typename F, int WAVES>
+__global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
   constexpr bool F64 = sizeof(F) == sizeof(double);
   static_assert(TLDS || F64, "FP32 stepping stages its tables in LDS");
   using Tab = typename std::conditional<F64, Tab64, Tab32>::type;
@@ -297,19 +325,25 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
     air = {tl + 4 * K, tl + 5 * K, tl + 6 * K, tl + 7 * K, ts + K};
     tab_bytes = (size_t)K * 72;
   } else if constexpr (TLDS) {
+    // [E, LE, R, S, 1/dLE, 1/dR] of rock, then of air
     double* tl = lds;
     for (int i = tid; i < K; i += bd) {
+      const bool in = i + 1 < K;
       tl[i] = tr[0 * K + i]; tl[K + i] = tr[1 * K + i]; tl[2 * K + i] = tr[3 * K + i];
       tl[3 * K + i] = tr[2 * K + i];
-      tl[4 * K + i] = ta[0 * K + i]; tl[5 * K + i] = ta[1 * K + i]; tl[6 * K + i] = ta[3 * K + i];
-      tl[7 * K + i] = ta[2 * K + i];
+      tl[4 * K + i] = in ? 1.0 / (tr[K + i + 1] - tr[K + i]) : 0.0;
+      tl[5 * K + i] = in ? 1.0 / (tr[3 * K + i + 1] - tr[3 * K + i]) : 0.0;
+      tl[6 * K + i] = ta[0 * K + i]; tl[7 * K + i] = ta[1 * K + i]; tl[8 * K + i] = ta[3 * K + i];
+      tl[9 * K + i] = ta[2 * K + i];
+      tl[10 * K + i] = in ? 1.0 / (ta[K + i + 1] - ta[K + i]) : 0.0;
+      tl[11 * K + i] = in ? 1.0 / (ta[3 * K + i + 1] - ta[3 * K + i]) : 0.0;
     }
-    rock = {tl, tl + K, tl + 2 * K, tl + 3 * K};
-    air = {tl + 4 * K, tl + 5 * K, tl + 6 * K, tl + 7 * K};
-    tab_bytes = (size_t)K * 64;
+    rock = {tl, tl + K, tl + 2 * K, tl + 3 * K, tl + 4 * K, tl + 5 * K};
+    air = {tl + 6 * K, tl + 7 * K, tl + 8 * K, tl + 9 * K, tl + 10 * K, tl + 11 * K};
+    tab_bytes = (size_t)K * 96;
   } else {
-    rock = {tr, tr + K, tr + 3 * K, tr + 2 * K};
-    air = {ta, ta + K, ta + 3 * K, ta + 2 * K};
+    rock = {tr, tr + K, tr + 3 * K, tr + 2 * K, nullptr, nullptr};
+    air = {ta, ta + K, ta + 3 * K, ta + 2 * K, nullptr, nullptr};
   }
   // the table bytes (a multiple of 8) keep the doubles after them aligned
   double* wsum = (double*)((char*)lds + tab_bytes);  // [nw][max_seg]
@@ -321,6 +355,15 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
   const double rho0 = a.dens[0];
   const F inv_mass = F(1.0 / a.mass);
   const F ctau = F(a.ctau);
+  const double inv_ctau = 1.0 / a.ctau;
+  // one segment map: FP64 with the entry lookups carried in `en`, or FP32
+  auto seg = [&](const Tab& T, F E, double lE, double gram, F path, int& ie, int& jr,
+                 Entry64& en) -> StepOutT<F> {
+    if constexpr (F64)
+      return csda_segment_g(T, K, a.pow2, E, lE, gram, path, inv_mass, ctau, inv_ctau, ie, jr, en);
+    else
+      return csda_segment_g(T, K, a.pow2, E, lE, gram, path, inv_mass, ctau, ie, jr);
+  };
 
   for (;;) {
     if (tid == 0) s_ray = atomicAdd(a.queue, a.per_pop);
@@ -358,19 +401,20 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
         const double E0 = active ? a.ne[n] : a.ne[0];
         F E = F(E0);
         double lE = log(E0);
-        double W = 1.0;
+        double W = 1.0, X = 0.0;   // stopping-power ratios; summed decay exponents
         bool ok = true;
         int ie = -1, jr = K / 2;   // interval guesses (rock tables)
+        Entry64 en;
+        en.valid = false;
         // ------------------------------------------------ forward sweep
         for (int s = 0; s < ns; ++s) {
           const int cell = a.cell[b0 + s];
           const double d = a.len[b0 + s];
           const double rho = a.dens[cell];
-          StepOutT<F> o = csda_segment_g(rock, K, a.pow2, E, lE, d * rho, F(d), inv_mass, ctau,
-                                         ie, jr);
+          StepOutT<F> o = seg(rock, E, lE, d * rho, F(d), ie, jr, en);
           ok = ok && o.ok;
           W = W * o.ratio;
-          W = W * o.dec;
+          X += o.dexp;
           F* p = sc + (size_t)s * 4 * bd + tid;
           p[0] = o.c;
           p[bd] = o.b;
@@ -382,20 +426,21 @@ __global__ __launch_bounds__(256, 4) void csda_cube_kernel(CsdaArgs a) {
         StepOutT<F> orem;
         orem.c = orem.b = orem.e = orem.k = F(0);
         if (has_rem) {
-          orem = csda_segment_g(rock, K, a.pow2, E, lE, rem * rho0, F(rem), inv_mass, ctau, ie,
-                                jr);
+          orem = seg(rock, E, lE, rem * rho0, F(rem), ie, jr, en);
           ok = ok && orem.ok;
           W = W * orem.ratio;
-          W = W * orem.dec;
+          X += orem.dexp;
           E = orem.E;
           lE = orem.lE;
         }
         int ie_air = -1, jr_air = jr;   // air tables: a fresh entry search
-        StepOutT<F> oair = csda_segment_g(air, K, a.pow2, E, lE, Xair, F(path_air), inv_mass,
-                                          ctau, ie_air, jr_air);
+        Entry64 en_air;
+        en_air.valid = false;
+        StepOutT<F> oair = seg(air, E, lE, Xair, F(path_air), ie_air, jr_air, en_air);
         ok = ok && oair.ok;
         W = W * oair.ratio;
-        W = W * oair.dec;
+        X += oair.dexp;
+        W = W * exp(-X);
         double Es;   // surface energy
         if constexpr (F64) Es = oair.E; else Es = exp(oair.lE);
         Dd phi = flux_gccly_d(cosr, Es, active ? a.nq[n] : 0.0);
@@ -605,22 +650,34 @@ int check_paths(const dp_paths* p, const dp_csda_nodes* nd, int32_t n_cells, dp_
   return DP_OK;
 }
 
-// Tables: FP64 8 K doubles; FP32 stepping 8 K doubles (log energy, range and
+// Tables: FP64 12 K doubles (with the knot reciprocals); FP32 stepping 8 K doubles (log energy, range and
 // their knot reciprocals) and 2 K floats (stopping powers).
 size_t lds_bytes(bool tlds, int K, int nw, int max_seg, size_t fsize = sizeof(double)) {
-  const size_t tab = !tlds ? 0 : (size_t)K * (fsize == sizeof(float) ? 72 : 64);
+  const size_t tab = !tlds ? 0 : (size_t)K * (fsize == sizeof(float) ? 72 : 96);
   return tab + ((size_t)nw * max_seg + 2 * nw) * sizeof(double);
 }
 
-// Launch knobs of csda_cube_kernel: k0 = tables in LDS (1) or read through
-// L2 (0), k1 = rays per queue pop, k2 = grid multiplier over one resident
-// wave of blocks.  The first candidate is the default.  FP32 stepping always
-// stages its (float) tables in LDS.
+// Launch knobs of csda_cube_kernel: k0 = tables in LDS (bit 0: 1) or read
+// through L2 (0), plus 2 for the 3-wave occupancy bound (otherwise 4 waves
+// per SIMD); k1 = rays per queue pop; k2 = grid multiplier over one resident
+// wave of blocks.  The first candidate is the default: measured fastest at
+// the differentiation workload (FP64 10.5 ms at 3 waves against 14.9 ms at
+// 4, where its carried entry lookups spill; FP32 stepping 7.0 ms at 4 waves
+// against 7.8 ms at 3).  FP32 stepping always stages its tables in LDS.
 const std::vector<dph::TuneCfg>& csda_candidates(bool fp32) {
   static const std::vector<dph::TuneCfg> v = {
-      {1, 1, 1}, {0, 1, 1}, {1, 4, 1}, {0, 4, 1}, {1, 1, 2}, {0, 1, 2}};
-  static const std::vector<dph::TuneCfg> v32 = {{1, 1, 1}, {1, 4, 1}, {1, 1, 2}};
+      {3, 1, 1}, {1, 1, 1}, {2, 1, 1}, {3, 4, 1}, {3, 1, 2}, {1, 1, 2}, {2, 1, 2}};
+  static const std::vector<dph::TuneCfg> v32 = {{1, 1, 1}, {3, 1, 1}, {1, 4, 1}, {1, 1, 2},
+                                                {3, 1, 2}};
   return fp32 ? v32 : v;
+}
+
+using CsdaKernel = void (*)(CsdaArgs);
+CsdaKernel csda_kernel_for(bool f32, int k0) {
+  const bool tlds = (k0 & 1) != 0, w3 = (k0 & 2) != 0;
+  if (f32) return w3 ? csda_cube_kernel<true, float, 3> : csda_cube_kernel<true, float, 4>;
+  if (tlds) return w3 ? csda_cube_kernel<true, double, 3> : csda_cube_kernel<true, double, 4>;
+  return w3 ? csda_cube_kernel<false, double, 3> : csda_cube_kernel<false, double, 4>;
 }
 
 int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* nd,
@@ -642,18 +699,11 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
   std::vector<int> grids;
   int grid_max = 1;
   for (const auto& cfg : all) {
-    const size_t lds = lds_bytes(cfg.k0 != 0, K, nw, max_seg, fsz);
+    const size_t lds = lds_bytes((cfg.k0 & 1) != 0, K, nw, max_seg, fsz);
     if (lds > 160 * 1024) continue;
     int occ = 0;
-    if (f32)
-      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<true, float>, bd,
-                                                          lds));
-    else if (cfg.k0)
-      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<true, double>, bd,
-                                                          lds));
-    else
-      DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_cube_kernel<false, double>,
-                                                          bd, lds));
+    DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csda_kernel_for(f32, cfg.k0), bd,
+                                                        lds));
     if (occ < 1) occ = 1;
     long g = (long)c->n_cu * occ * cfg.k2;
     const long pops = (p->n_rays + cfg.k1 - 1) / cfg.k1;
@@ -685,19 +735,12 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
     if (i == (int)cands.size()) i = 0;
     CsdaArgs b = a;
     b.per_pop = cands[i].k1;
-    const size_t lds = lds_bytes(cands[i].k0 != 0, K, nw, max_seg, fsz);
+    const size_t lds = lds_bytes((cands[i].k0 & 1) != 0, K, nw, max_seg, fsz);
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, c->stream, queue, err,
                        reset_err ? 1 : 0);
     dph::record(c, c->ev0);
-    if (f32)
-      hipLaunchKernelGGL((csda_cube_kernel<true, float>), dim3(grids[i]), dim3(bd), lds, c->stream,
-                         b);
-    else if (cands[i].k0)
-      hipLaunchKernelGGL((csda_cube_kernel<true, double>), dim3(grids[i]), dim3(bd), lds,
-                         c->stream, b);
-    else
-      hipLaunchKernelGGL((csda_cube_kernel<false, double>), dim3(grids[i]), dim3(bd), lds,
-                         c->stream, b);
+    hipLaunchKernelGGL(csda_kernel_for(f32, cands[i].k0), dim3(grids[i]), dim3(bd), lds,
+                       c->stream, b);
     dph::record(c, c->ev1);
     DP_HIP(hipGetLastError());
     return DP_OK;
