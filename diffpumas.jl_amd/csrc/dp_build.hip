@@ -5,6 +5,7 @@
 // Runs once per physics object; the transport kernels only read its output.
 #include "dp_internal.hpp"
 #include "dp_models.hpp"
+#include "dp_atomic_shells.hpp"
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -502,9 +503,169 @@ int check_recipe(const dp_material_recipe* r) {
   return DP_OK;
 }
 
+// ------------------------------------------------- electronic stopping
+// Atomic shells of a material (atomic_shell_create, src/Materials.jl:378-436):
+// every element's Geant4 shells with occupancy weighted by its mass fraction
+// over A (the fictitious Z = 11, A = 22 standard-rock element takes the
+// shells of C, 3 O and Ca scaled by 11/50), then oscillator strengths
+// normalised to 1 and binding energies scaled so that the strength-weighted
+// mean log energy equals log(I) in units of the plasma energy.
+struct Shell {
+  double f, E;
+};
+int shell_count(int Z) {
+  const int z = std::min(std::max(Z, 1), dpshell::SHELL_N_Z);
+  return dpshell::SHELL_INDEX[z] - dpshell::SHELL_INDEX[z - 1];
+}
+void add_shells(std::vector<Shell>& out, int Z, double w) {
+  const int z = std::min(std::max(Z, 1), dpshell::SHELL_N_Z);
+  for (int i = dpshell::SHELL_INDEX[z - 1]; i < dpshell::SHELL_INDEX[z]; ++i)
+    out.push_back({w * (double)dpshell::SHELL_OCCUPANCY[i], (double)dpshell::SHELL_ENERGY_EV[i]});
+}
+std::vector<Shell> atomic_shells(const std::vector<Elem>& el, double ZoA, double I,
+                                 double density) {
+  std::vector<Shell> sh;
+  for (const Elem& e : el) {
+    const double w = e.f / e.A;
+    if (e.Z == 11.0 && e.A == 22.0) {
+      const double ws = w * 11.0 / (6.0 + 3 * 8.0 + 20.0);
+      add_shells(sh, 6, ws);
+      add_shells(sh, 8, 3 * ws);
+      add_shells(sh, 20, ws);
+    } else {
+      add_shells(sh, (int)std::lround(e.Z), w);
+    }
+  }
+  double ftot = 0.0, lnI = 0.0;
+  for (const Shell& s : sh) {
+    ftot += s.f;
+    lnI += s.f * std::log(s.E);
+  }
+  const double inv_ftot = 1.0 / ftot;
+  lnI *= inv_ftot;
+  const double wp = 28.816 * std::sqrt(ZoA * density * 1e-3);
+  const double aS = I * 1e9 / std::exp(lnI);
+  const double r = aS / wp;
+  for (Shell& s : sh) {
+    s.f *= inv_ftot;
+    s.E *= r;
+  }
+  return sh;
+}
+
+// Fano density effect delta(gamma) by the bisection of PUMAS
+// (electronic_density_effect, src/Materials.jl:442-475): the root l2 of
+// sum f/(E^2 + l2) = 1/gamma^2, then delta = sum f log(1 + l2/E^2) - l2/gamma^2.
+double density_effect(const std::vector<Shell>& sh, double gamma) {
+  const double y = 1.0 / (gamma * gamma);
+  double ymax = 0.0;
+  for (const Shell& s : sh) ymax += s.f / (s.E * s.E);
+  if (ymax <= y) return 0.0;
+  const double eps = 2.220446049250313e-16;
+  double l2min = 0.0, l2max = 1.0 / y, l2 = 0.0;
+  for (int it = 0; it < 200; ++it) {
+    l2 = 0.5 * (l2min + l2max);
+    double yi = 0.0;
+    for (const Shell& s : sh) yi += s.f / (s.E * s.E + l2);
+    if (std::fabs(yi - y) <= eps || (l2max - l2min) <= eps) break;
+    if (yi > y) l2min = l2;
+    else l2max = l2;
+  }
+  double delta = -l2 * y;
+  for (const Shell& s : sh) delta += s.f * std::log(1.0 + l2 / (s.E * s.E));
+  return delta;
+}
+
+// Electronic stopping power, GeV m2/kg: Bethe with the maximum energy
+// transfer Qmax, the radiative correction Delta and the Fano density effect
+// (_electronic_energy_loss, src/Materials.jl:500-520).
+double electronic_loss(double ZoA, double I, const std::vector<Shell>& sh, double mass,
+                       double K) {
+  const double E = K + mass;
+  const double P2 = K * (K + 2.0 * mass);
+  const double beta2 = P2 / (E * E);
+  const double gamma = E / mass;
+  const double r = ELECTRON_MASS / mass;
+  const double Qmax = 2.0 * r * P2 / (mass * (1.0 + r * r) + 2.0 * r * E);
+  const double lQ = std::log(1.0 + 2.0 * Qmax / ELECTRON_MASS);
+  const double Delta = ALPHA_EM / (2 * PI) * (std::log(2.0 * gamma) - lQ / 3.0) * lQ * lQ;
+  const double delta = density_effect(sh, gamma);
+  return 2 * PI * ELECTRON_RADIUS * ELECTRON_RADIUS * ELECTRON_MASS * AVOGADRO_NUMBER * ZoA /
+         (beta2 * 1e-3) *
+         (std::log(2.0 * ELECTRON_MASS * beta2 * gamma * gamma * Qmax / (I * I)) - 2.0 * beta2 -
+          delta + 0.25 * Qmax * Qmax / (E * E) + Delta);
+}
+
 }  // namespace
 
 extern "C" {
+
+double dp_electronic_stopping_power(const dp_material_recipe* r, double mass, double K) {
+  if (check_recipe(r)) return NAN;
+  const std::vector<Elem> el = elements_of(r);
+  return electronic_loss(r->ZoA, r->I_material,
+                         atomic_shells(el, r->ZoA, r->I_material, r->density), mass, K);
+}
+
+double dp_density_effect(const dp_material_recipe* r, double gamma) {
+  if (check_recipe(r)) return NAN;
+  const std::vector<Elem> el = elements_of(r);
+  return density_effect(atomic_shells(el, r->ZoA, r->I_material, r->density), gamma);
+}
+
+int dp_build_material_table_empirical(const dp_material_recipe* r, const double* energies,
+                                      int32_t n, const dp_physics_settings* st, double* out) {
+  int rc = check_recipe(r);
+  if (rc) return rc;
+  if (!energies || n <= 2 || !st || !out)
+    return dph::fail(DP_E_ARGUMENT, "dp_build_material_table_empirical: bad arguments");
+  for (int i = 0; i < n; ++i)
+    if (!(energies[i] > 0) || (i > 0 && !(energies[i] > energies[i - 1])))
+      return dph::fail(DP_E_ARGUMENT,
+                       "dp_build_material_table_empirical: energies must be positive and increasing");
+  const int K = n;
+  const double mass = st->mass, cutoff = st->cutoff;
+  for (size_t i = 0; i < (size_t)DP_T_NARRAYS * K; ++i) out[i] = 0.0;
+  auto col = [&](int a) { return out + (size_t)a * K; };
+  const std::vector<Elem> el = elements_of(r);
+  const std::vector<Shell> sh = atomic_shells(el, r->ZoA, r->I_material, r->density);
+  const int npts = 180;
+  const double x_low = 1e-6;
+  for (int i = 0; i < K; ++i) {
+    const double Ki = energies[i];
+    col(DP_T_ENERGY)[i] = Ki;
+    double brem = 0.0, pair = 0.0, photo = 0.0, hard = 0.0, cs = 0.0, strag = 0.0;
+    if (Ki >= 1e-6) {
+      for (const Elem& e : el) {
+        brem += e.f * dcs_integral(BREMS, e.Z, e.A, e.I, mass, Ki, x_low, 1.0, 1, npts);
+        pair += e.f * dcs_integral(PAIR, e.Z, e.A, e.I, mass, Ki, x_low, 1.0, 1, npts);
+        photo += e.f * dcs_integral(PHOTO, e.Z, e.A, e.I, mass, Ki, x_low, 1.0, 1, npts);
+        for (int p : {BREMS, PAIR, PHOTO, IONISATION}) {
+          hard += e.f * dcs_integral(p, e.Z, e.A, e.I, mass, Ki, cutoff, 1.0, 1, npts);
+          cs += e.f * dcs_integral(p, e.Z, e.A, e.I, mass, Ki, cutoff, 1.0, 0, npts);
+        }
+        strag += e.f * dcs_integral(IONISATION, e.Z, e.A, e.I, mass, Ki, 0.0, cutoff, 2, npts);
+      }
+    }
+    const double elec = electronic_loss(r->ZoA, r->I_material, sh, mass, Ki);
+    col(DP_T_IONIZATION)[i] = elec;
+    col(DP_T_BREMS)[i] = brem;
+    col(DP_T_PAIR)[i] = pair;
+    col(DP_T_PHOTO)[i] = photo;
+    col(DP_T_CSDA_STOP)[i] = elec + brem + pair + photo;
+    col(DP_T_MIXED_STOP)[i] = std::max(col(DP_T_CSDA_STOP)[i] - hard, 1e-20);
+    col(DP_T_STRAGGLING)[i] = strag;
+    col(DP_T_CROSS_SECTION)[i] = cs;
+  }
+  integrate_range(col(DP_T_ENERGY), col(DP_T_CSDA_STOP), col(DP_T_CSDA_RANGE), K);
+  integrate_range(col(DP_T_ENERGY), col(DP_T_MIXED_STOP), col(DP_T_MIXED_RANGE), K);
+  coulomb_tables(el, mass, st->elastic_ratio, cutoff, col(DP_T_ENERGY), col(DP_T_CSDA_RANGE), K,
+                 r->ZoA, r->I_material, r->density, col(DP_T_TRANSPORT_PATH),
+                 col(DP_T_ELASTIC_PATH), col(DP_T_SCREENING));
+  kinematic_columns(out, K, mass, true);
+  return DP_OK;
+}
+
 
 double dp_dcs(int32_t process, double Z, double A, double I, double mass, double K, double q) {
   if (process == IONISATION) return dcs_ionisation(Z, I, mass, K, q);

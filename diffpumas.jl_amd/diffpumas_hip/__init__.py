@@ -7,7 +7,8 @@ All compute goes through libdiffpumas_hip.so; there is no CPU fallback.
 from ._lib import (ArgumentError, CapacityError, Context, DomainError, HipError,
                    TransportError, default_context, load)
 from .physics import (PhysicsTables, create_physics, create_physics_csda, load_dedx_file, load_mdf,
-                      MUON_MASS, MUON_C_TAU)
+                      MUON_MASS, MUON_C_TAU, builtin_materials, create_energy_grid,
+                      electronic_stopping_power, electronic_density_effect)
 from .cube import (CubeGeometry, RayPaths, compute_flux_gradient_cube, csda_cube_flux,
                    csda_cube_vjp, ray_direction, trace_cube_paths, trace_edge_paths)
 from .topography import ElevationMap, ray_surface_exit_distance

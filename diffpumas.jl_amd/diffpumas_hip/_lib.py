@@ -256,6 +256,11 @@ _SIGS = {
                                 [C.c_int32] + [C.c_void_p] * 3),
     "dp_residual_dev": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
+    "dp_build_material_table_empirical": (C.c_int, [C.POINTER(MaterialRecipe), dbl_p, C.c_int32,
+                                                    C.POINTER(PhysicsSettings), dbl_p]),
+    "dp_electronic_stopping_power": (C.c_double, [C.POINTER(MaterialRecipe), C.c_double,
+                                                  C.c_double]),
+    "dp_density_effect": (C.c_double, [C.POINTER(MaterialRecipe), C.c_double]),
     "dp_build_material_table": (C.c_int, [C.POINTER(MaterialRecipe), C.POINTER(DedxColumns),
                                           C.POINTER(PhysicsSettings), dbl_p]),
     "dp_build_composite_table": (C.c_int, [C.c_int32, C.POINTER(dbl_p), dbl_p, C.c_int32,

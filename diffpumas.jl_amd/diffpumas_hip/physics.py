@@ -10,6 +10,7 @@ import ctypes as C
 import math
 import os
 import re
+import warnings
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -120,6 +121,44 @@ class BaseMaterial:
             num = sum(f * e.Z / e.A * math.log(e.I) for e, f in zip(self.elements, self.fractions))
             I = math.exp(num / self.ZoA)
         self.I = float(I)
+
+
+def builtin_materials():
+    """The reference's built-in recipes (ELEMENTS, STANDARD_ROCK, AIR, WATER;
+    src/Materials.jl:35-127): elements with Z, A and I (GeV), material
+    density (kg/m3), mean excitation energy and mass fractions.  These are the
+    materials create_physics tabulates when it has no PUMAS dE/dx file."""
+    E = {n: Element(n, Z, A, I * 1e-9) for n, Z, A, I in (
+        ("H", 1, 1.00794, 19.2), ("C", 6, 12.0107, 78.0), ("N", 7, 14.0067, 82.0),
+        ("O", 8, 15.9994, 95.0), ("Na", 11, 22.9898, 149.0), ("Mg", 12, 24.305, 156.0),
+        ("Al", 13, 26.9815, 166.0), ("Si", 14, 28.0855, 173.0), ("K", 19, 39.0983, 190.0),
+        ("Ca", 20, 40.078, 191.0), ("Fe", 26, 55.845, 286.0), ("Ar", 18, 39.948, 188.0))}
+    mats = {
+        "StandardRock": (2650.0, 136.4e-9, ("O", "Si", "Al", "Fe", "Ca", "Na", "Mg", "K"),
+                         (0.466, 0.277, 0.081, 0.050, 0.036, 0.028, 0.021, 0.026)),
+        "Air": (1.205, 85.7e-9, ("N", "O", "Ar"), (0.755, 0.232, 0.013)),
+        "Water": (1000.0, 75.0e-9, ("H", "O"), (0.1119, 0.8881)),
+    }
+    return {n: BaseMaterial(n, rho, [E[e] for e in els], fr, I)
+            for n, (rho, I, els, fr) in mats.items()}
+
+
+def create_energy_grid(n_energies, K_min, K_max):
+    """create_energy_grid (src/Physics.jl:125-127): log-spaced kinetic energies."""
+    return np.array([K_min * (K_max / K_min) ** (i / (n_energies - 1))
+                     for i in range(n_energies)])
+
+
+def electronic_stopping_power(material, energy, mass=MUON_MASS):
+    """electronic_stopping_power (src/Materials.jl:486-520), GeV m2/kg."""
+    r, keep = _recipe(material)
+    return float(_lib.load().dp_electronic_stopping_power(C.byref(r), mass, float(energy)))
+
+
+def electronic_density_effect(material, gamma):
+    """The Fano density effect delta(gamma) (src/Materials.jl:442-480)."""
+    r, keep = _recipe(material)
+    return float(_lib.load().dp_density_effect(C.byref(r), float(gamma)))
 
 
 def load_mdf(path):
@@ -327,36 +366,59 @@ def flatten_composite(name, components, fractions):
 
 
 def create_physics(materials=("StandardRock", "Water", "Air"), composites=(), data_dir=None,
-                   mdf=None, cutoff=0.05, elastic_ratio=0.05):
-    """create_physics (src/Physics.jl:534-599) for tabulated materials: every
-    MaterialTable array, built by the library's host table builder from the
-    PUMAS dE/dx files and the MDF element recipes."""
+                   mdf=None, cutoff=0.05, elastic_ratio=0.05, n_energies=200, K_min=1e-3,
+                   K_max=1e9, dedx_dir=None):
+    """create_physics (src/Physics.jl:534-599): every MaterialTable array,
+    built by the library's host table builder.  A material with a PUMAS dE/dx
+    file in dedx_dir (default: the package's pumas-materials/dedx/muon) is
+    tabulated on the file's knots with its MDF recipe
+    (create_material_table_from_dedx); one without it, as in the reference,
+    on create_energy_grid(n_energies, K_min, K_max) from the empirical model
+    (create_material_table_empirical) with the reference's built-in recipe
+    of that name (or the MDF one), with a warning.  All materials must end
+    up on one knot grid."""
     data_dir = data_dir or default_data_dir()
     mdf = mdf or os.path.join(data_dir, "materials.xml")
+    dedx_dir = dedx_dir or os.path.join(data_dir, "pumas-materials", "dedx", "muon")
     _, recipes, comp_defs = load_mdf(mdf)
+    builtin = builtin_materials()
     lib = _lib.load()
     settings = _lib.PhysicsSettings(cutoff, elastic_ratio, MUON_MASS)
     names, tabs, scal = [], [], []
     K = None
     for name in materials:
-        d = load_dedx_file(os.path.join(data_dir, "pumas-materials", "dedx", "muon",
-                                        dedx_file_name(name)))
-        n = len(d.kinetic_energy)
+        path = os.path.join(dedx_dir, dedx_file_name(name))
+        if os.path.isfile(path):
+            d = load_dedx_file(path)
+            n = len(d.kinetic_energy)
+            rec = recipes[name]
+            r, keep = _recipe(rec)
+            cols = [d.kinetic_energy, d.ionization, d.bremsstrahlung, d.pair_production,
+                    d.photonuclear, d.stopping_power, d.csda_range]
+            cols = [np.ascontiguousarray(c) for c in cols]
+            dc = _lib.DedxColumns(n, *[_lib.ptr(c) for c in cols], d.ZoA)
+            out = np.zeros((T_NARRAYS, n))
+            _lib.check(lib.dp_build_material_table(C.byref(r), C.byref(dc), C.byref(settings),
+                                                   _lib.ptr(out)))
+            zoa = d.ZoA
+        else:
+            warnings.warn(f"PUMAS dE/dx file not found for {name} at {path}. "
+                          "Using empirical formulas.")
+            rec = builtin.get(name) or recipes[name]
+            recipes[name] = rec
+            energies = np.ascontiguousarray(create_energy_grid(n_energies, K_min, K_max))
+            n = len(energies)
+            r, keep = _recipe(rec)
+            out = np.zeros((T_NARRAYS, n))
+            _lib.check(lib.dp_build_material_table_empirical(C.byref(r), _lib.ptr(energies), n,
+                                                             C.byref(settings), _lib.ptr(out)))
+            zoa = rec.ZoA
         K = n if K is None else K
         if n != K:
             raise ValueError("all materials must share one energy grid")
-        rec = recipes[name]
-        r, keep = _recipe(rec)
-        cols = [d.kinetic_energy, d.ionization, d.bremsstrahlung, d.pair_production,
-                d.photonuclear, d.stopping_power, d.csda_range]
-        cols = [np.ascontiguousarray(c) for c in cols]
-        dc = _lib.DedxColumns(n, *[_lib.ptr(c) for c in cols], d.ZoA)
-        out = np.zeros((T_NARRAYS, n))
-        _lib.check(lib.dp_build_material_table(C.byref(r), C.byref(dc), C.byref(settings),
-                                               _lib.ptr(out)))
         names.append(name)
         tabs.append(out)
-        scal.append([rec.density, rec.I, d.ZoA, 0.0])
+        scal.append([rec.density, rec.I, zoa, 0.0])
     for cname in composites:
         parts = comp_defs[cname]
         comps = [recipes[p] for p, _ in parts]

@@ -191,6 +191,20 @@ int dp_build_composite_table(int32_t n_components, const double* const* tables,
                              const double* fractions, int32_t n_knots,
                              const dp_material_recipe* flat, const dp_physics_settings* settings,
                              double* out);
+/* Replaces: create_material_table_empirical (src/Physics.jl:280-408), the
+ * table create_physics builds when a material has no PUMAS dE/dx file:
+ * out[DP_T_NARRAYS * n] on the given increasing kinetic energies (GeV), with
+ * the electronic stopping power of the Fano model and the radiative ones
+ * from the DCS integrals.                                                  */
+int dp_build_material_table_empirical(const dp_material_recipe* recipe, const double* energies,
+                                      int32_t n, const dp_physics_settings* settings,
+                                      double* out);
+/* electronic_stopping_power (src/Materials.jl:486-520; GeV m2/kg) and the
+ * Fano density effect delta(gamma) (electronic_density_effect, :442-480) of
+ * a material, with its atomic shells built as atomic_shell_create (:378-436).
+ * NaN for an invalid recipe.                                                */
+double dp_electronic_stopping_power(const dp_material_recipe* recipe, double mass, double K);
+double dp_density_effect(const dp_material_recipe* recipe, double gamma);
 /* Per-atom dσ/dq (m2/GeV) of process 0 brems, 1 pair, 2 photonuclear,
  * 3 ionisation (src/Materials.jl:534-926), and its moment integral per kg
  * of element (compute_dcs_integral, src/Materials.jl:939-1007). */

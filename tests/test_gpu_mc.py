@@ -139,3 +139,24 @@ def test_stochastic_deterministic_and_gradient(ph3, ctx):
 def test_stochastic_rejects_bad_threshold(ph3, ctx):
     with pytest.raises(dp.ArgumentError):
         dp.run_backward_mc(ph3, n_samples=10, source_energy_max=1e12, ctx=ctx)
+
+
+def test_golden_air_only_flux_on_gpu(ctx):
+    """The reference's deterministic logged flux (test/test_geometry_ad.jl:
+    81-89 on create_physics(MUON; n_energies=50, K_min=0.01, K_max=1e6) with
+    empirical tables): 0.4610008899907813 in the current reference logs
+    (docs/revision_evidence/integrated_kernel_tests/final_precision_fix/
+    julia-1.11-pkg-tests.txt).  That event drew a positive charge; seed 1
+    draws one here too.  Zero rock: gradient 0, gradient flux = flux."""
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ph = dp.create_physics(("StandardRock", "Air"), n_energies=50, K_min=0.01, K_max=1e6,
+                               dedx_dir="/nonexistent-dedx-dir")
+    r = dp.run_backward_mc(ph, rock_thickness=0.0, rock_density=2650.0, elevation=90.0,
+                           energy_min=10.0, n_samples=1, seed=1, compute_gradient=True,
+                           straggling=False, scattering=False, ctx=ctx)
+    _say(f"air-only flux on GPU {r['flux']!r} (reference 0.4610008899907813)")
+    assert r["flux"] == pytest.approx(0.4610008899907813, rel=1e-13)
+    assert r["gradient"] == 0.0 and r["sigma"] == 0.0
+    assert r["gradient_flux"] == pytest.approx(r["flux"], rel=1e-15)
