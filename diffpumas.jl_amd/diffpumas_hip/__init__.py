@@ -28,6 +28,8 @@ from .tomography import (DirectionalPaths, EnergySamples, MaterialConfig, SiteCo
                          set_csda_correction, get_csda_correction,
                          set_csda_material_correction, get_csda_material_correction)
 
+from .sampling import sample_del_event, sample_mixture_material
+
 __all__ = [
     "ArgumentError", "CapacityError", "Context", "DomainError", "HipError", "TransportError",
     "default_context", "load", "PhysicsTables", "create_physics", "create_physics_csda", "load_dedx_file",
@@ -45,4 +47,6 @@ __all__ = [
     "compute_directional_flux_mc_and_csda",
     "make_csda_operator", "normal_equations", "calibrate_csda_correction", "path_rock_length",
     "TetraVolume", "build_tetra_neighbors", "trace_tetra_paths", "trace_tetra_rays",
+    "sample_del_event", "sample_mixture_material", "builtin_materials", "create_energy_grid",
+    "electronic_stopping_power", "electronic_density_effect",
 ]

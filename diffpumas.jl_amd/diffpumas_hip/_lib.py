@@ -261,6 +261,10 @@ _SIGS = {
     "dp_electronic_stopping_power": (C.c_double, [C.POINTER(MaterialRecipe), C.c_double,
                                                   C.c_double]),
     "dp_density_effect": (C.c_double, [C.POINTER(MaterialRecipe), C.c_double]),
+    "dp_sample_mixture_material": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, i32_p, dbl_p,
+                                             C.c_int64, dbl_p, C.c_uint64, i32_p]),
+    "dp_sample_del_event": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int64] + [dbl_p] * 5 +
+                            [C.c_int32, C.c_uint64, C.POINTER(C.c_uint8), dbl_p, dbl_p, i32_p]),
     "dp_build_material_table": (C.c_int, [C.POINTER(MaterialRecipe), C.POINTER(DedxColumns),
                                           C.POINTER(PhysicsSettings), dbl_p]),
     "dp_build_composite_table": (C.c_int, [C.c_int32, C.POINTER(dbl_p), dbl_p, C.c_int32,

@@ -609,6 +609,27 @@ int dp_normal_equations_dev(dp_ctx ctx, const dp_csr* J, const double* weights,
 int dp_residual_dev(dp_ctx ctx, int64_t n, const double* pred, const double* obs,
                     const double* weights, double* residual, double* cost);
 
+/* Batched forms of two exported samplers the current transport path no
+ * longer calls.  Sample i draws its uniforms, in the reference's order, from
+ * Philox keyed (seed, i).  Host pointers; return after synchronising.
+ *
+ * Replaces: sample_mixture_material (src/Physics.jl:1331-1365): for each
+ * energy, the material of an interaction in the mixture (1-8 components,
+ * fractions normalised as MaterialMixture does), drawn by fraction x
+ * tabulated hard cross-section (by fraction alone when that total is 0).  */
+int dp_sample_mixture_material(dp_ctx ctx, dp_tables tables, int32_t n_components,
+                               const int32_t* materials, const double* fractions, int64_t n,
+                               const double* energies, uint64_t seed, int32_t* out);
+/* Replaces: sample_del_event (src/Straggling.jl:217-317), the legacy
+ * discrete-energy-loss sampler over a step of grammage dX ending at Xtot:
+ * occurred (0/1), the interaction grammage X_del, the energy after the event
+ * k_del and the process (1 brems, 2 pair, 3 photonuclear; 0 when none), with
+ * the tables' cutoff as the lower end of the 1/nu^2 transfer law.          */
+int dp_sample_del_event(dp_ctx ctx, dp_tables tables, int32_t material, int64_t n,
+                        const double* ki, const double* kf, const double* dX, const double* Xtot,
+                        const double* ratio, int32_t backward, uint64_t seed, uint8_t* occurred,
+                        double* X_del, double* k_del, int32_t* process);
+
 /* Last kernel duration in ms, timed with HIP events on the context's stream
  * around the dominant kernel of the previous call (0 if none). */
 double dp_last_kernel_ms(dp_ctx ctx);

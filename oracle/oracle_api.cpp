@@ -967,6 +967,125 @@ int orc_ehs_sample(int M, int K, const double* data, const double* scal, double 
   return rng.script_pos;
 }
 
+// sample_mixture_material, src/Physics.jl:1331-1365: fractions as given
+// (MaterialMixture normalises positive sums, src/Types.jl:360-369).
+static int sample_mixture_material_o(const Tables& T, int n, const int32_t* mats,
+                                     const double* fr, double energy, Philox4x32& rng) {
+  if (n == 1) return mats[0];
+  double tot = 0.0;
+  for (int j = 0; j < n; ++j) tot += fr[j] * property(T, mats[j], T_CROSS_SECTION, energy);
+  if (tot <= 0.0) {
+    const double u = rng.uniform();
+    double c = 0.0;
+    for (int j = 0; j < n; ++j) {
+      c += fr[j];
+      if (u <= c) return mats[j];
+    }
+    return mats[n - 1];
+  }
+  const double u = rng.uniform() * tot;
+  double c = 0.0;
+  for (int j = 0; j < n; ++j) {
+    c += fr[j] * property(T, mats[j], T_CROSS_SECTION, energy);
+    if (u <= c) return mats[j];
+  }
+  return mats[n - 1];
+}
+
+// compute_del_cross_section, src/Straggling.jl:186-199
+static double del_cross_section_o(const Tables& T, int m, double k) {
+  if (k < 1e-3) return 0.0;
+  return std::max(property(T, m, T_CROSS_SECTION, k), 0.0);
+}
+
+// sample_del_event, src/Straggling.jl:217-317 (BMC_ALPHA = 2).
+// out: occurred, X_del, k_del, process.
+static void sample_del_event_o(const Tables& T, int m, double ki, double kf, double dX,
+                               double Xtot, double ratio, double cutoff, bool backward,
+                               Philox4x32& rng, double* out) {
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  const double sgn = backward ? -1.0 : 1.0;
+  const double xs = std::max(del_cross_section_o(T, m, ki), del_cross_section_o(T, m, kf));
+  if (xs <= 0.0) return;
+  const double zeta = rng.uniform();
+  if (zeta <= 1e-12 || zeta >= 1.0 - 1e-12) return;
+  const double X = -std::log(zeta) / xs;
+  if (X >= dX) return;
+  const double kh = prop_kinetic(T, true, m, Xtot - sgn * X);
+  const double kat = ki - sgn * ratio * std::fabs(ki - kh);
+  if (kat <= 0.0) return;
+  const double r = del_cross_section_o(T, m, kat) / xs;
+  if (rng.uniform() > r) return;
+  const double alpha = 2.0;
+  const double u = rng.uniform();
+  const double ct = std::pow(cutoff, 1.0 - alpha);
+  double nu = std::pow(u * (1.0 - ct) + ct, 1.0 / (1.0 - alpha));
+  nu = std::min(std::max(nu, cutoff), 1.0);
+  const double kd = backward ? kat / (1.0 - nu) : kat * (1.0 - nu);
+  const double b = property(T, m, T_BREMS, kat), p = property(T, m, T_PAIR, kat),
+               ph = property(T, m, T_PHOTO, kat);
+  const double tot = b + p + ph;
+  int proc = 1;
+  if (tot > 0.0) {
+    const double rp = rng.uniform() * tot;
+    proc = rp < b ? 1 : (rp < b + p ? 2 : 3);
+  }
+  out[0] = 1.0; out[1] = X; out[2] = kd; out[3] = proc;
+}
+
+// n seeded draws (sample i from Philox(seed, i)), or one scripted call
+// (returns the draws consumed).
+int orc_sample_mixture_material(int M, int K, const double* data, const double* scal,
+                                int n_mix, const int32_t* mats, const double* fracs, int n,
+                                const double* energies, unsigned long long seed,
+                                const double* script, int n_script, int32_t* out) {
+  Tables T = make_tables(M, K, data, scal, MUON_MASS, MUON_C_TAU);
+  double tot = 0.0;
+  for (int j = 0; j < n_mix; ++j) tot += fracs[j];
+  std::vector<double> fr(n_mix);
+  for (int j = 0; j < n_mix; ++j) fr[j] = tot > 0 ? fracs[j] / tot : fracs[j];
+  if (n_script > 0) {
+    Philox4x32 rng(0, 0);
+    rng.script = script; rng.script_n = n_script;
+    try {
+      out[0] = sample_mixture_material_o(T, n_mix, mats, fr.data(), energies[0], rng);
+    } catch (const std::exception&) {
+      return -1;   // the script ran out of draws
+    }
+    return rng.script_pos;
+  }
+  for (int i = 0; i < n; ++i) {
+    Philox4x32 rng(seed, (uint64_t)i);
+    out[i] = sample_mixture_material_o(T, n_mix, mats, fr.data(), energies[i], rng);
+  }
+  return 0;
+}
+
+int orc_sample_del_event(int M, int K, const double* data, const double* scal, int m, int n,
+                         const double* ki, const double* kf, const double* dX,
+                         const double* Xtot, const double* ratio, double cutoff, int backward,
+                         unsigned long long seed, const double* script, int n_script,
+                         double* out /* 4 per sample */) {
+  Tables T = make_tables(M, K, data, scal, MUON_MASS, MUON_C_TAU);
+  if (n_script > 0) {
+    Philox4x32 rng(0, 0);
+    rng.script = script; rng.script_n = n_script;
+    try {
+      sample_del_event_o(T, m, ki[0], kf[0], dX[0], Xtot[0], ratio[0], cutoff, backward != 0,
+                         rng, out);
+    } catch (const std::exception&) {
+      return -1;   // the script ran out of draws
+    }
+    return rng.script_pos;
+  }
+  for (int i = 0; i < n; ++i) {
+    Philox4x32 rng(seed, (uint64_t)i);
+    sample_del_event_o(T, m, ki[i], kf[i], dX[i], Xtot[i], ratio[i], cutoff, backward != 0, rng,
+                       out + 4 * (size_t)i);
+  }
+  return 0;
+}
+
 // property_range (MIXED or CSDA) and compute_ehs_mean_free_path on a
 // material or runtime mixture.
 double orc_mix_range(int M, int K, const double* data, const double* scal, double mass,

@@ -771,3 +771,40 @@ def cube_fd_ld(ph, geo, rp, cells, lens, cos_t, z_exit, dens, ne, nw, nq, voxel,
                          len(ne), P(f64(ne)), P(f64(nw)), P(f64(nq)), scale, int(voxel), h,
                          threads, P(out))
     return out[0], out[1]
+
+
+def sample_mixture_material(ph, mats, fracs, energies, seed=0, script=None):
+    """sample_mixture_material per energy (seeded), or one scripted call:
+    (material, draws consumed)."""
+    m, f = i32(mats), f64(fracs)
+    e = f64(np.atleast_1d(energies))
+    out = np.zeros(len(e), np.int32)
+    sc = None if script is None else f64(script)
+    used = lib().orc_sample_mixture_material(ph.n_materials, ph.n_knots, P(ph.data),
+                                             P(ph.scalars), len(m), P(m, C.c_int32), P(f),
+                                             len(e), P(e), C.c_ulonglong(seed),
+                                             None if sc is None else P(sc),
+                                             0 if sc is None else len(sc), P(out, C.c_int32))
+    if used < 0:
+        raise RuntimeError("scripted draws exhausted")
+    return (int(out[0]), used) if script is not None else out
+
+
+def sample_del_event(ph, m, ki, kf, dX, Xtot, ratio, cutoff=0.05, backward=False, seed=0,
+                     script=None):
+    """sample_del_event: rows (occurred, X_del, k_del, process) per sample
+    (seeded), or one scripted call: (row, draws consumed)."""
+    cols = [f64(np.atleast_1d(np.asarray(x, dtype=np.float64))) for x in
+            np.broadcast_arrays(ki, kf, dX, Xtot, ratio)]
+    n = len(cols[0])
+    out = np.zeros(4 * n)
+    sc = None if script is None else f64(script)
+    used = lib().orc_sample_del_event(ph.n_materials, ph.n_knots, P(ph.data), P(ph.scalars), m,
+                                      n, *[P(c) for c in cols], C.c_double(cutoff),
+                                      1 if backward else 0,
+                                      C.c_ulonglong(seed), None if sc is None else P(sc),
+                                      0 if sc is None else len(sc), P(out))
+    if used < 0:
+        raise RuntimeError("scripted draws exhausted")
+    rows = out.reshape(n, 4)
+    return (rows[0], used) if script is not None else rows
