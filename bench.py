@@ -956,6 +956,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mc-events", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run every kernel at its default launch shape (profiling runs)")
+    ap.add_argument("--no-per-ray", action="store_true",
+                    help="skip the per-ray launch-loop comparators (counter runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration)")
     a = ap.parse_args()
@@ -973,7 +977,8 @@ def main():
     ctx = dp.Context(0)
     # The first (warm-up) call of each tuned kernel times its candidate launch
     # configurations; the timed steps run the fastest.
-    ctx.set_autotune(True)
+    tune = not a.no_autotune
+    ctx.set_autotune(tune)
     physics = dp.create_physics(("StandardRock", "Water", "Air"))
     mc = run_mc(dp, physics, ctx, torch, a.mc_events, a.steps, a.warmup)
     log(f"mc(straggled): {mc['value']:.4g} events/s, kernel {mc['kernel_ms']:.3f} ms, "
@@ -982,8 +987,8 @@ def main():
     log(f"adjoint: {adj['value']:.4g} rays*voxels/s, {adj['ms_per_step']:.3f} ms/step, "
         f"kernel {adj['kernel_ms']:.3f} ms")
     ctx.set_autotune(False)   # the per-ray comparators run the default launch shape
-    per_ray_adj = run_adjoint_per_ray(dp, L, ctx, torch, adj)
-    ctx.set_autotune(True)
+    per_ray_adj = None if a.no_per_ray else run_adjoint_per_ray(dp, L, ctx, torch, adj)
+    ctx.set_autotune(tune)
     a32 = run_adjoint(dp, L, physics, ctx, torch, a.steps, a.warmup, fp32=True)
     f64d, f32d = adj["_device"], a32.pop("_device")
     a32.pop("_problem")
@@ -994,11 +999,11 @@ def main():
                  speedup_vs_fp64=adj["kernel_ms"] / a32["kernel_ms"])
     log(f"adjoint fp32 stepping: {a32['value']:.4g} rays*voxels/s, kernel {a32['kernel_ms']:.3f} ms, "
         f"flux err vs fp64 {fe:.2g}")
-    ctx.set_autotune(False)
-    log(f"adjoint per-ray loop: {per_ray_adj['device_ms']:.1f} ms device for "
-        f"{per_ray_adj['launches']} launches, batched is {per_ray_adj['speedup_of_batched']:.0f}x "
-        f"faster, bitwise equal {per_ray_adj['bitwise_equal_to_batched']}")
-    ctx.set_autotune(True)
+    if per_ray_adj:
+        log(f"adjoint per-ray loop: {per_ray_adj['device_ms']:.1f} ms device for "
+            f"{per_ray_adj['launches']} launches, batched is "
+            f"{per_ray_adj['speedup_of_batched']:.0f}x faster, bitwise equal "
+            f"{per_ray_adj['bitwise_equal_to_batched']}")
     adj.pop("_device")
     problem = adj.pop("_problem")
     wm = run_wmodel(dp, L, physics, ctx, torch, a.steps, a.warmup)
@@ -1007,10 +1012,11 @@ def main():
     pm = run_pathmc(dp, physics, ctx)
     log(f"pathmc: {pm['value']:.4g} events/s, kernel {pm['kernel_ms']:.3f} ms")
     ctx.set_autotune(False)
-    per_ray_pm = run_pathmc_per_ray(dp, physics, ctx, pm)
-    ctx.set_autotune(True)
-    log(f"pathmc per-ray loop: {per_ray_pm['device_ms_all_rays']:.1f} ms device (scaled), "
-        f"batched is {per_ray_pm['speedup_of_batched']:.1f}x faster")
+    per_ray_pm = None if a.no_per_ray else run_pathmc_per_ray(dp, physics, ctx, pm)
+    ctx.set_autotune(tune)
+    if per_ray_pm:
+        log(f"pathmc per-ray loop: {per_ray_pm['device_ms_all_rays']:.1f} ms device (scaled), "
+            f"batched is {per_ray_pm['speedup_of_batched']:.1f}x faster")
     pm["per_ray_loop"] = per_ray_pm
     pprob = pm.pop("_problem")
     dr = run_direct(dp, L, physics, ctx, torch, a.steps, a.warmup)

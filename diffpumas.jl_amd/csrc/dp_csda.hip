@@ -10,10 +10,13 @@
 // (examples/diff_flux_cube.jl:386-413).  For segment v with E_in -> E_out
 //   c_v = dE_out/dE_in,  r_v = dE_out/drho_v,
 //   b_v = dlog w_v/dE_in, e_v = dlog w_v/dE_out      (w_v = S(E_out)/S(E_in) * decay)
-// are stored in the block's scratch slice.  The tail (remaining rock at
-// rho[0], air, flux_gccly) is differentiated in registers.  Reverse sweep:
-//   a* = T + e_v;  d flux/d rho_v = flux * a* * r_v;  T <- a* c_v + b_v
-// with T = d log(flux)/dE at the segment's exit.  These are the derivatives
+// define the reverse recursion  a* = T + e_v;  d flux/d rho_v = flux a* r_v;
+// T <- a* c_v + b_v  with T = d log(flux)/dE at the segment's exit.  The tail
+// (remaining rock at rho[0], air, flux_gccly) is differentiated in
+// registers.  FP64 runs the recursion in prefix form during the forward
+// sweep (two tape values per segment and node, summed over nodes once the
+// node's flux is known; see the kernel); the FP32 stepping variant stores the
+// four partials and sweeps backward.  These are the derivatives
 // of the executed discrete program (bracket slopes; zero where a lookup is
 // clamped), i.e. what Zygote returns on the reference.  Per-segment sums over
 // nodes use a fixed wave/LDS tree, so results are deterministic.
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
   double* wflux = wrem + nw;                         // [nw]
   __shared__ int s_ray;
 
-  F* sc = (F*)a.scratch + (size_t)blockIdx.x * a.max_seg * 4 * bd;
+  F* sc = (F*)a.scratch + (size_t)blockIdx.x * a.max_seg * (F64 ? 2 : 4) * bd;
   const double rho0 = a.dens[0];
   const F inv_mass = F(1.0 / a.mass);
   const F ctau = F(a.ctau);
@@ -406,6 +409,21 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
         int ie = -1, jr = K / 2;   // interval guesses (rock tables)
         Entry64 en;
         en.valid = false;
+        // FP64: prefix form of the reverse recursion T_s = T_s+1 c_s + beta_s
+        // (beta_s = e_s c_s + b_s).  With P_s = prod_{t<s} c_t and
+        // Q_s = sum_{t<s} beta_t P_t, T_s+1 = (K - Q_s+1) / P_s+1 where
+        // K = T_N P_N + Q_N, so d flux/d rho_s = fl r_s (T_s+1 + e_s)
+        //   = fl alpha_s + (fl K) gamma_s,
+        // alpha_s = r_s (e_s - Q_s+1 / P_s+1), gamma_s = r_s / P_s+1:
+        // two tape values per segment and node, weighted by per-node end
+        // values, and no sequential reverse sweep.  (Q_s+1 telescopes to
+        // T_0 - T_s+1 P_s+1, and P is a ratio of stopping powers, so the two
+        // terms stay of the size of T P.)  A segment with c_s = 0 (a clamped
+        // lookup) decouples what lies before it: T_s = beta_s, so the
+        // earlier segments of the current prefix are closed there with
+        // K = beta_s P_s + Q_s and a new prefix starts after it.
+        double P = 1.0, Q = 0.0;
+        int base = 0;
         // ------------------------------------------------ forward sweep
         for (int s = 0; s < ns; ++s) {
           const int cell = a.cell[b0 + s];
@@ -415,11 +433,38 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
           ok = ok && o.ok;
           W = W * o.ratio;
           X += o.dexp;
-          F* p = sc + (size_t)s * 4 * bd + tid;
-          p[0] = o.c;
-          p[bd] = o.b;
-          p[2 * bd] = o.e;
-          p[3 * bd] = o.k * F(d);
+          if constexpr (F64) {
+            double* p = sc + (size_t)s * 2 * bd + tid;
+            const double rr = o.k * d;
+            const double beta = o.e * o.c + o.b;
+            if (o.c != 0.0) {
+              const double P1 = P * o.c;
+              const double Q1 = Q + beta * P;
+              const double iP1 = 1.0 / P1;
+              p[0] = rr * (o.e - Q1 * iP1);
+              p[bd] = rr * iP1;
+              P = P1;
+              Q = Q1;
+            } else {
+              const double Kc = beta * P + Q;
+              for (int t = base; t < s; ++t) {
+                double* q = sc + (size_t)t * 2 * bd + tid;
+                q[0] += Kc * q[bd];
+                q[bd] = 0.0;
+              }
+              p[0] = rr * o.e;
+              p[bd] = rr;
+              P = 1.0;
+              Q = 0.0;
+              base = s + 1;
+            }
+          } else {
+            F* p = sc + (size_t)s * 4 * bd + tid;
+            p[0] = o.c;
+            p[bd] = o.b;
+            p[2 * bd] = o.e;
+            p[3 * bd] = o.k * F(d);
+          }
           E = o.E;
           lE = o.lE;
         }
@@ -461,15 +506,26 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
           if (lane == 0) wrem[wave] += gs;
         }
         if (a.jac) {
-          // ---------------------------------------------- reverse sweep
-          for (int s = ns - 1; s >= 0; --s) {
-            const F* p = sc + (size_t)s * 4 * bd + tid;
-            F c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
-            F as = T + e;
-            double g = fl != 0.0 ? fl * (double)as * (double)rr : 0.0;
-            T = as * c + bb;
-            double gs = wave_sum(g);
-            if (lane == 0) wsum[wave * a.max_seg + s] += gs;
+          if constexpr (F64) {
+            // ------------------------------- per-segment sums over nodes
+            const double flK = fl != 0.0 ? fl * ((double)T * P + Q) : 0.0;
+            for (int s = 0; s < ns; ++s) {
+              const double* p = sc + (size_t)s * 2 * bd + tid;
+              const double g = fl != 0.0 ? fl * p[0] + flK * p[bd] : 0.0;
+              const double gs = wave_sum(g);
+              if (lane == 0) wsum[wave * a.max_seg + s] += gs;
+            }
+          } else {
+            // -------------------------------------------- reverse sweep
+            for (int s = ns - 1; s >= 0; --s) {
+              const F* p = sc + (size_t)s * 4 * bd + tid;
+              F c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
+              F as = T + e;
+              double g = fl != 0.0 ? fl * (double)as * (double)rr : 0.0;
+              T = as * c + bb;
+              double gs = wave_sum(g);
+              if (lane == 0) wsum[wave * a.max_seg + s] += gs;
+            }
           }
         }
       }
@@ -661,13 +717,14 @@ size_t lds_bytes(bool tlds, int K, int nw, int max_seg, size_t fsize = sizeof(do
 // through L2 (0), plus 2 for the 3-wave occupancy bound (otherwise 4 waves
 // per SIMD); k1 = rays per queue pop; k2 = grid multiplier over one resident
 // wave of blocks.  The first candidate is the default: measured fastest at
-// the differentiation workload (FP64 10.5 ms at 3 waves against 14.9 ms at
-// 4, where its carried entry lookups spill; FP32 stepping 7.0 ms at 4 waves
-// against 7.8 ms at 3).  FP32 stepping always stages its tables in LDS.
+// the differentiation workload (FP64 7.83 ms at 3 waves with twice the
+// resident grid, 8.11 ms with one, 12.4 ms at 4 waves where it spills; FP32
+// stepping 7.0-7.1 ms at 4 waves, 7.8 ms at 3).  FP32 stepping always
+// stages its tables in LDS.
 const std::vector<dph::TuneCfg>& csda_candidates(bool fp32) {
   static const std::vector<dph::TuneCfg> v = {
-      {3, 1, 1}, {1, 1, 1}, {2, 1, 1}, {3, 4, 1}, {3, 1, 2}, {1, 1, 2}, {2, 1, 2}};
-  static const std::vector<dph::TuneCfg> v32 = {{1, 1, 1}, {3, 1, 1}, {1, 4, 1}, {1, 1, 2},
+      {3, 1, 2}, {3, 1, 1}, {1, 1, 1}, {2, 1, 1}, {3, 4, 1}, {1, 1, 2}, {2, 1, 2}};
+  static const std::vector<dph::TuneCfg> v32 = {{1, 1, 2}, {1, 1, 1}, {3, 1, 1}, {1, 4, 1},
                                                 {3, 1, 2}};
   return fp32 ? v32 : v;
 }
@@ -712,7 +769,8 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
     grids.push_back((int)g);
     if (g > grid_max) grid_max = (int)g;
   }
-  size_t scratch = (size_t)grid_max * max_seg * 4 * bd * fsz;
+  // tape: FP64 2 doubles (alpha, gamma), FP32 4 floats per segment and node
+  size_t scratch = (size_t)grid_max * max_seg * (f32 ? 4 : 2) * bd * fsz;
   int rc = dph::ensure(c->scratch, scratch);
   if (rc) return rc;
   CsdaArgs a;

@@ -11,6 +11,7 @@ import pytest
 
 import _oracle as O
 import diffpumas_hip as dp
+from diffpumas_hip import _lib as L
 from workloads import cube_rays, energy_nodes, lognormal_cavity_densities, uniform_densities
 
 pytestmark = pytest.mark.gpu
@@ -109,6 +110,32 @@ def test_csda_cube_flux_and_jacobian(physics, ctx, n_nodes):
         og = ograds[r]
         assert set(k for k, v in g.items() if v != 0.0) <= set(og)
         scale = max(abs(v) for v in og.values())
+        for cell, v in og.items():
+            assert g.get(cell, 0.0) == pytest.approx(v, rel=RTOL, abs=1e-13 * scale), (r, cell)
+
+
+def test_csda_cube_clamped_lookups(physics, ctx):
+    """Node energies at and above the top of the tables (1e9 GeV): lookups
+    clamp, c = dE_out/dE_in is 0 on those segments, and the FP64 kernel's
+    prefix form of the reverse recursion closes and restarts there.  Flux and
+    per-cell gradients against the oracle's forward-mode derivative."""
+    geo, paths, rho, _, _, _ = _problem(physics, ctx, n_nodes=8)
+    ne, nw, nq = energy_nodes(64, 3e8, 5e9)
+    top = physics.data[geo.rock_material, L.T_ENERGY][-1]
+    assert ne.min() < top < ne.max()
+    _say(f"csda_cube clamped: rays={paths.n_rays} nodes 3e8-5e9 GeV (table top {top:g})")
+    flux, jac, jrem = dp.csda_cube_flux(physics, geo, paths, rho, ne, nw, nq, ctx=ctx)
+    oflux, ograds = O.cube_csda(physics, geo, paths.row_ptr, paths.cell, paths.length,
+                                paths.cos_theta, paths.z_exit, rho, ne, nw, nq)
+    np.testing.assert_allclose(flux, oflux, rtol=RTOL, atol=0)
+    for r in range(paths.n_rays):
+        g = {}
+        for s in range(paths.row_ptr[r], paths.row_ptr[r + 1]):
+            g[int(paths.cell[s])] = g.get(int(paths.cell[s]), 0.0) + jac[s]
+        if jrem[r] != 0.0:
+            g[0] = g.get(0, 0.0) + jrem[r]
+        og = ograds[r]
+        scale = max([abs(v) for v in og.values()] + [1e-300])
         for cell, v in og.items():
             assert g.get(cell, 0.0) == pytest.approx(v, rel=RTOL, abs=1e-13 * scale), (r, cell)
 
