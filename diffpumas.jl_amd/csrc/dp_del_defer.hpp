@@ -179,6 +179,21 @@ __device__ __forceinline__ DelOut del_finish(const DevKernel& c, double mass, do
 // those lanes).  Lanes with del get reverse_del_event(details = true) at
 // energy k in medium mi: the same draws, ancestor energy, weight and target as
 // the inline evaluation.
+//
+// Convergence: the routine does not assume that the whole wave, or any
+// particular set of lanes, is present.  It is reached from per-lane step
+// loops, so the lanes executing one call are whatever subset the EXEC mask
+// holds there; the routine takes that subset from its own __ballot(1) and
+// exchanges data only among those lanes: every loop that contains a
+// wave_sync() is bounded by ballot results (uniform over the subset), and the
+// per-lane work loop between two wave_sync() calls reconverges before the
+// next one.  A lane's result depends only on its own event and draws; the
+// subset only decides which lanes evaluate which DCS components, so splitting
+// the lanes over several calls (by the hardware or by the compiler) gives the
+// same results.  The LDS buffer belongs to the block's single wave, and
+// successive calls are ordered by the final wave_sync().  A host emulation
+// that requires all 64 lanes at every ballot therefore cannot run this
+// routine; the per-event GPU tests against the oracle cover it.
 __device__ __noinline__ static DelOut coop_del_event(const DelLog& lg, int mi, double mass,
                                                      double k, Philox& rng, bool del, int& err) {
   CoopBuf* cb = lg.coop;

@@ -160,7 +160,7 @@ struct McArgs {
 };
 
 __global__ __launch_bounds__(256) void mc_flat_csda_kernel(McArgs a) {
-  extern __shared__ double lds[];
+  HIP_DYNAMIC_SHARED(double, lds)
   const int K = a.K;
   MatView mv[3];
   const int mats[3] = {a.g.rock, a.g.air, a.g.porous >= 0 ? a.g.porous : a.g.rock};
