@@ -13,10 +13,10 @@
 // define the reverse recursion  a* = T + e_v;  d flux/d rho_v = flux a* r_v;
 // T <- a* c_v + b_v  with T = d log(flux)/dE at the segment's exit.  The tail
 // (remaining rock at rho[0], air, flux_gccly) is differentiated in
-// registers.  FP64 runs the recursion in prefix form during the forward
-// sweep (two tape values per segment and node, summed over nodes once the
-// node's flux is known; see the kernel); the FP32 stepping variant stores the
-// four partials and sweeps backward.  These are the derivatives
+// registers.  The recursion runs in prefix form during the forward sweep:
+// two tape values per segment and node, summed over nodes once the node's
+// flux is known (see the kernel), so there is no backward sweep.  These are
+// the derivatives
 // of the executed discrete program (bracket slopes; zero where a lookup is
 // clamped), i.e. what Zygote returns on the reference.  Per-segment sums over
 // nodes use a fixed wave/LDS tree, so results are deterministic.
@@ -292,9 +292,9 @@ __device__ __forceinline__ StepOutT<float> csda_segment_g(const Tab32& T, int K,
 // through L1/L2 from the table block.  Both give the same values; the
 // autotuner picks one per workload.  F: the type of the segment arithmetic.
 // With F = float (the FP32 stepping variant) the segment maps, their
-// partials, the tape and the reverse sweep are float around the FP64
-// log-energy spine; the node weights, the flux model and every sum over
-// nodes stay FP64.
+// partials and the tape values are float around the FP64 log-energy spine;
+// the prefix products and sums, the node weights, the flux model and every
+// sum over nodes stay FP64.
 template <bool TLDS, typename F, int WAVES>
 __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
   constexpr bool F64 = sizeof(F) == sizeof(double);
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
   double* wflux = wrem + nw;                         // [nw]
   __shared__ int s_ray;
 
-  F* sc = (F*)a.scratch + (size_t)blockIdx.x * a.max_seg * (F64 ? 2 : 4) * bd;
+  F* sc = (F*)a.scratch + (size_t)blockIdx.x * a.max_seg * 2 * bd;
   const double rho0 = a.dens[0];
   const F inv_mass = F(1.0 / a.mass);
   const F ctau = F(a.ctau);
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
         int ie = -1, jr = K / 2;   // interval guesses (rock tables)
         Entry64 en;
         en.valid = false;
-        // FP64: prefix form of the reverse recursion T_s = T_s+1 c_s + beta_s
+        // Prefix form of the reverse recursion T_s = T_s+1 c_s + beta_s
         // (beta_s = e_s c_s + b_s).  With P_s = prod_{t<s} c_t and
         // Q_s = sum_{t<s} beta_t P_t, T_s+1 = (K - Q_s+1) / P_s+1 where
         // K = T_N P_N + Q_N, so d flux/d rho_s = fl r_s (T_s+1 + e_s)
@@ -433,37 +433,34 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
           ok = ok && o.ok;
           W = W * o.ratio;
           X += o.dexp;
-          if constexpr (F64) {
-            double* p = sc + (size_t)s * 2 * bd + tid;
-            const double rr = o.k * d;
-            const double beta = o.e * o.c + o.b;
-            if (o.c != 0.0) {
-              const double P1 = P * o.c;
+          {
+            // the prefix recursion in FP64 for both stepping types; the tape
+            // holds alpha, gamma as F
+            F* p = sc + (size_t)s * 2 * bd + tid;
+            const double c = o.c, e = o.e;
+            const double rr = (double)o.k * d;
+            const double beta = e * c + (double)o.b;
+            if (c != 0.0) {
+              const double P1 = P * c;
               const double Q1 = Q + beta * P;
               const double iP1 = 1.0 / P1;
-              p[0] = rr * (o.e - Q1 * iP1);
-              p[bd] = rr * iP1;
+              p[0] = F(rr * (e - Q1 * iP1));
+              p[bd] = F(rr * iP1);
               P = P1;
               Q = Q1;
             } else {
               const double Kc = beta * P + Q;
               for (int t = base; t < s; ++t) {
-                double* q = sc + (size_t)t * 2 * bd + tid;
-                q[0] += Kc * q[bd];
-                q[bd] = 0.0;
+                F* q = sc + (size_t)t * 2 * bd + tid;
+                q[0] = F((double)q[0] + Kc * (double)q[bd]);
+                q[bd] = F(0);
               }
-              p[0] = rr * o.e;
-              p[bd] = rr;
+              p[0] = F(rr * e);
+              p[bd] = F(rr);
               P = 1.0;
               Q = 0.0;
               base = s + 1;
             }
-          } else {
-            F* p = sc + (size_t)s * 4 * bd + tid;
-            p[0] = o.c;
-            p[bd] = o.b;
-            p[2 * bd] = o.e;
-            p[3 * bd] = o.k * F(d);
           }
           E = o.E;
           lE = o.lE;
@@ -506,26 +503,13 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
           if (lane == 0) wrem[wave] += gs;
         }
         if (a.jac) {
-          if constexpr (F64) {
-            // ------------------------------- per-segment sums over nodes
-            const double flK = fl != 0.0 ? fl * ((double)T * P + Q) : 0.0;
-            for (int s = 0; s < ns; ++s) {
-              const double* p = sc + (size_t)s * 2 * bd + tid;
-              const double g = fl != 0.0 ? fl * p[0] + flK * p[bd] : 0.0;
-              const double gs = wave_sum(g);
-              if (lane == 0) wsum[wave * a.max_seg + s] += gs;
-            }
-          } else {
-            // -------------------------------------------- reverse sweep
-            for (int s = ns - 1; s >= 0; --s) {
-              const F* p = sc + (size_t)s * 4 * bd + tid;
-              F c = p[0], bb = p[bd], e = p[2 * bd], rr = p[3 * bd];
-              F as = T + e;
-              double g = fl != 0.0 ? fl * (double)as * (double)rr : 0.0;
-              T = as * c + bb;
-              double gs = wave_sum(g);
-              if (lane == 0) wsum[wave * a.max_seg + s] += gs;
-            }
+          // --------------------------------- per-segment sums over nodes
+          const double flK = fl != 0.0 ? fl * ((double)T * P + Q) : 0.0;
+          for (int s = 0; s < ns; ++s) {
+            const F* p = sc + (size_t)s * 2 * bd + tid;
+            const double g = fl != 0.0 ? fl * (double)p[0] + flK * (double)p[bd] : 0.0;
+            const double gs = wave_sum(g);
+            if (lane == 0) wsum[wave * a.max_seg + s] += gs;
           }
         }
       }
@@ -769,8 +753,8 @@ int launch_csda(dp_ctx c, dp_tables t, const dp_paths* p, const dp_csda_nodes* n
     grids.push_back((int)g);
     if (g > grid_max) grid_max = (int)g;
   }
-  // tape: FP64 2 doubles (alpha, gamma), FP32 4 floats per segment and node
-  size_t scratch = (size_t)grid_max * max_seg * (f32 ? 4 : 2) * bd * fsz;
+  // tape: alpha, gamma per segment and node (double, or float for FP32 stepping)
+  size_t scratch = (size_t)grid_max * max_seg * 2 * bd * fsz;
   int rc = dph::ensure(c->scratch, scratch);
   if (rc) return rc;
   CsdaArgs a;

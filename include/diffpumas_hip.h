@@ -104,8 +104,8 @@ int dp_check_errors(dp_ctx ctx);
 int dp_set_autotune(dp_ctx ctx, int32_t on);
 /* FP32 stepping for the per-voxel-density CSDA flux and adjoint (dp_csda_cube
  * and every entry point built on it) of this context: the segment maps'
- * exponentials, stopping powers, decay factors, their partials, the adjoint
- * tape and the reverse sweep in float, around an FP64 spine (the walk's log
+ * exponentials, stopping powers, decay factors, their partials and the
+ * adjoint tape in float, around an FP64 spine (the walk's log
  * energy, the range lookup and the exit-grammage inversion, which take
  * subtractions and products only); node weights, the flux model, every sum
  * over nodes and the outputs stay FP64.  The reference computes in Float64
