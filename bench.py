@@ -637,6 +637,110 @@ def run_wmodel(dp, L, physics, ctx, torch, steps, warmup, n_nodes=32, correction
             "_problem": (paths, w, s, m)}
 
 
+def run_inverse(dp, L, ctx, torch, steps, warmup, n_nodes=32, max_free=3000):
+    """One model evaluation of the Gauss-Newton inverse loop at its full size
+    (gauss_newton_reconstruct, src/Tomography_inverse.jl:448-603): 257x257
+    diamond-square DEM, 96x96x48 voxels of 10 m, 4 detectors x 4096
+    directions, water fractions w (0.03 with two buried anomalies, w0 = 0.1 on
+    the free cells), Poisson-weighted residual.  The device work of one
+    evaluation -- w-model flux and Jacobian, residual and data cost, normal
+    matrix and gradient over the best-covered free cells -- is captured once
+    as a HIP graph; a step is one replay.  Units: ray*voxels per evaluation."""
+    import ctypes as C
+    import numpy as np
+    from workloads import inverse_workload
+    ph = dp.create_physics_csda(("StandardRock", "Water", "Air"))
+    paths, (xe, ye, ze), n_cells = inverse_workload(dp, ctx)
+    m = dp.MaterialConfig(0, 1, 2, -1, ph.scalars[0, 0], ph.scalars[1, 0])
+    nodes = dp.csda_energy_quadrature(n_nodes, 1.0, 1e4)
+    site = dp.SiteConfig(963.0, 3e4)
+    xc, yc, zc = (0.5 * (e[1:] + e[:-1]) for e in (xe, ye, ze))
+    Zc, Yc, Xc = np.meshgrid(zc, yc, xc, indexing="ij")
+    w_true = np.full(n_cells, 0.03)
+    for ax, ay, az_, rad, wa in ((-150.0, -150.0, 1008.0, 30.0, 0.4),
+                                 (150.0, 150.0, 1023.0, 35.0, 0.35)):
+        w_true[((Xc - ax) ** 2 + (Yc - ay) ** 2 + (Zc - az_) ** 2).ravel() <= rad ** 2] = wa
+    f_true, _ = dp.flux_and_segment_jacobian(ph, np.zeros(n_cells, np.uint8), m, site, paths,
+                                             w_true, nodes, ctx=ctx)
+    expo = 5e7
+    obs = np.random.default_rng(13).poisson(f_true * expo) / expo
+    W = expo / np.maximum(obs, 1.0 / expo)
+    rows = np.repeat(np.arange(paths.n_rays), np.diff(paths.row_ptr))
+    pairs = np.unique(np.stack([paths.cells.astype(np.int64), rows]), axis=1)
+    cross = np.bincount(pairs[0], minlength=n_cells)
+    mc = 16
+    while (cross >= mc).sum() > max_free:
+        mc += 1
+    active = np.flatnonzero(cross >= mc).astype(np.int32)
+    na = len(active)
+    w0 = w_true.copy()
+    w0[active] = 0.1
+    dev = torch.device("cuda:0")
+    T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+    keep = dict(rp=T(paths.row_ptr, torch.int32), cell=T(paths.cells, torch.int32),
+                len=T(paths.lengths, torch.float64), cos=T(paths.cos_theta, torch.float64),
+                rem=T(paths.remaining_rock, torch.float64), ez=T(paths.exit_z, torch.float64),
+                val=T(paths.valid, torch.uint8), w=T(w0, torch.float64),
+                sh=torch.zeros(n_cells, dtype=torch.uint8, device=dev),
+                e=T(nodes.energy, torch.float64), wt=T(nodes.weight, torch.float64),
+                q=T(nodes.charge, torch.float64), obs=T(obs, torch.float64), W=T(W, torch.float64),
+                act=T(active, torch.int32))
+    nr, ns = paths.n_rays, paths.n_segments
+    z = lambda n: torch.zeros(max(n, 1), dtype=torch.float64, device=dev)
+    flux, jac, res, cost, N, g = z(nr), z(ns), z(nr), z(1), z(na * na), z(na)
+    dptr = lambda t, ty: C.cast(t.data_ptr(), ty)
+    k = keep
+    cp = L.DirectionalPaths(nr, ns, paths.max_segments(), dptr(k["rp"], L.i32_p),
+                            dptr(k["cell"], L.i32_p), dptr(k["len"], L.dbl_p),
+                            dptr(k["cos"], L.dbl_p), dptr(k["rem"], L.dbl_p),
+                            dptr(k["ez"], L.dbl_p), dptr(k["val"], C.POINTER(C.c_uint8)))
+    es = L.EnergySamples(len(nodes.energy), dptr(k["e"], L.dbl_p), dptr(k["wt"], L.dbl_p),
+                         dptr(k["q"], L.dbl_p))
+    mcfg, sc = m.c(), L.SiteConfig(963.0, 3e4)
+    csr = L.Csr(nr, n_cells, ns, k["rp"].data_ptr(), k["cell"].data_ptr(), jac.data_ptr())
+    h = ph.device_handle(ctx)
+    lib = ctx.lib
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream.cuda_stream)
+
+    def work():
+        L.check(lib.dp_csda_w_flux_jacobian_dev(ctx.h, h, C.byref(mcfg), C.byref(sc), C.byref(cp),
+                                                n_cells, k["w"].data_ptr(), k["sh"].data_ptr(),
+                                                C.byref(es), flux.data_ptr(), jac.data_ptr()))
+        L.check(lib.dp_residual_dev(ctx.h, nr, flux.data_ptr(), k["obs"].data_ptr(),
+                                    k["W"].data_ptr(), res.data_ptr(), cost.data_ptr()))
+        L.check(lib.dp_normal_equations_dev(ctx.h, C.byref(csr), k["W"].data_ptr(),
+                                            res.data_ptr(), na, k["act"].data_ptr(),
+                                            N.data_ptr(), g.data_ptr()))
+
+    work()
+    ctx.synchronize()
+    ctx.check_errors()
+    graph = ctx.capture(work)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        graph.launch()
+        if ev is not None:
+            ev[1].record(stream)
+
+    log(f"inverse: {nr} bins, {ns} ray*voxels, {n_cells} cells, {na} free cells")
+    ms_step, graph_ms = _timed(torch, stream, step, steps, warmup)
+    c0 = float(cost.cpu()[0])
+    graph.close()
+    ctx.set_stream(None)
+    ctx.check_errors()
+    return {"value": ns / (graph_ms * 1e-3), "unit": "rays*voxels/s (per model evaluation)",
+            "ms_per_step": ms_step, "graph_ms": graph_ms, "bins": nr, "segments": ns,
+            "cells": n_cells, "free_cells": na, "nodes": n_nodes, "cost_at_w0": c0,
+            "workload": "Gauss-Newton model evaluation (w-model flux + Jacobian, residual + cost, "
+                        "normal equations) replayed from one HIP graph: 257x257 diamond-square "
+                        "DEM, 96x96x48 voxels of 10 m, 4 detectors x 4096 directions, "
+                        f"{na} free cells"}
+
+
 def run_normal(dp, L, physics, ctx, torch, steps, warmup, min_crossings=4):
     """Gauss-Newton normal equations Ja' (W .* Ja) and Ja' (W .* r) of the
     w-model Jacobian of the tomography workload (4096 bins, 32 nodes), free
@@ -1028,6 +1132,8 @@ def main():
     wc = run_wmodel(dp, L, physics, ctx, torch, a.steps, a.warmup, corrections=True)
     wcprob = wc.pop("_problem")
     log(f"wmodel+corrections: {wc['value']:.4g} rays*voxels/s, {wc['ms_per_step']:.3f} ms/step")
+    inv = run_inverse(dp, L, ctx, torch, a.steps, a.warmup)
+    log(f"inverse: {inv['value']:.4g} rays*voxels/s per evaluation, graph {inv['graph_ms']:.3f} ms")
     ne = run_normal(dp, L, physics, ctx, torch, a.steps, a.warmup)
     log(f"normal: {ne['n_free']} free cells, {ne['kernel_ms']:.3f} ms, "
         f"{ne['roofline']['achieved']:.1f} GB/s")
@@ -1091,6 +1197,7 @@ def main():
         "dem": dm,
         "normal": ne,
         "tetra": tt,
+        "inverse": inv,
         "autotune": {k: {"best": list(v["best"]), "ms": v["ms"],
                          "tried": [[list(c), m] for c, m in v["tried"]]}
                      for k, v in ctx.tuning().items()},

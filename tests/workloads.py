@@ -115,3 +115,41 @@ def kuhn_mesh(nx, ny, nz, h, origin=(0.0, 0.0, 0.0), jitter=0.0, seed=0):
                         ids.append(vid(*v))
                     tets.append(ids)
     return pts, np.array(tets, np.int32)
+
+
+def inverse_workload(dp, ctx, n_pow=8, spacing=20.0, nv=(96, 96, 48), vsize=10.0, n_dirs=4096,
+                     dets=((-150.0, -150.0), (150.0, -150.0), (-150.0, 150.0), (150.0, 150.0)),
+                     z_det=963.0, seed=11, zen_max=60.0):
+    """The Gauss-Newton inverse workload: a (2^n_pow + 1)^2-node diamond-square
+    DEM (400 m relief on a dome above 1200 m), detectors at z_det under a
+    block of nv voxels of vsize m, n_dirs directions per detector (zenith
+    U[0, zen_max] deg); DEM exits and voxel traversal on the GPU.  Returns the
+    traced paths (bins whose ray leaves through the terrain), the edges and
+    the cell count."""
+    from diffpumas_hip.tomography import direction_vector
+    n = 2 ** n_pow + 1
+    xs = (np.arange(n) - (n - 1) / 2) * spacing
+    X, Y = np.meshgrid(xs, xs)
+    z = 1200.0 + diamond_square(n_pow, 400.0, seed) + 300.0 * np.exp(
+        -(X ** 2 + Y ** 2) / (0.35 * xs[-1]) ** 2)
+    rng = np.random.default_rng(seed + 1)
+    zen_all, az_all, surf_all, ez_all, org_all = [], [], [], [], []
+    for xd, yd in dets:
+        emap = dp.ElevationMap(n, n, (xs[0] - xd) * 1e-3, (xs[0] - yd) * 1e-3, spacing * 1e-3,
+                               spacing * 1e-3, z)
+        zen = rng.uniform(0.0, zen_max, n_dirs)
+        az = rng.uniform(0.0, 360.0, n_dirs)
+        dirs = np.array([direction_vector(a, b) for a, b in zip(zen, az)])
+        dist, found, ez = dp.ray_surface_exit_distance(emap, dirs, z_det, ctx=ctx)
+        zen_all.append(zen[found]); az_all.append(az[found])
+        surf_all.append(dist[found]); ez_all.append(ez[found])
+        org_all.append(np.tile([xd, yd, z_det], (int(found.sum()), 1)))
+    zen, az = np.concatenate(zen_all), np.concatenate(az_all)
+    nx, ny, nz = nv
+    xe = (np.arange(nx + 1) - nx / 2) * vsize
+    ye = (np.arange(ny + 1) - ny / 2) * vsize
+    ze = z_det - vsize + np.arange(nz + 1) * vsize
+    paths = dp.trace_voxel_paths(xe, ye, ze, zen, az, np.concatenate(surf_all),
+                                 np.concatenate(ez_all), detectors=np.concatenate(org_all),
+                                 ctx=ctx)
+    return paths, (xe, ye, ze), nx * ny * nz
