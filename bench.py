@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_BYTES_PER_UNIT = {
     "mc_flat_stochastic_kernel": 387521.0,    # per event
     "csda_cube_kernel": 8909.2,               # per ray-voxel (all 256 nodes)
-    "csda_cube_kernel_fp32": 8826.9,          # per ray-voxel (FP32 stepping)
+    "csda_cube_kernel_fp32": 4691.0,          # per ray-voxel (FP32 stepping)
     "wcsda_kernel": 2127.0,                   # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 4314.0,         # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 271403.0,               # per event
@@ -78,7 +78,7 @@ VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 MEASURED_VALU_PER_UNIT = {
     "mc_flat_stochastic_kernel": 36622.0,     # per event
     "csda_cube_kernel": 1166.0,               # per ray-voxel (all 256 nodes)
-    "csda_cube_kernel_fp32": 1174.4,          # per ray-voxel (FP32 stepping)
+    "csda_cube_kernel_fp32": 1309.8,          # per ray-voxel (FP32 stepping)
     "wcsda_kernel": 10120.0,                  # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 10495.0,        # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 13768.9,                # per event
