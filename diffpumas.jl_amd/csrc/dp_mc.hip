@@ -206,9 +206,14 @@ __global__ __launch_bounds__(256) void mc_flat_csda_kernel(McArgs a) {
 }
 
 // ------------------------------------------------- stochastic transport
+// The table and medium descriptors that the out-of-line step functions take
+// by reference live in device memory (Tg, media), not in the kernel
+// arguments: a reference into the arguments makes the compiler copy them to
+// every lane's stack, and each descriptor read in a step becomes a per-lane
+// scratch load instead of a load that all lanes of a wave share.
 struct StochArgs {
   DevTables T;
-  Medium rock, air, porous;
+  const DevTables* Tg;  // T in device memory
   bool has_porous;
   double thick, rock_rho, porous_rho, porous_thick, palt;
   double cos_t, sin_t, emin, emax, thr, sw;
@@ -218,7 +223,7 @@ struct StochArgs {
   double* out_w;
   int* err;            // [0] error bits, [1] spare
   unsigned long long* bad_event;
-  const Medium* media;  // rock, air, porous in device memory (indexed by DelLog::medium)
+  const Medium* media;  // rock, air, porous in device memory (DelLog::medium indexes it)
   const uint32_t* order;  // events in order of source-energy bucket (null: identity)
 };
 
@@ -227,7 +232,7 @@ struct StochArgs {
 // and the acceptance of compute_flux_single (:739-757).
 __device__ double flat_event(const StochArgs& a, double energy, double charge, Philox& rng,
                              int& err) {
-  const DevTables& T = a.T;
+  const DevTables& T = *a.Tg;
   McState st;
   st.E = energy; st.w = 1.0; st.t = 0.0; st.L = 0.0;
   st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
@@ -255,13 +260,13 @@ __device__ double flat_event(const StochArgs& a, double energy, double charge, P
       double density, stb;
       const bool in_air = !(z < a.thick);
       if (!in_air) {
-        if (a.has_porous && z >= porous_z) { md = &a.porous; density = a.porous_rho; }
-        else { md = &a.rock; density = a.rock_rho; }
+        if (a.has_porous && z >= porous_z) { md = &a.media[2]; density = a.porous_rho; }
+        else { md = &a.media[0]; density = a.rock_rho; }
         if (uz > EPS64) stb = (((a.has_porous && z < porous_z) ? porous_z : a.thick) - z) / uz;
         else if (uz < -EPS64) stb = (((a.has_porous && z >= porous_z) ? porous_z : 0.0) - z) / uz;
         else stb = 1000.0;
       } else {
-        md = &a.air;
+        md = &a.media[1];
         density = 1.205 * exp(-z / 12e3);
         if (uz > EPS64) stb = (palt - z) / uz;
         else if (uz < -EPS64) stb = (a.thick - z) / uz;
@@ -297,7 +302,7 @@ __device__ double flat_event(const StochArgs& a, double energy, double charge, P
 // event's energy once and hands it to the step functions.
 __device__ double flat_event_f(const StochArgs& a, double energy, double charge, Philox& rng,
                                int& err, DelLog* dlog) {
-  const DevTables& T = a.T;
+  const DevTables& T = *a.Tg;
   McState st;
   st.E = energy; st.w = 1.0; st.t = 0.0; st.L = 0.0;
   st.pos[0] = st.pos[1] = st.pos[2] = 0.0;
@@ -326,13 +331,13 @@ __device__ double flat_event_f(const StochArgs& a, double energy, double charge,
       double density, stb;
       const bool in_air = !(z < a.thick);
       if (!in_air) {
-        if (a.has_porous && z >= porous_z) { md = &a.porous; density = a.porous_rho; }
-        else { md = &a.rock; density = a.rock_rho; }
+        if (a.has_porous && z >= porous_z) { md = &a.media[2]; density = a.porous_rho; }
+        else { md = &a.media[0]; density = a.rock_rho; }
         if (uz > EPS64) stb = (((a.has_porous && z < porous_z) ? porous_z : a.thick) - z) / uz;
         else if (uz < -EPS64) stb = (((a.has_porous && z >= porous_z) ? porous_z : 0.0) - z) / uz;
         else stb = 1000.0;
       } else {
-        md = &a.air;
+        md = &a.media[1];
         density = 1.205 * exp(-z / 12e3);
         if (uz > EPS64) stb = (palt - z) / uz;
         else if (uz < -EPS64) stb = (a.thick - z) / uz;
@@ -351,7 +356,7 @@ __device__ double flat_event_f(const StochArgs& a, double energy, double charge,
       double step = fmin(stb + 1e-7, lim);
       step = fmin(fmax(step, 1e-7), 1000.0);
       g = L.i;
-      if (dlog) dlog->medium = md == &a.rock ? 0 : (md == &a.air ? 1 : 2);
+      if (dlog) dlog->medium = (int)(md - a.media);
       if (use_str) straggled_step_f(T, *md, st, L, lb0, density, step, use_scat, limit, rng, err, &g, dlog);
       else if (use_mixed) mixed_step_f(T, *md, st, L, density, step, limit, rng, err, dlog);
       else csda_mix_step_f(T, *md, st, L, density, step, &g);
@@ -482,7 +487,7 @@ __global__ __launch_bounds__(MC_THREADS, 4) void mc_flat_stochastic_kernel(Stoch
       __syncthreads();  // every record of the round is in the pool
       m = min(pool_n, MC_POOL);
       pool_eval(pool, m, a.T.mass, [&](int md) -> const DevKernel& {
-        return md == 0 ? a.rock.kern : (md == 1 ? a.air.kern : a.porous.kern);
+        return a.media[md < 0 ? 0 : (md > 2 ? 2 : md)].kern;
       });
       __syncthreads();  // every factor is in the pool
     }
@@ -814,7 +819,8 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
   size_t o_flag = o_res + 256;
   size_t o_bad = o_flag + 256;
   size_t o_med = o_bad + 256;
-  size_t o_cnt = dph::align_up(o_med + 3 * sizeof(Medium), 256);
+  size_t o_tab = dph::align_up(o_med + 3 * sizeof(Medium), 256);
+  size_t o_cnt = dph::align_up(o_tab + sizeof(DevTables), 256);
   size_t o_ord = dph::align_up(o_cnt + 4 * MC_BUCKETS, 256);
   int rc = dph::ensure(c->scratch, dph::align_up(o_ord + 4 * (size_t)n, 256));
   if (rc) return rc;
@@ -841,10 +847,12 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
   a.out_w = stochastic ? nullptr : out_w;
   a.out_gf = (double*)(b + o_gf); a.out_gd = (double*)(b + o_gd);
   a.err = flags;
-  Medium hm[3];  // host copy of the media, read by an async copy before the final sync
+  Medium hm[3];  // host copies of the media and tables, read by async copies before the
+  DevTables ht;   // final sync
   if (stochastic) {
     StochArgs s;
     s.T = {t->data, t->scalars, t->n_knots, t->pow2, t->mass, t->ctau, t->elastic_ratio};
+    ht = s.T;
     std::vector<int> rm;
     std::vector<double> rf;
     if (mixture) {
@@ -852,10 +860,10 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     } else {
       rm = {geo->rock_material}; rf = {1.0};
     }
-    if ((rc = make_medium(t, rm, rf, s.rock))) return rc;
-    if ((rc = make_medium(t, {geo->air_material}, {1.0}, s.air))) return rc;
+    if ((rc = make_medium(t, rm, rf, hm[0]))) return rc;
+    if ((rc = make_medium(t, {geo->air_material}, {1.0}, hm[1]))) return rc;
     s.has_porous = geo->porous_material >= 0 && geo->porous_thickness > 0;
-    if ((rc = make_medium(t, {s.has_porous ? geo->porous_material : rm[0]}, {1.0}, s.porous))) return rc;
+    if ((rc = make_medium(t, {s.has_porous ? geo->porous_material : rm[0]}, {1.0}, hm[2]))) return rc;
     s.thick = geo->rock_thickness; s.rock_rho = geo->rock_density;
     s.porous_rho = geo->porous_density; s.porous_thick = geo->porous_thickness;
     s.palt = geo->primary_altitude;
@@ -863,9 +871,10 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     s.thr = thr; s.sw = cfg->mode_switch;
     s.straggling = cfg->straggling != 0; s.scattering = cfg->scattering != 0;
     s.n = n; s.seed = cfg->seed; s.out_w = out_w; s.err = flags; s.bad_event = bad;
-    hm[0] = s.rock; hm[1] = s.air; hm[2] = s.porous;
     DP_HIP(hipMemcpyAsync(b + o_med, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream));
+    DP_HIP(hipMemcpyAsync(b + o_tab, &ht, sizeof(ht), hipMemcpyHostToDevice, c->stream));
     s.media = (const Medium*)(b + o_med);
+    s.Tg = (const DevTables*)(b + o_tab);
     dph::record(c, c->ev0);
     s.order = nullptr;
     if (n < 0x7fffffff) {

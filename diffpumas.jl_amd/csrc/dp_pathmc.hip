@@ -25,8 +25,12 @@ namespace {
 constexpr int PM_THREADS = 128;
 constexpr int PM_POOL = 384;  // pooled hard-event records per block
 
+// The out-of-line step functions take the tables by reference from Tg, a copy
+// in device memory: a reference into the kernel arguments would make the
+// compiler copy them to every lane's stack.
 struct PathArgs {
   DevTables T;
+  const DevTables* Tg;  // T in device memory
   const Medium* media;  // n_media cell media, then rock, then air
   const int32_t* cell_medium;
   const double* cell_density;
@@ -101,6 +105,7 @@ __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, d
                              int& g, DelLog* dlog, bool straggling) {
   double remaining = distance;
   const double elimit = a.elimit;
+  const DevTables& Tc = *a.Tg;  // for the out-of-line steps
   int it = 0;
   while (remaining > 1e-8 && st.E < elimit - EPS64) {
     if (++it > 1000000) { err |= ERR_CAP; return; }
@@ -127,16 +132,16 @@ __device__ void path_segment(const PathArgs& a, const Medium& md, McState& st, d
     const double oldL = st.L, oldE = st.E;
     if (FAST) {
       if (st.E < a.thr_low - EPS64 && straggling)
-        straggled_step_f(a.T, md, st, L, 0.0, dens, step, false, cur, rng, err, &g, dlog);
+        straggled_step_f(Tc, md, st, L, 0.0, dens, step, false, cur, rng, err, &g, dlog);
       else if (straggling)
-        mixed_step_f(a.T, md, st, L, dens, step, cur, rng, err, dlog);
+        mixed_step_f(Tc, md, st, L, dens, step, cur, rng, err, dlog);
       else
         csda_mix_step_f(a.T, md, st, L, dens, step, &g);
     } else if (st.E < a.thr_low - EPS64) {
-      if (straggling) straggled_step(a.T, md, st, dens, step, false, cur, rng, err);
+      if (straggling) straggled_step(Tc, md, st, dens, step, false, cur, rng, err);
       else csda_mix_step(a.T, md, st, dens, step);
     } else {
-      if (straggling) mixed_step(a.T, md, st, dens, step, cur, rng, err);
+      if (straggling) mixed_step(Tc, md, st, dens, step, cur, rng, err);
       else csda_mix_step(a.T, md, st, dens, step);
     }
     if (err) return;
@@ -366,7 +371,8 @@ int run_paths(const std::string& fn, dp_ctx c, dp_tables t, const dp_material_co
   const int64_t n_ev = (int64_t)nr * nn;
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
-  const size_t o_med = take(sizeof(Medium) * hm.size()), o_cm = take(4 * (size_t)n_cells),
+  const size_t o_med = take(sizeof(Medium) * hm.size()), o_tab = take(sizeof(DevTables)),
+               o_cm = take(4 * (size_t)n_cells),
                o_cd = take(8 * (size_t)n_cells), o_rp = take(4 * (size_t)(nr + 1)),
                o_cell = take(4 * (size_t)ns), o_len = take(8 * (size_t)ns),
                o_cos = take(8 * (size_t)nr), o_rem = take(8 * (size_t)nr),
@@ -418,6 +424,8 @@ int run_paths(const std::string& fn, dp_ctx c, dp_tables t, const dp_material_co
   for (double e : t->last_energy) supported = e < supported ? e : supported;
   PathArgs a;
   a.T = {t->data, t->scalars, t->n_knots, t->pow2, t->mass, t->ctau, t->elastic_ratio};
+  DP_HIP(hipMemcpyAsync(b + o_tab, &a.T, sizeof(DevTables), hipMemcpyHostToDevice, st));
+  a.Tg = (const DevTables*)(b + o_tab);
   a.media = (const Medium*)(b + o_med);
   a.cell_medium = (const int32_t*)(b + o_cm);
   a.cell_density = (const double*)(b + o_cd);
