@@ -142,14 +142,32 @@ __device__ __forceinline__ double pool_apply(const PoolRec* pool, int m, int own
 // for the pool: the chosen process deflects the muon in the same step.  Its
 // components are evaluated at once by every lane of the wave that is in the
 // same straggled step; the owner sums them in order and picks the target.
-constexpr int COOP_SLOTS = 8;  // events per round
+//
+// The lanes split the work below the component level: one item per
+// quadrature node of the pair-production (12) and photonuclear (9) cross
+// sections of every element, then one per bremsstrahlung and ionisation
+// value.  Items of one kind run together, so the wave evaluates one DDCS
+// node per pass instead of a whole quadrature per component.  Each component
+// is then assembled by one lane from its node terms in node order (the
+// arithmetic of dcs_pair / dcs_photo), and the owner sums the components in
+// order, so every value equals the inline evaluation.
+constexpr int COOP_SLOTS = 2;  // events per round (rarely more than two hard events per step)
 constexpr int COOP_EL = 8;     // elements per medium (more: the owner evaluates alone)
+constexpr int COOP_LEAF = 23;  // per element: bremsstrahlung, 12 pair nodes, 9 photonuclear nodes, ionisation
 
 struct CoopBuf {
+  double leaf[COOP_SLOTS][COOP_EL][COOP_LEAF];
   double comp[COOP_SLOTS][4 * COOP_EL];
   double anc[COOP_SLOTS], q[COOP_SLOTS];
   int mi[COOP_SLOTS];
 };
+
+// Slot and element of the e-th element of a round (slots in order).
+__device__ __forceinline__ void coop_elem(int e, const int* ne, int ng, int& s, int& i) {
+  s = 0;
+  i = e;
+  while (s + 1 < ng && i >= ne[s]) { i -= ne[s]; ++s; }
+}
 
 // Orders the LDS accesses of the lanes of one wave (no hardware barrier).
 __device__ __forceinline__ void wave_sync() {
@@ -232,19 +250,59 @@ __device__ __noinline__ static DelOut coop_del_event(const DelLog& lg, int mi, d
       cb->mi[slot] = mi;
     }
     wave_sync();
-    int off[COOP_SLOTS + 1];
-    int ms[COOP_SLOTS];
-    off[0] = 0;
+    int ms[COOP_SLOTS], ne[COOP_SLOTS];
+    int nt = 0;  // elements of the round
     for (int s = 0; s < ng; ++s) {
       ms[s] = min(max(cb->mi[s], 0), lg.n_media - 1);
-      off[s + 1] = off[s] + 4 * min(lg.media[ms[s]].kern.n_el, COOP_EL);
+      ne[s] = min(lg.media[ms[s]].kern.n_el, COOP_EL);
+      nt += ne[s];
     }
-    for (int w = rank; w < off[ng]; w += A) {
-      int s = 0;
-      while (w >= off[s + 1]) ++s;
-      const int j = w - off[s];
-      cb->comp[s][j] = diff_component(lg.media[ms[s]].kern, mass, j >> 2, (j & 3) + 1,
-                                      cb->anc[s], cb->q[s]);
+    // node terms and single values (differential_components, src/MixedTransport.jl:99-106)
+    for (int w = rank; w < 12 * nt; w += A) {
+      const int e = w / 12, node = w - 12 * e;
+      int s, i;
+      coop_elem(e, ne, ng, s, i);
+      const double* el = lg.media[ms[s]].kern.el + 4 * i;
+      cb->leaf[s][i][1 + node] = dpm::dcs_pair_node(el[0], el[1], mass, cb->anc[s], cb->q[s], node);
+    }
+    for (int w = rank; w < 9 * nt; w += A) {
+      const int e = w / 9, node = w - 9 * e;
+      int s, i;
+      coop_elem(e, ne, ng, s, i);
+      const double* el = lg.media[ms[s]].kern.el + 4 * i;
+      cb->leaf[s][i][13 + node] = dpm::dcs_photo_node(el[0], el[1], mass, cb->anc[s], cb->q[s], node);
+    }
+    for (int w = rank; w < 2 * nt; w += A) {
+      int s, i;
+      coop_elem(w >> 1, ne, ng, s, i);
+      const double* el = lg.media[ms[s]].kern.el + 4 * i;
+      if (w & 1) cb->leaf[s][i][22] = dpm::dcs_ionisation(el[0], el[2], mass, cb->anc[s], cb->q[s]);
+      else cb->leaf[s][i][0] = dpm::dcs_brems(el[0], el[1], mass, cb->anc[s], cb->q[s]);
+    }
+    wave_sync();
+    // components in (element, process) order: dcs_pair and dcs_photo sums in
+    // node order, then the factor of diff_component
+    for (int w = rank; w < 4 * nt; w += A) {
+      int s, i;
+      coop_elem(w >> 2, ne, ng, s, i);
+      const int p = w & 3;
+      const double* lf = cb->leaf[s][i];
+      double raw;
+      if (p == 0) {
+        raw = lf[0];
+      } else if (p == 1) {
+        double I = 0.0;
+        for (int n = 0; n < 12; ++n) I -= lf[1 + n];
+        raw = I < 0.0 ? 0.0 : I;
+      } else if (p == 2) {
+        double ds = 0.0;
+        for (int n = 0; n < 9; ++n) ds += lf[13 + n];
+        raw = ds < 0.0 ? 0.0 : ds;
+      } else {
+        raw = lf[22];
+      }
+      const double* el = lg.media[ms[s]].kern.el + 4 * i;
+      cb->comp[s][4 * i + p] = el[3] * dpm::AVOGADRO_NUMBER / (el[1] * .001) * cb->anc[s] * raw;
     }
     wave_sync();
     if (mine) {

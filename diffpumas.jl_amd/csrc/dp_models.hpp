@@ -294,32 +294,53 @@ DPM_HD double dcs_pair_d2(const PairInv& c, double rho_in) {
   return (diag_e + diag_mu) * c.scale;
 }
 
-// dcs_pair_production_ssr, src/Materials.jl:802-832
-DPM_HD_COLD double dcs_pair(double Z, double A, double mass, double K, double q) {
-  if (Z <= 0 || A <= 0 || mass <= 0 || K <= 0 || q <= 0) return 0.0;
-  if (q <= 4.0 * ELECTRON_MASS) return 0.0;
+// dcs_pair_production_ssr, src/Materials.jl:802-832, split into the
+// quadrature's range (ok = false: the cross-section is 0) and its node terms,
+// so that the nodes can also be evaluated apart and summed in node order.
+struct PairQuad {
+  double tmin;
+  bool ok;
+};
+DPM_HD PairQuad pair_quad(double Z, double A, double mass, double K, double q) {
+  PairQuad p{0.0, false};
+  if (Z <= 0 || A <= 0 || mass <= 0 || K <= 0 || q <= 0) return p;
+  if (q <= 4.0 * ELECTRON_MASS) return p;
   const double sqrte = 1.6487212707;
   const double Z13 = pow(Z, 1.0 / 3.0);
-  if (q >= K + mass * (1.0 - 0.75 * sqrte * Z13)) return 0.0;
+  if (q >= K + mass * (1.0 - 0.75 * sqrte * Z13)) return p;
   const double gamma = 1.0 + K / mass;
   const double x0 = 1.0 - 4.0 * ELECTRON_MASS / q;
   const double x1 = 1.0 - 6.0 / (gamma * (gamma - q / mass));
-  if (!(x0 > 0.0 && x1 > 0.0)) return 0.0;
+  if (!(x0 > 0.0 && x1 > 0.0)) return p;
   const double rmax = sqrt(x0) * x1;
   const double ri = 1.0 - rmax;
-  if (ri <= 0.0 || ri >= 1.0) return 0.0;
-  const double tmin = log(ri);
+  if (ri <= 0.0 || ri >= 1.0) return p;
+  p.tmin = log(ri);
+  p.ok = true;
+  return p;
+}
+// Term of node i: the quadrature is I = 0 - term(0) - term(1) - ... - term(11).
+DPM_HD double pair_node(const PairInv& c, double tmin, int i) {
+  const double r = exp(gq12_x(i) * tmin);
+  return dcs_pair_d2(c, r) * r * gq12_w(i) * tmin;
+}
+DPM_HD_COLD double dcs_pair(double Z, double A, double mass, double K, double q) {
+  const PairQuad p = pair_quad(Z, A, mass, K, q);
+  if (!p.ok) return 0.0;
   const PairInv c = pair_invariants(Z, A, mass, K, q);
   double I = 0.0;
   // Rolled on purpose: twelve inlined copies of the DDCS overflow the device
   // register file and spill around every node; one rolled body with the
   // dilogarithm inlined keeps its state in registers.
 #pragma unroll 1
-  for (int i = 0; i < 12; ++i) {
-    const double r = exp(gq12_x(i) * tmin);
-    I -= dcs_pair_d2(c, r) * r * gq12_w(i) * tmin;
-  }
+  for (int i = 0; i < 12; ++i) I -= pair_node(c, p.tmin, i);
   return I < 0.0 ? 0.0 : I;
+}
+// One node term of dcs_pair on its own (0 where the cross-section is 0).
+DPM_HD_COLD double dcs_pair_node(double Z, double A, double mass, double K, double q, int i) {
+  const PairQuad p = pair_quad(Z, A, mass, K, q);
+  if (!p.ok) return 0.0;
+  return pair_node(pair_invariants(Z, A, mass, K, q), p.tmin, i);
 }
 
 // -------------------------------------------------------- photonuclear
@@ -372,29 +393,51 @@ DPM_HD double photo_d2(double Z, double A, double ml, double K, double q, double
                      0.25 / (E * E * Q2);
   return cf * F2A * dds / q;
 }
-// dcs_photonuclear_drss, src/Materials.jl:897-926
-DPM_HD_COLD double dcs_photo(double Z, double A, double m, double K, double q) {
-  if (Z <= 0 || A <= 0 || m <= 0 || K <= 0 || q <= 0) return 0.0;
+// dcs_photonuclear_drss, src/Materials.jl:897-926, split like dcs_pair.
+struct PhotoQuad {
+  double lmin, dx;
+  bool ok;
+};
+DPM_HD PhotoQuad photo_quad(double Z, double A, double m, double K, double q) {
+  PhotoQuad p{0.0, 0.0, false};
+  if (Z <= 0 || A <= 0 || m <= 0 || K <= 0 || q <= 0) return p;
   const double M = 0.5 * (NEUTRON_MASS + PROTON_MASS);
   const double mpi = PION_MASS;
   const double qmin = mpi + 0.5 * mpi * mpi / M;
   const double qmax = K + m - 0.5 * (M + m * m / M);
-  if (q <= qmin || q >= qmax) return 0.0;
+  if (q <= qmin || q >= qmax) return p;
   const double E = K + m;
   const double ml2 = m * m;
   const double Q2min = ml2 * (q * q - 0.5 * ml2) / (E * (E - q));
   const double Q2max = 2.0 * M * (q - mpi) - mpi * mpi;
-  if (Q2max < Q2min || Q2min < 0.0) return 0.0;
-  const double lmin = log(Q2min);
-  const double dx = log(Q2max) - lmin;
+  if (Q2max < Q2min || Q2min < 0.0) return p;
+  p.lmin = log(Q2min);
+  p.dx = log(Q2max) - p.lmin;
+  p.ok = true;
+  return p;
+}
+// Term of node i (the reference skips a node whose DDCS is not positive; its
+// term here is +0, and adding +0 to the non-negative running sum changes
+// nothing): ds = 0 + term(0) + ... + term(8).
+DPM_HD double photo_node(double Z, double A, double m, double K, double q, const PhotoQuad& p,
+                         int i) {
+  const double Q2 = exp(p.lmin + p.dx * gq9_x(i));
+  const double v = photo_d2(Z, A, m, K, q, Q2);
+  return v > 0.0 ? v * (gq9_w(i) * p.dx * Q2) : 0.0;
+}
+DPM_HD_COLD double dcs_photo(double Z, double A, double m, double K, double q) {
+  const PhotoQuad p = photo_quad(Z, A, m, K, q);
+  if (!p.ok) return 0.0;
   double ds = 0.0;
 #pragma unroll 1
-  for (int i = 0; i < 9; ++i) {
-    const double Q2 = exp(lmin + dx * gq9_x(i));
-    const double v = photo_d2(Z, A, m, K, q, Q2);
-    if (v > 0.0) ds += v * (gq9_w(i) * dx * Q2);
-  }
+  for (int i = 0; i < 9; ++i) ds += photo_node(Z, A, m, K, q, p, i);
   return ds < 0.0 ? 0.0 : ds;
+}
+// One node term of dcs_photo on its own (0 where the cross-section is 0).
+DPM_HD_COLD double dcs_photo_node(double Z, double A, double m, double K, double q, int i) {
+  const PhotoQuad p = photo_quad(Z, A, m, K, q);
+  if (!p.ok) return 0.0;
+  return photo_node(Z, A, m, K, q, p, i);
 }
 
 // -------------------------------------------------------- ionisation
