@@ -344,12 +344,12 @@ __device__ double flat_event_f(const StochArgs& a, double energy, double charge,
         else stb = 1000.0;
       }
       // _backward_geometric_step_limit, src/Geometry.jl:463-476
-      const Loc L = loc_e(T, st.E, log(st.E), g);
-      const double Xi = mix_range_L(T, md->mix, false, L);
+      const Loc L = loc_e(a.T, st.E, log(st.E), g);
+      const double Xi = mix_range_L(a.T, md->mix, false, L);
       double lim = 0.01 * Xi / fmax(density, 1e-6);
       double lb0 = 0.0;
       if (use_scat) {
-        lb0 = mix_transport_path_L(T, md->mix, L);
+        lb0 = mix_transport_path_L(a.T, md->mix, L);
         if (lb0 > 0 && density > 0) lim = fmin(lim, 0.01 * lb0 / density);
       }
       if (in_air) lim = fmin(lim, 0.05 * 12e3 / fmax(fabs(uz), 0.05));
