@@ -300,7 +300,7 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
   constexpr bool F64 = sizeof(F) == sizeof(double);
   static_assert(TLDS || F64, "FP32 stepping stages its tables in LDS");
   using Tab = typename std::conditional<F64, Tab64, Tab32>::type;
-  extern __shared__ double lds[];
+  HIP_DYNAMIC_SHARED(double, lds)
   const int K = a.K;
   const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = bd >> 6;

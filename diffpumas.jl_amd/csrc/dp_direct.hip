@@ -45,7 +45,7 @@ __device__ __forceinline__ R decay_direct(double mass, double ctau, double path,
 // 4 waves per SIMD (128 VGPRs, no spill): measured 5 % faster than the
 // unbounded 141 VGPRs at 3 waves per SIMD.
 __global__ __launch_bounds__(256, 4) void csda_direct_kernel(DirectArgs a) {
-  extern __shared__ double lds[];  // rock, then air: E, log E, S_csda, R_csda
+  HIP_DYNAMIC_SHARED(double, lds)  // rock, then air: E, log E, S_csda, R_csda
   const int K = a.K;
   {
     const int mats[2] = {a.rock, a.air};

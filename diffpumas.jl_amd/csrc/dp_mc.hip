@@ -28,7 +28,10 @@ constexpr int MC_ARRS = 5;  // energy, log energy, csda range, csda stopping, tr
 // Stochastic MC blocks: one wave, with a pool of MC_POOL deferred hard-event
 // records evaluated by the whole wave after each round of 64 events.
 constexpr int MC_THREADS = 64;
-constexpr int MC_POOL = 192;
+#ifndef DP_MC_POOL
+#define DP_MC_POOL 192  // a smaller pool (host emulation builds) exercises the inline fallback
+#endif
+constexpr int MC_POOL = DP_MC_POOL;
 constexpr int MC_BUCKETS = 256;  // source-energy buckets of the event order
 
 struct FlatGeo {

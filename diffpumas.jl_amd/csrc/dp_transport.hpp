@@ -610,7 +610,7 @@ __device__ __noinline__ static void mixed_step(const DevTables& T, const Medium&
 }
 
 // transport_backward_step (deterministic CSDA) on a mixture, src/Geometry.jl:424-457
-__device__ void csda_mix_step(const DevTables& T, const Medium& md, McState& st, double density,
+__device__ static void csda_mix_step(const DevTables& T, const Medium& md, McState& st, double density,
                               double ds) {
   const DevMix& x = md.mix;
   const double ki = st.E, lki = log(ki), gs = ds * density;

@@ -350,7 +350,7 @@ __device__ __forceinline__ double half_sum(double v) {
 // others keep the default bound, which measured faster for the plain Jacobian.
 template <bool JAC, bool CORR, bool PACK>
 __global__ __launch_bounds__(256, (JAC && CORR) ? 2 : 1) void wcsda_kernel(WArgs a) {
-  extern __shared__ double lds[];
+  HIP_DYNAMIC_SHARED(double, lds)
   const int K = a.K;
   const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int grp = PACK ? (tid >> 5) : 0;      // bin slot of this lane

@@ -29,11 +29,13 @@
 #define __shared__ static
 #define __constant__
 #define HIP_DYNAMIC_SHARED(type, var) type* var = (type*)g_dyn_shared;
+#define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 {
   unsigned x = 1, y = 1, z = 1;
   dim3() {}
   dim3(unsigned a) : x(a) {}
+  dim3(unsigned a, unsigned b, unsigned c = 1) : x(a), y(b), z(c) {}
 };
 inline thread_local dim3 blockIdx, threadIdx, blockDim, gridDim;
 inline void* g_dyn_shared = nullptr;
@@ -65,6 +67,33 @@ inline double __shfl_xor(double v, int off, int width = 64) {
 #define __builtin_amdgcn_wave_barrier() pthread_barrier_wait(&g_bar)
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 inline int __lane_id() { return threadIdx.x & 63; }
+// value of lane k of the caller's wave (every thread of the block calls it)
+inline int __builtin_amdgcn_readlane(int v, int k) {
+  static int xch[1024];
+  pthread_barrier_wait(&g_bar);
+  xch[threadIdx.x] = v;
+  pthread_barrier_wait(&g_bar);
+  const int r = xch[(threadIdx.x & ~63u) + (unsigned)k];
+  pthread_barrier_wait(&g_bar);
+  return r;
+}
+inline int __double2hiint(double x) {
+  long long b;
+  memcpy(&b, &x, 8);
+  return (int)(b >> 32);
+}
+inline int __double2loint(double x) {
+  long long b;
+  memcpy(&b, &x, 8);
+  return (int)(unsigned)(b & 0xffffffffll);
+}
+inline double __hiloint2double(int hi, int lo) {
+  const unsigned long long b = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+  double x;
+  memcpy(&x, &b, 8);
+  return x;
+}
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 
 // ------------------------------------------------------ lane intrinsics
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
@@ -191,3 +220,8 @@ inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorN
 inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
 inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
 #define HIP_SYMBOL(x) (&x)
+// device symbols are host objects: their address is the object's address
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
