@@ -494,6 +494,7 @@ void orc_ray_surface_exit(int nx, int ny, double x0, double y0, double dx, doubl
                           const double* dirs, double step, double max_distance, double* dist,
                           uint8_t* found, double* exit_z) {
   Dem m{nx, ny, x0, y0, dx, dy, data};
+#pragma omp parallel for schedule(dynamic, 16)
   for (int i = 0; i < n; ++i) {
     bool f;
     ray_surface_exit(m, xy_scale, z_ref, dirs + 3 * i, step, max_distance, dist[i], f, exit_z[i]);
@@ -521,6 +522,7 @@ void orc_trace_tetra(int np, int nt, const double* pts, const int* tets, const i
                      const double* origin, const double* dir, const double* surface, int* counts,
                      const int* row_ptr, int* cells, double* lens, double* traveled, int* start) {
   TetMesh m{np, nt, pts, tets, nbr};
+#pragma omp parallel for schedule(dynamic, 1)
   for (int r = 0; r < n; ++r) {
     const double* o = origin + 3 * r;
     const double* d = dir + 3 * r;
@@ -557,6 +559,7 @@ void orc_csda_direct(int M, int K, const double* data, const double* scal, doubl
                      const double* rho, const double* thick, const double* elev,
                      const double* energy, const double* charge, double* flux, double* dflux) {
   Tables T = make_tables(M, K, data, scal, mass, ctau);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     Dual f = flux_csda_direct(T, rock, air, Dual(rho[i], 1.0), thick[i], elev[i], energy[i],
                               charge[i], primary_altitude);

@@ -35,19 +35,25 @@ inline void normal_equations(int n_rows, int n_cols, const int* rp, const int* c
       by_col[a].push_back({r, v});   // rows ascending: r runs in order
       by_row[r].push_back({a, v});
     }
+  // columns are independent (each keeps its row order): spread over threads
   if (normal) {
-    std::vector<double> x(n_act);
-    for (int j = 0; j < n_act; ++j) {
-      for (int i = 0; i < n_act; ++i) x[i] = 0.0;
-      for (const auto& rv : by_col[j]) {
-        const int r = rv.first;
-        const double nz_b = W ? W[r] * rv.second : rv.second;   // (W .* Ja)[r, j]
-        for (const auto& iv : by_row[r]) x[iv.first] += iv.second * nz_b;
+#pragma omp parallel
+    {
+      std::vector<double> x(n_act);
+#pragma omp for schedule(dynamic, 8)
+      for (int j = 0; j < n_act; ++j) {
+        for (int i = 0; i < n_act; ++i) x[i] = 0.0;
+        for (const auto& rv : by_col[j]) {
+          const int r = rv.first;
+          const double nz_b = W ? W[r] * rv.second : rv.second;   // (W .* Ja)[r, j]
+          for (const auto& iv : by_row[r]) x[iv.first] += iv.second * nz_b;
+        }
+        for (int i = 0; i < n_act; ++i) normal[(size_t)j * n_act + i] = x[i];
       }
-      for (int i = 0; i < n_act; ++i) normal[(size_t)j * n_act + i] = x[i];
     }
   }
   if (grad) {
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < n_act; ++j) {
       double g = 0.0;
       for (const auto& rv : by_col[j]) {
