@@ -54,12 +54,12 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy), from the counter
 # runs summarised in profiles/; `traffic` = this x the units of one launch.
 MEASURED_BYTES_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 387521.0,    # per event
+    "mc_flat_stochastic_kernel": 269087.0,    # per event
     "csda_cube_kernel": 8909.2,               # per ray-voxel (all 256 nodes)
     "csda_cube_kernel_fp32": 4691.0,          # per ray-voxel (FP32 stepping)
     "wcsda_kernel": 2127.0,                   # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 4314.0,         # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 271403.0,               # per event
+    "path_mc_kernel": 253284.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
     "trace_cube_kernel": 13.0,                # per ray-voxel (fill launch)
     "ne_normal_kernel": 11.0,                 # per entry of the dense normal matrix
@@ -76,12 +76,12 @@ VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 # in profiles/).  The kernels are FP64 VALU bound rather than HBM bound; the
 # `valu` object prices the live kernel time against VALU_PEAK_WAVE_INSTS.
 MEASURED_VALU_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 36622.0,     # per event
+    "mc_flat_stochastic_kernel": 26531.5,     # per event
     "csda_cube_kernel": 1166.0,               # per ray-voxel (all 256 nodes)
     "csda_cube_kernel_fp32": 1309.8,          # per ray-voxel (FP32 stepping)
     "wcsda_kernel": 10120.0,                  # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 10495.0,        # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 13768.9,                # per event
+    "path_mc_kernel": 13906.2,                # per event
     "csda_direct_kernel": 39.99,              # per sample
     "trace_cube_kernel": 5.19,                # per ray-voxel (fill launch)
     "ray_exit_kernel": 951.5,                 # per ray
