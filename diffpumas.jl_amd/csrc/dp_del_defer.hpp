@@ -51,12 +51,13 @@ struct DelSample {
   bool deferred;  // weight = (sum of components at (anc, anc - k)) / denom
 };
 
-// The draws of reverse_del_event up to the ancestor energy.
-__device__ __forceinline__ DelSample del_sample(const DevKernel& c, double mass, double k,
+// The draws of reverse_del_event up to the ancestor energy; g guesses the
+// kernel-grid interval of k.
+__device__ __forceinline__ DelSample del_sample(const DevKernel& c, double mass, double k, int g,
                                                 Philox& rng, int& err) {
   DelSample d;
   d.anc = k; d.w = 0.0; d.denom = 1.0; d.deferred = false;
-  const double sigma = klookup(k, c.e, c.sigma, c.n, c.pow2, err);
+  const double sigma = klookup_g(k, c.e, c.sigma, c.n, c.pow2, g, err);
   if (err) return d;
   if (!(sigma > 0)) { err |= ERR_TRANSPORT; return d; }
   const double pcel = k >= c.onset ? 0.0
@@ -195,8 +196,8 @@ __device__ __forceinline__ DelOut del_finish(const DevKernel& c, double mass, do
 
 // Called by every lane active in the caller (the branch to it is uniform over
 // those lanes).  Lanes with del get reverse_del_event(details = true) at
-// energy k in medium mi: the same draws, ancestor energy, weight and target as
-// the inline evaluation.
+// energy k in medium mi (g guesses k's kernel-grid interval): the same draws,
+// ancestor energy, weight and target as the inline evaluation.
 //
 // Convergence: the routine does not assume that the whole wave, or any
 // particular set of lanes, is present.  It is reached from per-lane step
@@ -213,14 +214,15 @@ __device__ __forceinline__ DelOut del_finish(const DevKernel& c, double mass, do
 // that requires all 64 lanes at every ballot therefore cannot run this
 // routine; the per-event GPU tests against the oracle cover it.
 __device__ __noinline__ static DelOut coop_del_event(const DelLog& lg, int mi, double mass,
-                                                     double k, Philox& rng, bool del, int& err) {
+                                                     double k, int g, Philox& rng, bool del,
+                                                     int& err) {
   CoopBuf* cb = lg.coop;
   DelOut out{k, 0.0, 0, -1};
   DelSample d{k, 0.0, 1.0, false};
   bool need = false;
   if (del) {
     const DevKernel& c = lg.media[mi].kern;
-    d = del_sample(c, mass, k, rng, err);
+    d = del_sample(c, mass, k, g, rng, err);
     if (!err) {
       if (!d.deferred) out = DelOut{d.anc, d.w, 0, -1};
       else if (c.n_el > COOP_EL) out = del_finish(c, mass, k, d, rng);

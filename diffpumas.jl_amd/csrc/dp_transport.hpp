@@ -124,6 +124,26 @@ __device__ __forceinline__ double klookup(double x, const double* xs, const doub
   return d == 0 ? ys[i] : ys[i] + (x - xs[i]) / d * (ys[i + 1] - ys[i]);
 }
 
+// The same lookup from an interval guess g (the search returns the interval
+// the bisection returns, so values are those of klookup).
+__device__ __forceinline__ bool kloc(double x, const double* __restrict__ xs, int n, int pow2,
+                                     int g, int& i, int& err) {
+  if (!(x >= xs[0] && x <= xs[n - 1])) { err |= ERR_DOMAIN; i = 0; return false; }
+  i = locate_idx(xs, n, pow2, x, g);
+  return true;
+}
+__device__ __forceinline__ double kval(double x, const double* __restrict__ xs,
+                                       const double* __restrict__ ys, int i) {
+  const double d = xs[i + 1] - xs[i];
+  return d == 0 ? ys[i] : ys[i] + (x - xs[i]) / d * (ys[i + 1] - ys[i]);
+}
+__device__ __forceinline__ double klookup_g(double x, const double* xs, const double* ys, int n,
+                                            int pow2, int g, int& err) {
+  int i;
+  if (!kloc(x, xs, n, pow2, g, i, err)) return 0.0;
+  return kval(x, xs, ys, i);
+}
+
 // ------------------------------------------------------ random variates
 __device__ __forceinline__ double uniform(Philox& r) { return r.uniform(); }
 __device__ __forceinline__ double exponential(Philox& r) { return -log(1.0 - r.uniform()); }

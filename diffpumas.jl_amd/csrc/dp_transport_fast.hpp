@@ -109,24 +109,6 @@ __device__ __forceinline__ double ehs_path_L(const DevTables& T, const DevMix& x
   return inv > 0 ? 1.0 / inv : EHS_PATH_MAX;
 }
 
-// Mixed-kernel lookups (src/MixedTransport.jl:43-49) on a located interval.
-__device__ __forceinline__ bool kloc(double x, const double* __restrict__ xs, int n, int pow2,
-                                     int g, int& i, int& err) {
-  if (!(x >= xs[0] && x <= xs[n - 1])) { err |= ERR_DOMAIN; i = 0; return false; }
-  i = locate_idx(xs, n, pow2, x, g);
-  return true;
-}
-__device__ __forceinline__ double kval(double x, const double* __restrict__ xs,
-                                       const double* __restrict__ ys, int i) {
-  const double d = xs[i + 1] - xs[i];
-  return d == 0 ? ys[i] : ys[i] + (x - xs[i]) / d * (ys[i + 1] - ys[i]);
-}
-__device__ __forceinline__ double klookup_g(double x, const double* xs, const double* ys, int n,
-                                            int pow2, int g, int& err) {
-  int i;
-  if (!kloc(x, xs, n, pow2, g, i, err)) return 0.0;
-  return kval(x, xs, ys, i);
-}
 // Guess of the kernel-grid interval of a located energy: the kernel grid
 // refines the shared grid 8 times per interval in log energy.
 __device__ __forceinline__ int kguess(const DevKernel& K, const Loc& L) {
@@ -302,8 +284,8 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
   bool del = false, ehs = false;
   double X_del = __builtin_huge_val(), k_del = 0, X_ehs = __builtin_huge_val(), k_ehs = 0;
   Loc Le;
+  const int kg = kguess(K, Li);  // kernel-grid interval guess near ki
   {
-    const int kg = kguess(K, Li);
     const double rate = fmax(klookup_g(ki, K.e, K.sigma, K.n, K.pow2, kg, err),
                              klookup_g(kf, K.e, K.sigma, K.n, K.pow2, kg, err));
     if (err) return;
@@ -335,7 +317,7 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
     // the call is skipped (uniformly) when no lane has a DEL event, where it
     // would draw nothing and return nothing that is used
     DelOut o{k_del, 0.0, 0, -1};
-    if (__ballot(del)) o = coop_del_event(*dlog, dlog->medium, mass, k_del, rng, del, err);
+    if (__ballot(del)) o = coop_del_event(*dlog, dlog->medium, mass, k_del, kg, rng, del, err);
     if (del) {
       if (err) return;
       ckf = k_del;
@@ -346,7 +328,7 @@ __device__ __noinline__ static void straggled_step_f(const DevTables& T, const M
     ckf = k_del;
     if (!scattering && dlog) {
       // the target choice only consumes its draw when nothing is deflected
-      const DelSample d = del_sample(K, mass, k_del, rng, err);
+      const DelSample d = del_sample(K, mass, k_del, kg, rng, err);
       if (err) return;
       kf = d.anc;
       if (d.deferred) {
@@ -457,7 +439,7 @@ __device__ __noinline__ static void mixed_step_f(const DevTables& T, const Mediu
     ik = ip;
     if (occurs) {
       if (dlog) {
-        const DelSample d = del_sample(c, m, k, rng, err);
+        const DelSample d = del_sample(c, m, k, ik, rng, err);
         if (err) return;
         if (d.deferred && del_push(dlog, d, k, 0)) {
           k = d.anc;

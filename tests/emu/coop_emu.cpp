@@ -92,7 +92,9 @@ void lane_kernel(Args a) {
   DelLog lg{nullptr, nullptr, 0, t, 0, mi, &coop, a.media, a.n_media};
   Philox ref = rng;
   int err = 0;
-  const DelOut o = coop_del_event(lg, mi, MASS, k, rng, del, err);
+  // any interval guess gives the bisection's interval
+  const int g = (int)(a.media[mi].kern.n * ((t * 37 + 11) % 64) / 64.0);
+  const DelOut o = coop_del_event(lg, mi, MASS, k, g, rng, del, err);
   bool ok;
   if (del) {
     int err_ref = 0;
