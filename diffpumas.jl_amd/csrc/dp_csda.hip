@@ -425,10 +425,21 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
         double P = 1.0, Q = 0.0;
         int base = 0;
         // ------------------------------------------------ forward sweep
+        // The next segment's length and density are loaded one segment
+        // ahead: the density gather (cell index, then a random 8 MiB field)
+        // is a chain of two dependent loads that the segment map hides.
+        double d_nx = 0.0, rho_nx = 0.0;
+        if (ns > 0) {
+          d_nx = a.len[b0];
+          rho_nx = a.dens[a.cell[b0]];
+        }
         for (int s = 0; s < ns; ++s) {
-          const int cell = a.cell[b0 + s];
-          const double d = a.len[b0 + s];
-          const double rho = a.dens[cell];
+          const double d = d_nx;
+          const double rho = rho_nx;
+          if (s + 1 < ns) {
+            d_nx = a.len[b0 + s + 1];
+            rho_nx = a.dens[a.cell[b0 + s + 1]];
+          }
           StepOutT<F> o = seg(rock, E, lE, d * rho, F(d), ie, jr, en);
           ok = ok && o.ok;
           W = W * o.ratio;
