@@ -476,12 +476,28 @@ __global__ __launch_bounds__(256, (JAC && CORR) ? 2 : 1) void wcsda_kernel(WArgs
       const bool acc = CORR && C.enabled;
       bool ok = active;  // lanes past the last sample walk nothing and add 0
       // ------------------------------------------------ forward sweep
+      // The next segment's water fraction, porous flag and length are
+      // loaded one segment ahead (the gathers from the field hide behind the
+      // current segment's sub-steps).
+      double wv_nx = 0.0, d_nx = 0.0;
+      bool por_nx = false;
+      if (ns > 0 && scell[0] >= 0) {
+        wv_nx = a.w[scell[0]];
+        por_nx = M.porous_on && a.shallow[scell[0]] != 0;
+        d_nx = a.len[b0];
+      }
       for (int s = 0; s < ns && ok; ++s) {
         const int c = scell[s];
         if (c < 0) { ok = false; break; }
-        const double wv = a.w[c];
-        const bool porous = M.porous_on && a.shallow[c] != 0;
-        const double d = a.len[b0 + s];
+        const double wv = wv_nx;
+        const bool porous = por_nx;
+        const double d = d_nx;
+        if (s + 1 < ns && scell[s + 1] >= 0) {
+          const int cn = scell[s + 1];
+          wv_nx = a.w[cn];
+          por_nx = M.porous_on && a.shallow[cn] != 0;
+          d_nx = a.len[b0 + s + 1];
+        }
         if (JAC) {
           const D2 wd(wv, 1.0, 0.0);
           D2 e(E, 0.0, 1.0), g(W);
