@@ -279,12 +279,22 @@ __device__ bool air_segment(const WTab& T, double z0, double cos_t, double dista
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
     R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
-    R alt0 = z0 + cos_t * (traversed + 0.5 * cmin(remaining, MAX_STEP_M));
+    const R cap = cmin(remaining, MAX_STEP_M);
+    R alt0 = z0 + cos_t * (traversed + 0.5 * cap);
     R dens0 = 1.205 * exp(-alt0 / 12e3);
-    R ds = cmin(cmin(remaining, MAX_STEP_M), grel / cmax(dens0, 1e-12));
+    // ds = cmax(cmin(cap, grel / dens0), 1e-3); where that is cap itself (the
+    // usual case: the 60 m cap binds), the mid-step altitude below is alt0
+    // and its density dens0, so the exponential is not evaluated again
+    const R lim = grel / cmax(dens0, 1e-12);
+    const bool capped = val(cap) <= val(lim);
+    R ds = capped ? cap : lim;
+    const bool floored = !(val(ds) >= 1e-3);
     ds = cmax(ds, 1e-3);
-    R alt = z0 + cos_t * (traversed + 0.5 * ds);
-    R density = 1.205 * exp(-alt / 12e3);
+    R density = dens0;
+    if (!capped || floored) {
+      R alt = z0 + cos_t * (traversed + 0.5 * ds);
+      density = 1.205 * exp(-alt / 12e3);
+    }
     R gram = ds * density;
     R epred = energy + dedx0 * gram;
     R dedx1 = air_stop(T, epred, g);
