@@ -181,12 +181,16 @@ __device__ __forceinline__ R log_var_corr(R Es, double V, double H, double ct, c
 // compute_decay_weight_from_path, src/Geometry.jl:1083-1107 (the beta floor
 // branch is chosen on the primal value).
 template <class R, class P>
-__device__ __forceinline__ R decay_r(const WTab& T, P path, R eavg) {
+__device__ __forceinline__ R decay_exponent(const WTab& T, P path, R eavg) {
   R g = 1.0 + eavg / T.mass;
   R b2 = 1.0 - 1.0 / (g * g);
   R beta = sqrt(val(b2) >= 1e-12 ? b2 : R(1e-12));
   R L = beta * g * T.ctau;
-  return exp(-path / L);
+  return path / L;
+}
+template <class R, class P>
+__device__ __forceinline__ R decay_r(const WTab& T, P path, R eavg) {
+  return exp(-decay_exponent(T, path, eavg));
 }
 
 // interp_r with the interval searched from the guess g (updated); locate_idx
@@ -276,6 +280,9 @@ __device__ bool air_segment(const WTab& T, double z0, double cos_t, double dista
   R remaining = R(distance);
   R traversed = R(0.0);
   R dedx0 = air_stop(T, energy, g);
+  // the sub-steps' decay exponents are summed and applied once at the end
+  // (one exponential per walk instead of one per sub-step)
+  R xdecay = R(0.0);
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
     R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
@@ -302,12 +309,13 @@ __device__ bool air_segment(const WTab& T, double z0, double cos_t, double dista
     if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
     R dnext = air_stop(T, enext, g);
     weight = weight * (dnext / cmax(dedx0, 1e-30));
-    weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
+    xdecay = xdecay + decay_exponent(T, ds, 0.5 * (energy + enext));
     energy = enext;
     traversed = traversed + ds;
     remaining = remaining - ds;
     dedx0 = dnext;
   }
+  weight = weight * exp(-xdecay);
   return true;
 }
 
