@@ -5,6 +5,12 @@
 //                      the detector + POINT_EPS along the ray.  The lanes test
 //                      64 tets at a time; the lowest set bit of the ballot is
 //                      the first tet in index order, as the reference's scan.
+//                      Rays that share an origin scan only that origin's
+//                      candidate list: the tets, in index order, whose
+//                      bounding box, widened beyond the point-in-tet
+//                      tolerance and the POINT_EPS nudge, holds the origin.
+//                      A tet outside it cannot contain the nudged point, so
+//                      the first hit is the full scan's.
 //   tet_trace_kernel   one ray per lane: nearest exit face (Moller-Trumbore,
 //                      src/TriangleIntersect.jl:86-127, with the tolerant
 //                      plane hit of :484-501 as fallback), step to it, nudge
@@ -20,7 +26,9 @@
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstring>
 #include <tuple>
 #include <vector>
 
@@ -219,22 +227,42 @@ __device__ int nearest_face(const TetDev& m, int t, const double* p, const doubl
   return bf;
 }
 
+// group[r] >= 0: scan cand[gptr[group[r]] .. gptr[group[r] + 1]) (tet indices
+// in ascending order); -1 (or no groups): scan every tet.
 __global__ __launch_bounds__(64) void tet_locate_kernel(TetDev m, int n,
                                                         const double* __restrict__ origin,
                                                         const double* __restrict__ dir,
+                                                        const int32_t* __restrict__ group,
+                                                        const int32_t* __restrict__ gptr,
+                                                        const int32_t* __restrict__ cand,
                                                         int32_t* __restrict__ start) {
   for (int r = blockIdx.x; r < n; r += gridDim.x) {
     const double p[3] = {origin[3 * r] + dir[3 * r] * TET_POINT_EPS,
                          origin[3 * r + 1] + dir[3 * r + 1] * TET_POINT_EPS,
                          origin[3 * r + 2] + dir[3 * r + 2] * TET_POINT_EPS};
     int found = -1;
-    for (int c0 = 0; c0 < m.nt; c0 += 64) {
-      const int t = c0 + (int)threadIdx.x;
-      const bool in = t < m.nt && point_in_tet(m, t, p);
-      const unsigned long long mask = __ballot(in);
-      if (mask) {
-        found = c0 + __ffsll((unsigned long long)mask) - 1;
-        break;
+    const int g = group ? group[r] : -1;
+    if (g >= 0) {
+      const int c_beg = gptr[g], c_end = gptr[g + 1];
+      for (int c0 = c_beg; c0 < c_end; c0 += 64) {
+        const int j = c0 + (int)threadIdx.x;
+        const int t = j < c_end ? cand[j] : -1;
+        const bool in = t >= 0 && point_in_tet(m, t, p);
+        const unsigned long long mask = __ballot(in);
+        if (mask) {
+          found = cand[c0 + __ffsll((unsigned long long)mask) - 1];
+          break;
+        }
+      }
+    } else {
+      for (int c0 = 0; c0 < m.nt; c0 += 64) {
+        const int t = c0 + (int)threadIdx.x;
+        const bool in = t < m.nt && point_in_tet(m, t, p);
+        const unsigned long long mask = __ballot(in);
+        if (mask) {
+          found = c0 + __ffsll((unsigned long long)mask) - 1;
+          break;
+        }
       }
     }
     if (threadIdx.x == 0) start[r] = found;
@@ -368,13 +396,73 @@ int dp_trace_tetra(dp_ctx c, const dp_tet_mesh* mesh, int32_t n, const double* o
   }
   DP_HIP(hipSetDevice(c->device));
   const size_t np = (size_t)mesh->n_points, nt = (size_t)mesh->n_tets;
+  // Rays grouped by their exact origin; for at most MAX_GROUPS origins, each
+  // origin's candidate tets (ascending index): widened bounding box holds
+  // the origin.  Widening: the nudge radius POINT_EPS |dir| (largest of the
+  // group), 1e-2 of the box's largest extent (the barycentric tolerance 1e-8
+  // moves an accepted point at most 4e-8 extents outside the tet; the rest
+  // covers the solve's rounding on poorly shaped tets), and 1e-6 m.
+  constexpr int MAX_GROUPS = 64;
+  std::vector<int32_t> group((size_t)n, -1), gptr(1, 0), cand;
+  {
+    std::vector<std::array<double, 3>> orig;
+    std::vector<double> rad;
+    bool ok = true;
+    for (int r = 0; r < n && ok; ++r) {
+      const double* q = origin + 3 * (size_t)r;
+      int g = -1;
+      for (size_t k = 0; k < orig.size(); ++k)
+        if (std::memcmp(orig[k].data(), q, 24) == 0) { g = (int)k; break; }
+      if (g < 0) {
+        if ((int)orig.size() == MAX_GROUPS) { ok = false; break; }
+        orig.push_back({q[0], q[1], q[2]});
+        rad.push_back(0.0);
+        g = (int)orig.size() - 1;
+      }
+      const double* d = dir + 3 * (size_t)r;
+      const double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      if (!std::isfinite(dn)) { ok = false; break; }
+      rad[g] = std::max(rad[g], dn * TET_POINT_EPS);
+      group[r] = g;
+    }
+    if (!ok) {
+      std::fill(group.begin(), group.end(), -1);
+      orig.clear();
+    }
+    std::vector<std::vector<int32_t>> per(orig.size());
+    if (!orig.empty())
+      for (size_t t = 0; t < nt; ++t) {
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+        for (int k = 0; k < 4; ++k) {
+          const double* v = mesh->points + 3 * (size_t)mesh->tets[4 * t + k];
+          for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], v[a]); hi[a] = std::max(hi[a], v[a]); }
+        }
+        const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        for (size_t g = 0; g < orig.size(); ++g) {
+          const double w = 1e-2 * ext + 1.01 * rad[g] + 1e-6;
+          bool in = true;  // a box that is not finite keeps the tet
+          if (std::isfinite(w))
+            for (int a = 0; a < 3; ++a)
+              if (!(orig[g][a] >= lo[a] - w && orig[g][a] <= hi[a] + w)) in = false;
+          if (in) per[g].push_back((int32_t)t);
+        }
+      }
+    for (size_t g = 0; g < per.size(); ++g) {
+      cand.insert(cand.end(), per[g].begin(), per[g].end());
+      gptr.push_back((int32_t)cand.size());
+    }
+  }
+  const size_t n_groups = gptr.size() - 1;
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o = dph::align_up(o + bytes, 256); return at; };
   const size_t o_p = take(24 * np), o_t = take(16 * nt), o_nb = take(16 * nt),
                o_or = take(24 * (size_t)n), o_dir = take(24 * (size_t)n),
                o_sd = take(8 * (size_t)n), o_st = take(4 * (size_t)n), o_cnt = take(4 * (size_t)n),
                o_rp = take(4 * (size_t)(n + 1)), o_cell = take(4 * (size_t)nseg),
-               o_len = take(8 * (size_t)nseg), o_tr = take(8 * (size_t)n), o_fl = take(4);
+               o_len = take(8 * (size_t)nseg), o_tr = take(8 * (size_t)n), o_fl = take(4),
+               o_grp = take(4 * (size_t)n), o_gptr = take(4 * gptr.size()),
+               o_cand = take(4 * std::max<size_t>(cand.size(), 1));
   rc = dph::ensure(c->staging, o);
   if (rc) return rc;
   char* b = (char*)c->staging.ptr;
@@ -389,6 +477,12 @@ int dp_trace_tetra(dp_ctx c, const dp_tet_mesh* mesh, int32_t n, const double* o
     DP_HIP(hipMemcpyAsync(b + o_rp, row_ptr, 4 * (size_t)(n + 1), hipMemcpyHostToDevice, st));
   const int zero = 0;
   DP_HIP(hipMemcpyAsync(b + o_fl, &zero, 4, hipMemcpyHostToDevice, st));
+  if (n_groups > 0) {
+    DP_HIP(hipMemcpyAsync(b + o_grp, group.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    DP_HIP(hipMemcpyAsync(b + o_gptr, gptr.data(), 4 * gptr.size(), hipMemcpyHostToDevice, st));
+    if (!cand.empty())
+      DP_HIP(hipMemcpyAsync(b + o_cand, cand.data(), 4 * cand.size(), hipMemcpyHostToDevice, st));
+  }
   TetDev m{mesh->n_points, mesh->n_tets, (const double*)(b + o_p), (const int32_t*)(b + o_t),
            (const int32_t*)(b + o_nb)};
   const double* d_or = (const double*)(b + o_or);
@@ -396,7 +490,9 @@ int dp_trace_tetra(dp_ctx c, const dp_tet_mesh* mesh, int32_t n, const double* o
   int32_t* d_st = (int32_t*)(b + o_st);
   const unsigned loc_grid = (unsigned)std::min(n, 65536);
   dph::record(c, c->ev0);
-  hipLaunchKernelGGL(tet_locate_kernel, dim3(loc_grid), dim3(64), 0, st, m, n, d_or, d_dir, d_st);
+  hipLaunchKernelGGL(tet_locate_kernel, dim3(loc_grid), dim3(64), 0, st, m, n, d_or, d_dir,
+                     n_groups > 0 ? (const int32_t*)(b + o_grp) : nullptr,
+                     (const int32_t*)(b + o_gptr), (const int32_t*)(b + o_cand), d_st);
   hipLaunchKernelGGL(tet_trace_kernel, dim3((n + 63) / 64), dim3(64), 0, st, m, n, d_or, d_dir,
                      (const double*)(b + o_sd), (const int32_t*)d_st,
                      cells ? nullptr : (int32_t*)(b + o_cnt), (const int32_t*)(b + o_rp),
