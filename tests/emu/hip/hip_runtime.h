@@ -2,7 +2,7 @@
 // diffpumas.jl_amd/csrc compiles unchanged with g++ and runs on the CPU.
 //
 // Kernels: every block runs as blockDim.x OS threads meeting at a shared
-// barrier; blocks run one after the other.  __shared__ variables are
+// barrier; blocks (x, then y, then z of the grid) run one after the other.  __shared__ variables are
 // function statics (shared by the block's threads; blocks do not overlap).
 // __ballot, __shfl_xor and wave barriers are exchanges between all threads
 // of the block at a barrier, so they model 64-thread (one-wave) blocks and
@@ -136,18 +136,21 @@ inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v)
 template <class K, class... A>
 void emu_launch_shm(K k, dim3 g, dim3 b, size_t shm, A... args) {
   std::vector<unsigned char> dyn(shm > 0 ? shm : 8);
-  for (unsigned blk = 0; blk < g.x; ++blk) {
-    g_dyn_shared = dyn.data();
-    pthread_barrier_init(&g_bar, nullptr, b.x);
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < b.x; ++t)
-      th.emplace_back([=]() {
-        blockIdx.x = blk; threadIdx.x = t; blockDim = b; gridDim = g;
-        k(args...);
-      });
-    for (auto& x : th) x.join();
-    pthread_barrier_destroy(&g_bar);
-  }
+  for (unsigned bz = 0; bz < g.z; ++bz)
+    for (unsigned by = 0; by < g.y; ++by)
+      for (unsigned blk = 0; blk < g.x; ++blk) {
+        g_dyn_shared = dyn.data();
+        pthread_barrier_init(&g_bar, nullptr, b.x);
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < b.x; ++t)
+          th.emplace_back([=]() {
+            blockIdx.x = blk; blockIdx.y = by; blockIdx.z = bz;
+            threadIdx.x = t; blockDim = b; gridDim = g;
+            k(args...);
+          });
+        for (auto& x : th) x.join();
+        pthread_barrier_destroy(&g_bar);
+      }
 }
 template <class K, class... A>
 void emu_launch(K k, dim3 g, dim3 b, A... args) {

@@ -1,8 +1,12 @@
 """Device code of the library run on the host (tests/emu): the MC kernels
 with the lanes of each block as threads, event by event against the oracle
 (with the default hard-event pool and with a 4-record pool that forces the
-inline fallback), and the wave-cooperative hard-event evaluation in lockstep
-against the inline evaluation, plain and under AddressSanitizer/UBSan."""
+inline fallback), the wave-cooperative hard-event evaluation in lockstep
+against the inline evaluation, plain and under AddressSanitizer/UBSan, and the
+GPU parity modules of the tetrahedral walk, the DEM ray exit, the direct CSDA
+flux and the legacy samplers run unchanged against the host build (their
+kernels reach every ballot and barrier with the whole block, which is what
+the thread emulation models)."""
 import os
 import shutil
 import subprocess
@@ -53,3 +57,18 @@ def test_coop_hard_event_lockstep(tmp_path, sanitize):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert " 0 failures" in r.stdout
+
+
+@needs_gpp
+def test_gpu_parity_modules_on_host_build(tmp_path):
+    import build_emu
+    lib = build_emu.build(str(tmp_path / "lib"))
+    env = dict(os.environ)
+    env["DIFFPUMAS_HIP_LIB"] = lib
+    mods = ["test_gpu_tetra.py", "test_gpu_dem.py", "test_gpu_direct.py",
+            "test_gpu_legacy_samplers.py"]
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "gpu", "-p", "no:cacheprovider",
+                        *[os.path.join(ROOT, "tests", m) for m in mods]],
+                       capture_output=True, text=True, timeout=900, env=env, cwd=ROOT)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
