@@ -113,6 +113,26 @@ __device__ __forceinline__ double inv_step(const double* inv, const double* x, i
 // (not re-rounded through log).  Reciprocals that a quantity shares are
 // taken once; the decay factor is returned as its exponent so the walk
 // multiplies one exponential per node.
+// Reciprocal and reciprocal square root of a positive normal double: the
+// hardware estimate refined by one Newton step (the relative error squares:
+// about 1e-16 instead of a correctly rounded division; the adjoint's results
+// are compared at 1e-10).  Arguments here are stopping powers, Lorentz
+// factors, beta^2 >= 1e-12 and products of stopping-power ratios.
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x * r;
+  return fma(r, fma(-h, r, 0.5), r);
+}
+// The same for the FP32 stepping map (compared at 1e-5).
+__device__ __forceinline__ float rcpf_nr(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.f), r);
+}
+
 __device__ __forceinline__ StepOut csda_segment_g(const Tab64& T, int K, int pow2, double E,
                                                   double lE, double gram, double path,
                                                   double inv_mass, double ctau, double inv_ctau,
@@ -131,11 +151,11 @@ __device__ __forceinline__ StepOut csda_segment_g(const Tab64& T, int K, int pow
       const double d0 = sR[i + 1] - sR[i], d1 = sS[i + 1] - sS[i];
       en.y0 = sR[i] + t * d0;
       en.y1 = sS[i] + t * d1;
-      const double re = rdl / E;
+      const double re = rdl * rcp_nr(E);
       en.dy0 = d0 * re;
       en.dy1 = d1 * re;
     }
-    en.ry1 = 1.0 / en.y1;
+    en.ry1 = rcp_nr(en.y1);
   }
   const double y0 = en.y0, dy0 = en.dy0, dy1 = en.dy1, ry1 = en.ry1;
   const double Xf = y0 + gram;
@@ -169,20 +189,20 @@ __device__ __forceinline__ StepOut csda_segment_g(const Tab64& T, int K, int pow
     const double d0 = sR[i + 1] - sR[i], d1 = sS[i + 1] - sS[i];
     en.y0 = sR[i] + t * d0;
     en.y1 = sS[i] + t * d1;
-    const double re = rdl / Eo;
+    const double re = rdl * rcp_nr(Eo);
     en.dy0 = d0 * re;
     en.dy1 = d1 * re;
   }
-  const double rso = 1.0 / en.y1;
+  const double rso = rcp_nr(en.y1);
   en.ry1 = rso;
   en.valid = true;
   // decay_weight with its log-derivative (compute_decay_weight_from_path,
   // src/Geometry.jl:1083-1107): beta^2 = 1 - 1/gamma^2 floored at 1e-12
-  const double rg = 1.0 / (1.0 + 0.5 * (E + Eo) * inv_mass);   // 1/gamma
+  const double rg = rcp_nr(1.0 + 0.5 * (E + Eo) * inv_mass);   // 1/gamma
   const double b2 = 1.0 - rg * rg;
   double rbeta, dbg_dg;   // 1/beta, d(beta*gamma)/d gamma
   if (b2 >= 1e-12) {
-    rbeta = 1.0 / sqrt(b2);
+    rbeta = rsqrt_nr(b2);
     dbg_dg = rbeta;
   } else {
     rbeta = 1e6;
@@ -226,7 +246,7 @@ __device__ __forceinline__ StepOutT<float> csda_segment_g(const Tab32& T, int K,
     const float d1 = T.S[i + 1] - T.S[i];
     y0 = T.R[i] + t * d0;
     y1 = T.S[i] + (float)t * d1;
-    const float re = (float)rdl / E;
+    const float re = (float)rdl * rcpf_nr(E);
     dy0 = (float)d0 * re;
     dy1 = d1 * re;
   }
@@ -263,22 +283,22 @@ __device__ __forceinline__ StepOutT<float> csda_segment_g(const Tab32& T, int K,
     const float t = (float)((lEo - T.LE[i]) * rdl);
     const float dyy = T.S[i + 1] - T.S[i];
     so_y = T.S[i] + t * dyy;
-    so_dy = dyy * ((float)rdl / Eo);
+    so_dy = dyy * ((float)rdl * rcpf_nr(Eo));
   }
   const float g = 1.f + 0.5f * (E + Eo) * inv_mass;
-  const float b2 = 1.f - 1.f / (g * g);
+  const float b2 = 1.f - rcpf_nr(g * g);
   float beta, dbg_dg;
   if (b2 >= 1e-12f) {
     beta = sqrtf(b2);
-    dbg_dg = 1.f / beta;
+    dbg_dg = rcpf_nr(beta);
   } else {
     beta = 1e-6f;
     dbg_dg = beta;
   }
-  const float rL = 1.f / (beta * g * ctau);
+  const float rL = rcpf_nr(beta * g * ctau);
   o.dexp = (double)(path * rL);
   const float dlog = path * rL * rL * (ctau * dbg_dg * inv_mass);
-  const float ry1 = 1.f / y1, rso = 1.f / so_y;
+  const float ry1 = rcpf_nr(y1), rso = rcpf_nr(so_y);
   o.ratio = so_y * ry1;
   o.k = kdy;
   o.c = kdy * dy0;
@@ -454,7 +474,7 @@ __global__ __launch_bounds__(256, WAVES) void csda_cube_kernel(CsdaArgs a) {
             if (c != 0.0) {
               const double P1 = P * c;
               const double Q1 = Q + beta * P;
-              const double iP1 = 1.0 / P1;
+              const double iP1 = rcp_nr(P1);
               p[0] = F(rr * (e - Q1 * iP1));
               p[bd] = F(rr * iP1);
               P = P1;
