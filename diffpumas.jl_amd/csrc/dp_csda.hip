@@ -113,26 +113,6 @@ __device__ __forceinline__ double inv_step(const double* inv, const double* x, i
 // (not re-rounded through log).  Reciprocals that a quantity shares are
 // taken once; the decay factor is returned as its exponent so the walk
 // multiplies one exponential per node.
-// Reciprocal and reciprocal square root of a positive normal double: the
-// hardware estimate refined by one Newton step (the relative error squares:
-// about 1e-16 instead of a correctly rounded division; the adjoint's results
-// are compared at 1e-10).  Arguments here are stopping powers, Lorentz
-// factors, beta^2 >= 1e-12 and products of stopping-power ratios.
-__device__ __forceinline__ double rcp_nr(double x) {
-  const double r = __builtin_amdgcn_rcp(x);
-  return fma(r, fma(-x, r, 1.0), r);
-}
-__device__ __forceinline__ double rsqrt_nr(double x) {
-  const double r = __builtin_amdgcn_rsq(x);
-  const double h = 0.5 * x * r;
-  return fma(r, fma(-h, r, 0.5), r);
-}
-// The same for the FP32 stepping map (compared at 1e-5).
-__device__ __forceinline__ float rcpf_nr(float x) {
-  const float r = __builtin_amdgcn_rcpf(x);
-  return fmaf(r, fmaf(-x, r, 1.f), r);
-}
-
 __device__ __forceinline__ StepOut csda_segment_g(const Tab64& T, int K, int pow2, double E,
                                                   double lE, double gram, double path,
                                                   double inv_mass, double ctau, double inv_ctau,

@@ -335,6 +335,53 @@ __device__ __forceinline__ D2 pow(D2 x, double p) {
   double d = p * ::pow(x.v, p - 1.0);
   return {::pow(x.v, p), d * x.a, d * x.b};
 }
+// Reciprocal and reciprocal square root of a positive normal double: the
+// hardware estimate (about 5e-8 relative on gfx950) refined by two Newton
+// steps, in 5 operations instead of the 10 of a correctly rounded quotient.
+// Measured over 2^20 values spread over 26 decades: the reciprocal equal to
+// 1.0 / x, the reciprocal square root within 2.6e-16 of 1.0 / sqrt(x) (one
+// Newton step leaves 2.2e-15 and 4.1e-15).  A quotient a * rcp_nr(b) is two
+// roundings instead of one.  For the deterministic CSDA kernels, compared with
+// the oracle at 1e-10; the MC kernels keep correctly rounded quotients.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  r = fma(r, fma(-hx * r, r, 0.5), r);
+  return fma(r, fma(-hx * r, r, 0.5), r);
+}
+// The same for the FP32 stepping map (compared at 1e-5).
+__device__ __forceinline__ float rcpf_nr(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.f), r);
+}
+
+// Quotients, logarithm and square root built on them, for values and duals.
+__device__ __forceinline__ double qdiv(double a, double b) { return a * rcp_nr(b); }
+__device__ __forceinline__ DualD qdiv(DualD a, DualD b) {
+  const double r = rcp_nr(b.v), q = a.v * r;
+  return {q, (a.d - q * b.d) * r};
+}
+__device__ __forceinline__ DualD qdiv(DualD a, double b) {
+  const double r = rcp_nr(b);
+  return {a.v * r, a.d * r};
+}
+__device__ __forceinline__ DualD qdiv(double a, DualD b) {
+  const double r = rcp_nr(b.v), q = a * r;
+  return {q, -q * b.d * r};
+}
+__device__ __forceinline__ double qlog(double x) { return ::log(x); }
+__device__ __forceinline__ DualD qlog(DualD a) { return {::log(a.v), a.d * rcp_nr(a.v)}; }
+__device__ __forceinline__ double qsqrt(double x) { return ::sqrt(x); }
+__device__ __forceinline__ DualD qsqrt(DualD a) {
+  const double s = ::sqrt(a.v);
+  return {s, a.d * (0.5 * rcp_nr(s))};
+}
+
 using ::log;
 using ::exp;
 using ::sqrt;

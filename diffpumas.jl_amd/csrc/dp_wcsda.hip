@@ -209,6 +209,27 @@ template <class R>
 __device__ __forceinline__ R air_stop(const WTab& T, R e, int& g) {
   return interp_rg(T.aE, T.aLE, T.aS, T.K, T.pow2, e, log(e), g);
 }
+// The atmosphere walk's lookups and decay with qdiv/qlog/qsqrt quotients
+// (refined hardware reciprocals): about 500 sub-steps per node, each with
+// some twenty quotients in the dual form.
+template <class R>
+__device__ __forceinline__ R air_stop_q(const WTab& T, R e, int& g) {
+  const R lx = qlog(e);
+  if (val(e) <= T.aE[0]) return R(T.aS[0]);
+  if (val(e) >= T.aE[T.K - 1]) return R(T.aS[T.K - 1]);
+  const int i = locate_idx(T.aLE, T.K, T.pow2, val(lx), g);
+  g = i;
+  R t = qdiv(lx - T.aLE[i], T.aLE[i + 1] - T.aLE[i]);
+  return T.aS[i] + t * (T.aS[i + 1] - T.aS[i]);
+}
+template <class R, class P>
+__device__ __forceinline__ R decay_exponent_q(const WTab& T, P path, R eavg) {
+  R g = 1.0 + qdiv(eavg, T.mass);
+  R b2 = 1.0 - qdiv(1.0, g * g);
+  R beta = qsqrt(val(b2) >= 1e-12 ? b2 : R(1e-12));
+  R L = beta * g * T.ctau;
+  return qdiv(path, L);
+}
 template <class R>
 __device__ __forceinline__ R rock_stop(const WTab& T, R e, int& g) {
   return interp_rg(T.rE, T.rLE, T.rS, T.K, T.pow2, e, log(e), g);
@@ -279,20 +300,20 @@ __device__ bool air_segment(const WTab& T, double z0, double cos_t, double dista
                             R& weight, int& g) {
   R remaining = R(distance);
   R traversed = R(0.0);
-  R dedx0 = air_stop(T, energy, g);
+  R dedx0 = air_stop_q(T, energy, g);
   // the sub-steps' decay exponents are summed and applied once at the end
   // (one exponential per walk instead of one per sub-step)
   R xdecay = R(0.0);
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
-    R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
+    R grel = qdiv(MAX_REL_GAIN * cmax(energy, 1e-6), dedx0);
     const R cap = cmin(remaining, MAX_STEP_M);
     R alt0 = z0 + cos_t * (traversed + 0.5 * cap);
-    R dens0 = 1.205 * exp(-alt0 / 12e3);
+    R dens0 = 1.205 * exp(alt0 * (-1.0 / 12e3));
     // ds = cmax(cmin(cap, grel / dens0), 1e-3); where that is cap itself (the
     // usual case: the 60 m cap binds), the mid-step altitude below is alt0
     // and its density dens0, so the exponential is not evaluated again
-    const R lim = grel / cmax(dens0, 1e-12);
+    const R lim = qdiv(grel, cmax(dens0, 1e-12));
     const bool capped = val(cap) <= val(lim);
     R ds = capped ? cap : lim;
     const bool floored = !(val(ds) >= 1e-3);
@@ -300,16 +321,16 @@ __device__ bool air_segment(const WTab& T, double z0, double cos_t, double dista
     R density = dens0;
     if (!capped || floored) {
       R alt = z0 + cos_t * (traversed + 0.5 * ds);
-      density = 1.205 * exp(-alt / 12e3);
+      density = 1.205 * exp(alt * (-1.0 / 12e3));
     }
     R gram = ds * density;
     R epred = energy + dedx0 * gram;
-    R dedx1 = air_stop(T, epred, g);
+    R dedx1 = air_stop_q(T, epred, g);
     R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
     if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
-    R dnext = air_stop(T, enext, g);
-    weight = weight * (dnext / cmax(dedx0, 1e-30));
-    xdecay = xdecay + decay_exponent(T, ds, 0.5 * (energy + enext));
+    R dnext = air_stop_q(T, enext, g);
+    weight = weight * qdiv(dnext, cmax(dedx0, 1e-30));
+    xdecay = xdecay + decay_exponent_q(T, ds, 0.5 * (energy + enext));
     energy = enext;
     traversed = traversed + ds;
     remaining = remaining - ds;

@@ -94,6 +94,11 @@ inline double __hiloint2double(int hi, int lo) {
   return x;
 }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+// hardware reciprocal estimates: the host gives correctly rounded values (the
+// device's refined estimates agree to about 1e-16)
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+inline double __builtin_amdgcn_rsq(double x) { return 1.0 / sqrt(x); }
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 
 // ------------------------------------------------------ lane intrinsics
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
