@@ -376,6 +376,26 @@ __device__ __forceinline__ DualD qdiv(double a, DualD b) {
 }
 __device__ __forceinline__ double qlog(double x) { return ::log(x); }
 __device__ __forceinline__ DualD qlog(DualD a) { return {::log(a.v), a.d * rcp_nr(a.v)}; }
+__device__ __forceinline__ D2 qdiv(D2 x, D2 y) {
+  const double r = rcp_nr(y.v), q = x.v * r;
+  return {q, (x.a - q * y.a) * r, (x.b - q * y.b) * r};
+}
+__device__ __forceinline__ D2 qdiv(D2 x, double y) {
+  const double r = rcp_nr(y);
+  return {x.v * r, x.a * r, x.b * r};
+}
+__device__ __forceinline__ D2 qdiv(double x, D2 y) {
+  const double r = rcp_nr(y.v), q = x * r;
+  return {q, -q * y.a * r, -q * y.b * r};
+}
+__device__ __forceinline__ D2 qlog(D2 x) {
+  const double r = rcp_nr(x.v);
+  return {::log(x.v), x.a * r, x.b * r};
+}
+__device__ __forceinline__ D2 qsqrt(D2 x) {
+  const double s = ::sqrt(x.v), h = 0.5 * rcp_nr(s);
+  return {s, x.a * h, x.b * h};
+}
 __device__ __forceinline__ double qsqrt(double x) { return ::sqrt(x); }
 __device__ __forceinline__ DualD qsqrt(DualD a) {
   const double s = ::sqrt(a.v);

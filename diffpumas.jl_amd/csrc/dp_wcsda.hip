@@ -72,7 +72,7 @@ __device__ __forceinline__ R lerp_at(const double* ys, int i, R t) {
 template <class R>
 __device__ __forceinline__ R cell_stop(const WTab& T, R w, bool porous, R e, int& g) {
   const int K = T.K;
-  R le = log(e);
+  R le = qlog(e);
   R sr, sw, sp;
   if (T.shared) {
     if (val(e) <= T.rE[0]) {
@@ -82,7 +82,7 @@ __device__ __forceinline__ R cell_stop(const WTab& T, R w, bool porous, R e, int
     } else {
       const int i = locate_idx(T.rLE, K, T.pow2, val(le), g);
       g = i;
-      R t = (le - T.rLE[i]) / (T.rLE[i + 1] - T.rLE[i]);
+      R t = qdiv(le - T.rLE[i], T.rLE[i + 1] - T.rLE[i]);
       sr = lerp_at(T.rS, i, t);
       sw = lerp_at(T.wS, i, t);
       sp = porous ? lerp_at(T.pS, i, t) : R(0.0);
@@ -99,8 +99,9 @@ __device__ __forceinline__ R cell_stop(const WTab& T, R w, bool porous, R e, int
 // cell_density, src/Tomography.jl:295-302 (harmonic in the mass fraction).
 template <class R>
 __device__ __forceinline__ R cell_density(const WMat& m, R w, bool porous) {
-  if (porous) return 1.0 / (0.5 * (1.0 - w) / m.rho_p + 0.5 * (1.0 - w) / m.rho_r + w / m.rho_w);
-  return 1.0 / ((1.0 - w) / m.rho_r + w / m.rho_w);
+  if (porous)
+    return qdiv(1.0, qdiv(0.5 * (1.0 - w), m.rho_p) + qdiv(0.5 * (1.0 - w), m.rho_r) + qdiv(w, m.rho_w));
+  return qdiv(1.0, qdiv(1.0 - w, m.rho_r) + qdiv(w, m.rho_w));
 }
 
 // cell_straggling and cell_hardloss (src/Tomography.jl:329-352) at one
@@ -234,6 +235,16 @@ template <class R>
 __device__ __forceinline__ R rock_stop(const WTab& T, R e, int& g) {
   return interp_rg(T.rE, T.rLE, T.rS, T.K, T.pow2, e, log(e), g);
 }
+template <class R>
+__device__ __forceinline__ R rock_stop_q(const WTab& T, R e, int& g) {
+  const R lx = qlog(e);
+  if (val(e) <= T.rE[0]) return R(T.rS[0]);
+  if (val(e) >= T.rE[T.K - 1]) return R(T.rS[T.K - 1]);
+  const int i = locate_idx(T.rLE, T.K, T.pow2, val(lx), g);
+  g = i;
+  R t = qdiv(lx - T.rLE[i], T.rLE[i + 1] - T.rLE[i]);
+  return T.rS[i] + t * (T.rS[i + 1] - T.rS[i]);
+}
 
 // propagate_csda_cell_segment, src/Tomography.jl:444-473.  The stopping power
 // at the end of a sub-step is the next sub-step's starting value (same
@@ -246,9 +257,9 @@ __device__ bool cell_segment(const WTab& T, const WMat& m, R w, bool porous, dou
   R dedx0 = cell_stop(T, w, porous, energy, g);
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
-    R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
+    R grel = qdiv(MAX_REL_GAIN * cmax(energy, 1e-6), dedx0);
     R gstep = cmin(cmin(remaining * density, cmax(grel, 1e-6)), MAX_STEP_M * density);
-    R ds = cmin(remaining, gstep / cmax(density, 1e-12));
+    R ds = cmin(remaining, qdiv(gstep, cmax(density, 1e-12)));
     if (val(ds) <= 0.0) { weight = R(0.0); return false; }
     R gram = ds * density;
     R epred = energy + dedx0 * gram;
@@ -256,8 +267,8 @@ __device__ bool cell_segment(const WTab& T, const WMat& m, R w, bool porous, dou
     R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
     if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
     R dnext = cell_stop(T, w, porous, enext, g);
-    weight = weight * (dnext / cmax(dedx0, 1e-30));
-    weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
+    weight = weight * qdiv(dnext, cmax(dedx0, 1e-30));
+    weight = weight * exp(-decay_exponent_q(T, ds, 0.5 * (energy + enext)));
     energy = enext;
     remaining = remaining - ds;
     dedx0 = dnext;
@@ -270,21 +281,21 @@ template <class R>
 __device__ bool rock_segment(const WTab& T, double density, double distance, R& energy, R& weight,
                              int& g) {
   R remaining = R(distance);
-  R dedx0 = rock_stop(T, energy, g);
+  R dedx0 = rock_stop_q(T, energy, g);
   while (val(remaining) > 1e-8) {
     if (!isfinite(val(dedx0)) || val(dedx0) <= 0.0) { weight = R(0.0); return false; }
-    R grel = MAX_REL_GAIN * cmax(energy, 1e-6) / dedx0;
+    R grel = qdiv(MAX_REL_GAIN * cmax(energy, 1e-6), dedx0);
     R gstep = cmin(cmin(remaining * density, cmax(grel, 1e-6)), R(MAX_STEP_M * density));
-    R ds = cmin(remaining, gstep / fmax(density, 1e-12));
+    R ds = cmin(remaining, qdiv(gstep, fmax(density, 1e-12)));
     if (val(ds) <= 0.0) { weight = R(0.0); return false; }
     R gram = ds * density;
     R epred = energy + dedx0 * gram;
-    R dedx1 = rock_stop(T, epred, g);
+    R dedx1 = rock_stop_q(T, epred, g);
     R enext = energy + 0.5 * (dedx0 + dedx1) * gram;
     if (!isfinite(val(enext)) || val(enext) >= 1e12) { energy = enext; weight = R(0.0); return false; }
-    R dnext = rock_stop(T, enext, g);
-    weight = weight * (dnext / cmax(dedx0, 1e-30));
-    weight = weight * decay_r(T, ds, 0.5 * (energy + enext));
+    R dnext = rock_stop_q(T, enext, g);
+    weight = weight * qdiv(dnext, cmax(dedx0, 1e-30));
+    weight = weight * exp(-decay_exponent_q(T, ds, 0.5 * (energy + enext)));
     energy = enext;
     remaining = remaining - ds;
     dedx0 = dnext;
