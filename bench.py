@@ -55,10 +55,10 @@ HBM_PEAK_GBS = 8000.0
 # runs summarised in profiles/; `traffic` = this x the units of one launch.
 MEASURED_BYTES_PER_UNIT = {
     "mc_flat_stochastic_kernel": 269087.0,    # per event
-    "csda_cube_kernel": 8909.2,               # per ray-voxel (all 256 nodes)
-    "csda_cube_kernel_fp32": 4691.0,          # per ray-voxel (FP32 stepping)
-    "wcsda_kernel": 2127.0,                   # per ray-voxel (all 32 nodes)
-    "wcsda_kernel_corrected": 4314.0,         # per ray-voxel (all 32 nodes)
+    "csda_cube_kernel": 8847.8,               # per ray-voxel (all 256 nodes)
+    "csda_cube_kernel_fp32": 4701.2,          # per ray-voxel (FP32 stepping)
+    "wcsda_kernel": 2125.4,                   # per ray-voxel (all 32 nodes)
+    "wcsda_kernel_corrected": 4312.8,         # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 253284.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
     "trace_cube_kernel": 13.0,                # per ray-voxel (fill launch)
@@ -77,12 +77,12 @@ VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 # `valu` object prices the live kernel time against VALU_PEAK_WAVE_INSTS.
 MEASURED_VALU_PER_UNIT = {
     "mc_flat_stochastic_kernel": 26531.5,     # per event
-    "csda_cube_kernel": 1166.0,               # per ray-voxel (all 256 nodes)
-    "csda_cube_kernel_fp32": 1309.8,          # per ray-voxel (FP32 stepping)
-    "wcsda_kernel": 10120.0,                  # per ray-voxel (all 32 nodes)
-    "wcsda_kernel_corrected": 10495.0,        # per ray-voxel (all 32 nodes)
+    "csda_cube_kernel": 938.5,                # per ray-voxel (all 256 nodes)
+    "csda_cube_kernel_fp32": 1050.9,          # per ray-voxel (FP32 stepping)
+    "wcsda_kernel": 6701.4,                   # per ray-voxel (all 32 nodes)
+    "wcsda_kernel_corrected": 7010.1,         # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 13906.2,                # per event
-    "csda_direct_kernel": 39.99,              # per sample
+    "csda_direct_kernel": 36.02,              # per sample
     "trace_cube_kernel": 5.19,                # per ray-voxel (fill launch)
     "ray_exit_kernel": 951.5,                 # per ray
     "tet_trace_kernel": 24.22,                # per ray-tet (fill launch)
