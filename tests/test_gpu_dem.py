@@ -1,7 +1,8 @@
 """GPU parity of the topography ray exit (dp_ray_surface_exit,
 ray_surface_exit_distance over map_elevation) against the CPU oracle:
 distances, flags and exit heights bit-exact on a rough synthetic terrain,
-with rays that cross, rays that leave the map and rays pointing down.
+with rays that cross, rays that leave the map, rays pointing down, rays
+that run out of march steps, and ray counts that do not fill a wave.
 """
 import math
 
@@ -47,6 +48,31 @@ def test_ray_exit_bit_exact(ctx):
     assert np.array_equal(np.isnan(dist), np.isnan(od))
     assert np.array_equal(dist[found], od[of]) and np.array_equal(ez[found], oz[of])
     assert 0.5 < found.mean() < 0.99      # crossings, map exits and downward rays all occur
+
+
+@pytest.mark.parametrize("n,step,max_d", [(1, 40.0, 25_000.0), (77, 40.0, 25_000.0),
+                                           (301, 13.0, 900.0), (130, 40.0, 85.0)])
+def test_ray_exit_ragged_and_short_march(ctx, n, step, max_d):
+    """Ray counts that do not fill a wave, other step lengths, and march
+    limits short enough that rays run out of steps before crossing."""
+    emap = _terrain(4)
+    dirs = _dirs(n, 6 + n)
+    dist, found, ez = dp.ray_surface_exit_distance(emap, dirs, 963.0, step_m=step,
+                                                   max_distance_m=max_d, ctx=ctx)
+    od, of, oz = O.ray_surface_exit(dict(nx=emap.nx, ny=emap.ny, x0=emap.x0, y0=emap.y0,
+                                         dx=emap.dx, dy=emap.dy, data=emap.data), dirs,
+                                    z_ref=963.0, step=step, max_distance=max_d)
+    assert np.array_equal(found, of)
+    assert np.array_equal(np.isnan(dist), np.isnan(od))
+    assert np.array_equal(dist[found], od[of]) and np.array_equal(ez[found], oz[of])
+    if max_d < 100.0:
+        assert found.mean() < 0.5         # most rays stop at the march limit
+
+
+def test_ray_exit_empty(ctx):
+    emap = _terrain(4)
+    dist, found, ez = dp.ray_surface_exit_distance(emap, np.zeros((0, 3)), 963.0, ctx=ctx)
+    assert dist.shape == (0,) and found.shape == (0,) and ez.shape == (0,)
 
 
 def test_bad_map(ctx):
