@@ -32,6 +32,10 @@ constexpr int MC_THREADS = 64;
 #define DP_MC_POOL 192  // a smaller pool (host emulation builds) exercises the inline fallback
 #endif
 constexpr int MC_POOL = DP_MC_POOL;
+#ifndef DP_MC_WAVES
+#define DP_MC_WAVES 4  // waves per SIMD the register budget is sized for (128 VGPRs)
+#endif
+constexpr int MC_WAVES = DP_MC_WAVES;
 constexpr int MC_BUCKETS = 256;  // source-energy buckets of the event order
 
 struct FlatGeo {
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(256) void bucket_scatter(uint64_t seed, int64_t n, 
 // deferrable hard-event weights of a round go to the block's pool and are
 // evaluated by all its lanes once every lane has finished its event.
 template <bool FAST>
-__global__ __launch_bounds__(MC_THREADS, 4) void mc_flat_stochastic_kernel(StochArgs a) {
+__global__ __launch_bounds__(MC_THREADS, MC_WAVES) void mc_flat_stochastic_kernel(StochArgs a) {
   __shared__ PoolRec pool[FAST ? MC_POOL : 1];
   __shared__ int pool_n;
   __shared__ CoopBuf coop;

@@ -1,14 +1,40 @@
-// Ray -> voxel traversal (one ray per lane), bit-exact with the reference
-// tracers.  Built with -ffp-contract=off: every product and sum is rounded as
-// in the reference, IEEE division and floor are exact on gfx950.
+// Ray -> voxel traversal, bit-exact with the reference tracers.  Built with
+// -ffp-contract=off: every product and sum is rounded as in the reference,
+// IEEE division and floor are exact on gfx950.
 //
 // Two launches of the same kernel: the first writes per-ray segment counts,
 // the second (after the caller's exclusive scan into row_ptr) writes the CSR
-// segment arrays.  The write stream (12 B per ray·voxel) bounds the kernel.
+// segment arrays (12 B per ray-voxel).
+//
+// Each ray is a quad of lanes: lanes 0, 1, 2 own the x, y and z axis (the
+// coordinate, the direction component, the cell index along the axis and the
+// distance to the axis' next face); lane 3 shadows lane 0.  A step's per-axis
+// work (one division for the cell index, one for the face distance, or one
+// edge search) runs on the three lanes at once, and DPP quad broadcasts hand
+// the three indices and distances to every lane of the quad, which then picks
+// the nearest face in the reference's x, y, z order.  A wave holds 16 rays, so
+// 16 384 rays fill 1024 waves (one per SIMD); with one ray per lane they were
+// 256 waves, one per CU, each issuing all six divisions of a step: 0.20 ms for
+// the fill pass of 16 384 rays through 128 x 128 x 64 voxels.  Every loop runs
+// over the whole wave (a finished ray's quad idles), so each broadcast reads
+// active lanes.
 #include "dp_internal.hpp"
 #include "dp_device.hpp"
 
 namespace {
+
+constexpr int TRACE_RAYS_PER_WAVE = 16;
+
+// value of lane Q of the caller's quad of lanes (DPP quad_perm broadcast)
+template <int Q>
+__device__ __forceinline__ int quad_bcast(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, Q * 0x55, 0xf, 0xf, false);
+}
+template <int Q>
+__device__ __forceinline__ double quad_bcast(double v) {
+  const int hi = quad_bcast<Q>(__double2hiint(v)), lo = quad_bcast<Q>(__double2loint(v));
+  return __hiloint2double(hi, lo);
+}
 
 struct CubeDev {
   double x_min, x_max, y_min, y_max, z_min, z_max;
@@ -16,21 +42,14 @@ struct CubeDev {
   double dx, dy, dz;
 };
 
-// get_cell_indices, examples/diff_flux_cube.jl:132-144 (1-based)
-__device__ __forceinline__ bool cube_cell(const CubeDev& g, double x, double y, double z, int& i,
-                                          int& j, int& k) {
-  if (x < g.x_min || x > g.x_max || y < g.y_min || y > g.y_max || z < g.z_min || z > g.z_max)
-    return false;
-  i = min(max((int)floor((x - g.x_min) / g.dx) + 1, 1), g.nx);
-  j = min(max((int)floor((y - g.y_min) / g.dy) + 1, 1), g.ny);
-  k = min(max((int)floor((z - g.z_min) / g.dz) + 1, 1), g.nz);
-  return true;
-}
-
 // trace_cube_path, examples/diff_flux_cube.jl:304-364, with
-// distance_to_boundary (:194-280) inlined.  The cell found at the top of an
-// iteration is the cell distance_to_boundary looks up again at the same
-// point, so it is computed once.
+// distance_to_boundary (:194-280) and get_cell_indices (:132-144) inlined.
+// The cell found at the top of an iteration is the cell
+// distance_to_boundary looks up again at the same point, so it is computed
+// once.  Per axis (lane): in-range test, 1-based index
+// clamp(floor((p - min) / d) + 1, 1, n), the face the direction points to and
+// its distance; the quad then takes the smallest distance > eps in x, y, z
+// order with strict comparisons, as the reference's three if-blocks do.
 __global__ __launch_bounds__(64) void trace_cube_kernel(CubeDev g, int n_rays,
                                                          const double* __restrict__ start_xy,
                                                          const double* __restrict__ dir,
@@ -39,54 +58,62 @@ __global__ __launch_bounds__(64) void trace_cube_kernel(CubeDev g, int n_rays,
                                                          int32_t* __restrict__ cells,
                                                          double* __restrict__ lens,
                                                          double* __restrict__ z_exit_out) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rays) return;
-  const double ux = dir[3 * r], uy = dir[3 * r + 1], uz = dir[3 * r + 2];
-  double px = start_xy[2 * r], py = start_xy[2 * r + 1], pz = 0.0, z_exit = 0.0;
+  const int lane = (int)threadIdx.x, q = lane & 3;
+  const int ax = q < 3 ? q : 0;
+  const int r = blockIdx.x * TRACE_RAYS_PER_WAVE + (lane >> 2);
+  const bool live = r < n_rays;
+  const double lo = ax == 0 ? g.x_min : ax == 1 ? g.y_min : g.z_min;
+  const double hi = ax == 0 ? g.x_max : ax == 1 ? g.y_max : g.z_max;
+  const double d = ax == 0 ? g.dx : ax == 1 ? g.dy : g.dz;
+  const int na = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+  const double u = live ? dir[3 * r + ax] : 0.0;
+  double p = live && ax < 2 ? start_xy[2 * r + ax] : 0.0;
+  double p_seg = 0.0;  // the coordinate after the last segment (z: the exit height)
   const bool write = cells != nullptr;
-  const int out = write ? row_ptr[r] : 0;
-  const int cap = write ? row_ptr[r + 1] - out : 0;
+  const int out = write && live ? row_ptr[r] : 0;
+  const int cap = write && live ? row_ptr[r + 1] - out : 0;
   int n = 0;
   const double eps = 1e-10;
-  for (int step = 0; step < 1000; ++step) {
-    int i, j, k;
-    if (!cube_cell(g, px, py, pz, i, j, k)) break;
-    const int cell = (i - 1) + (j - 1) * g.nx + (k - 1) * g.nx * g.ny;
-    const double x_lo = g.x_min + (i - 1) * g.dx, x_hi = g.x_min + i * g.dx;
-    const double y_lo = g.y_min + (j - 1) * g.dy, y_hi = g.y_min + j * g.dy;
-    const double z_lo = g.z_min + (k - 1) * g.dz, z_hi = g.z_min + k * g.dz;
+  bool done = !live;
+  for (int step = 0; step < 1000 && __ballot(!done) != 0; ++step) {
+    const bool inb = !(p < lo || p > hi);
+    const int i = min(max((int)floor((p - lo) / d) + 1, 1), na);
+    double t = __builtin_huge_val();
+    bool ex = false;
+    if (fabs(u) > eps) {
+      if (u > 0) { t = (lo + i * d - p) / u; ex = i + 1 > na; }
+      else       { t = (lo + (i - 1) * d - p) / u; ex = i - 1 < 1; }
+    }
+    const int code = (inb ? 1 : 0) | (ex ? 2 : 0);
+    const int c0 = quad_bcast<0>(code), c1 = quad_bcast<1>(code), c2 = quad_bcast<2>(code);
+    const int i0 = quad_bcast<0>(i), i1 = quad_bcast<1>(i), i2 = quad_bcast<2>(i);
+    const double t0 = quad_bcast<0>(t), t1 = quad_bcast<1>(t), t2 = quad_bcast<2>(t);
+    if (done) continue;
+    if (!(c0 & c1 & c2 & 1)) { done = true; continue; }
+    const int cell = (i0 - 1) + (i1 - 1) * g.nx + (i2 - 1) * g.nx * g.ny;
     double t_min = __builtin_huge_val();
     bool exiting = false;
-    if (fabs(ux) > eps) {
-      if (ux > 0) { double t = (x_hi - px) / ux; if (t > eps && t < t_min) { t_min = t; exiting = (i + 1 > g.nx); } }
-      else        { double t = (x_lo - px) / ux; if (t > eps && t < t_min) { t_min = t; exiting = (i - 1 < 1); } }
-    }
-    if (fabs(uy) > eps) {
-      if (uy > 0) { double t = (y_hi - py) / uy; if (t > eps && t < t_min) { t_min = t; exiting = (j + 1 > g.ny); } }
-      else        { double t = (y_lo - py) / uy; if (t > eps && t < t_min) { t_min = t; exiting = (j - 1 < 1); } }
-    }
-    if (fabs(uz) > eps) {
-      if (uz > 0) { double t = (z_hi - pz) / uz; if (t > eps && t < t_min) { t_min = t; exiting = (k + 1 > g.nz); } }
-      else        { double t = (z_lo - pz) / uz; if (t > eps && t < t_min) { t_min = t; exiting = (k - 1 < 1); } }
-    }
+    if (t0 > eps && t0 < t_min) { t_min = t0; exiting = c0 & 2; }
+    if (t1 > eps && t1 < t_min) { t_min = t1; exiting = c1 & 2; }
+    if (t2 > eps && t2 < t_min) { t_min = t2; exiting = c2 & 2; }
     const bool exits = exiting || t_min == __builtin_huge_val();
     const double dist = t_min;
     if (dist <= 1e-10 || !isfinite(dist)) {
-      px += ux * 0.01; py += uy * 0.01; pz += uz * 0.01;
+      p += u * 0.01;
       continue;
     }
-    if (write && n < cap) {
+    if (write && n < cap && q == 0) {
       cells[out + n] = cell;
       lens[out + n] = dist;
     }
     ++n;
-    px += ux * dist; py += uy * dist; pz += uz * dist;
-    z_exit = pz;
-    if (exits) break;
+    p += u * dist;
+    p_seg = p;
+    if (exits) done = true;
   }
-  if (!write) {
-    counts[r] = n;
-    if (z_exit_out) z_exit_out[r] = z_exit;
+  if (!write && live) {
+    if (q == 0) counts[r] = n;
+    if (q == 2 && z_exit_out) z_exit_out[r] = p_seg;
   }
 }
 
@@ -102,56 +129,65 @@ __device__ __forceinline__ int ssl(const double* __restrict__ e, int n, double x
 
 // trace_path(::VoxelVolume), examples/lvd_tomography.jl:738-779, with
 // voxel_cell_indices (:583-594) and distance_to_voxel_boundary (:596-627).
+// Per axis (lane): in-range test, the edge search, the face distance; the
+// quad takes the smallest distance > e (fmin, order-free) and every lane
+// keeps the same copy of the travelled length.
 __global__ __launch_bounds__(64) void trace_edges_kernel(
     const double* __restrict__ xe, int nxe, const double* __restrict__ ye, int nye,
     const double* __restrict__ ze, int nze, int n_rays, const double* __restrict__ origin,
     const double* __restrict__ dir, const double* __restrict__ surface,
     int32_t* __restrict__ counts, const int32_t* __restrict__ row_ptr, int32_t* __restrict__ cells,
     double* __restrict__ lens, double* __restrict__ traveled_out) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rays) return;
+  const int lane = (int)threadIdx.x, q = lane & 3;
+  const int ax = q < 3 ? q : 0;
+  const int r = blockIdx.x * TRACE_RAYS_PER_WAVE + (lane >> 2);
+  const bool live = r < n_rays;
   const double POINT_EPS = 1e-4;
-  const double ux = dir[3 * r], uy = dir[3 * r + 1], uz = dir[3 * r + 2];
-  double x = origin[3 * r] + ux * POINT_EPS;
-  double y = origin[3 * r + 1] + uy * POINT_EPS;
-  double z = origin[3 * r + 2] + uz * POINT_EPS;
-  const double sd = surface[r];
+  const double* E = ax == 0 ? xe : ax == 1 ? ye : ze;
+  const int ne = ax == 0 ? nxe : ax == 1 ? nye : nze;
+  const double u = live ? dir[3 * r + ax] : 0.0;
+  double p = live ? origin[3 * r + ax] + u * POINT_EPS : 0.0;
+  const double sd = live ? surface[r] : 0.0;
   double traveled = POINT_EPS;
   const int nx = nxe - 1, ny = nye - 1;
   const bool write = cells != nullptr;
-  const int out = write ? row_ptr[r] : 0;
-  const int cap = write ? row_ptr[r + 1] - out : 0;
+  const int out = write && live ? row_ptr[r] : 0;
+  const int cap = write && live ? row_ptr[r + 1] - out : 0;
   int n = 0;
   const double e = 1e-10;
-  while (traveled < sd - POINT_EPS) {
-    if (x < xe[0] || x > xe[nxe - 1] || y < ye[0] || y > ye[nye - 1] || z < ze[0] || z > ze[nze - 1])
-      break;
-    const int i = min(max(ssl(xe, nxe, x), 1), nxe - 1);
-    const int j = min(max(ssl(ye, nye, y), 1), nye - 1);
-    const int k = min(max(ssl(ze, nze, z), 1), nze - 1);
-    const int cell = (i - 1) + (j - 1) * nx + (k - 1) * nx * ny;
-    double t_min = __builtin_huge_val();
-    if (fabs(ux) > e) { double t = ux > 0 ? (xe[i] - x) / ux : (xe[i - 1] - x) / ux; if (t > e) t_min = fmin(t_min, t); }
-    if (fabs(uy) > e) { double t = uy > 0 ? (ye[j] - y) / uy : (ye[j - 1] - y) / uy; if (t > e) t_min = fmin(t_min, t); }
-    if (fabs(uz) > e) { double t = uz > 0 ? (ze[k] - z) / uz : (ze[k - 1] - z) / uz; if (t > e) t_min = fmin(t_min, t); }
+  bool done = !live || !(traveled < sd - POINT_EPS);
+  while (__ballot(!done) != 0) {
+    const bool inb = !(p < E[0] || p > E[ne - 1]);
+    const int i = min(max(ssl(E, ne, p), 1), ne - 1);
+    double t = __builtin_huge_val();
+    if (fabs(u) > e) {
+      const double tt = u > 0 ? (E[i] - p) / u : (E[i - 1] - p) / u;
+      if (tt > e) t = tt;
+    }
+    const int c0 = quad_bcast<0>((int)inb), c1 = quad_bcast<1>((int)inb), c2 = quad_bcast<2>((int)inb);
+    const int i0 = quad_bcast<0>(i), i1 = quad_bcast<1>(i), i2 = quad_bcast<2>(i);
+    const double t0 = quad_bcast<0>(t), t1 = quad_bcast<1>(t), t2 = quad_bcast<2>(t);
+    if (done) continue;
+    if (!(c0 & c1 & c2)) { done = true; continue; }
+    const int cell = (i0 - 1) + (i1 - 1) * nx + (i2 - 1) * nx * ny;
+    const double t_min = fmin(fmin(fmin(__builtin_huge_val(), t0), t1), t2);
     const double dist = isfinite(t_min) ? t_min : 0.0;
-    if (!isfinite(dist) || dist <= 0.0) break;
+    if (!isfinite(dist) || dist <= 0.0) { done = true; continue; }
     const double remaining = sd - traveled;
     const double step = fmin(dist, remaining);
     if (step > 1e-8) {
-      if (write && n < cap) {
+      if (write && n < cap && q == 0) {
         cells[out + n] = cell;
         lens[out + n] = step;
       }
       ++n;
       traveled += step;
     }
-    if (dist >= remaining - 1e-6) break;
-    x += ux * (step + POINT_EPS);
-    y += uy * (step + POINT_EPS);
-    z += uz * (step + POINT_EPS);
+    if (dist >= remaining - 1e-6) { done = true; continue; }
+    p += u * (step + POINT_EPS);
+    if (!(traveled < sd - POINT_EPS)) done = true;
   }
-  if (!write) {
+  if (!write && live && q == 0) {
     counts[r] = n;
     if (traveled_out) traveled_out[r] = traveled;
   }
@@ -163,8 +199,7 @@ int launch_cube(dp_ctx c, const dp_cube_grid* grid, int32_t n, const double* sxy
   CubeDev g{grid->x_min, grid->x_max, grid->y_min, grid->y_max, grid->z_min, grid->z_max,
             grid->nx, grid->ny, grid->nz, grid->dx, grid->dy, grid->dz};
   if (n == 0) return DP_OK;
-  // 64-lane blocks: 16 384 rays still give one wave to every CU
-  dim3 b(64), gr((n + 63) / 64);
+  dim3 b(64), gr((n + TRACE_RAYS_PER_WAVE - 1) / TRACE_RAYS_PER_WAVE);
   dph::record(c, c->ev0);
   hipLaunchKernelGGL(trace_cube_kernel, gr, b, 0, c->stream, g, n, sxy, dir, counts, row_ptr,
                      cells, lens, z_exit);
@@ -277,7 +312,7 @@ int dp_trace_edges(dp_ctx c, const double* xe, int32_t nxe, const double* ye, in
   DP_HIP(hipMemcpyAsync(b + o_sd, surface, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
   if (cells) DP_HIP(hipMemcpyAsync(b + o_rp, row_ptr, 4 * (size_t)(n + 1), hipMemcpyHostToDevice, c->stream));
   dph::record(c, c->ev0);
-  hipLaunchKernelGGL(trace_edges_kernel, dim3((n + 63) / 64), dim3(64), 0, c->stream,
+  hipLaunchKernelGGL(trace_edges_kernel, dim3((n + TRACE_RAYS_PER_WAVE - 1) / TRACE_RAYS_PER_WAVE), dim3(64), 0, c->stream,
                      (double*)(b + o_x), nxe, (double*)(b + o_y), nye, (double*)(b + o_z), nze, n,
                      (double*)(b + o_or), (double*)(b + o_dir), (double*)(b + o_sd),
                      cells ? nullptr : (int32_t*)(b + o_cnt), (int32_t*)(b + o_rp),

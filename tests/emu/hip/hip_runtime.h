@@ -77,6 +77,21 @@ inline int __builtin_amdgcn_readlane(int v, int k) {
   pthread_barrier_wait(&g_bar);
   return r;
 }
+// DPP with a quad_perm control (ctrl < 0x100): lane 4m + j takes src of lane
+// 4m + ((ctrl >> 2j) & 3).  Other DPP controls are not emulated.
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask,
+                                       bool bound_ctrl) {
+  (void)old; (void)row_mask; (void)bank_mask; (void)bound_ctrl;
+  static int xch[1024];
+  if (ctrl >= 0x100) abort();
+  pthread_barrier_wait(&g_bar);
+  xch[threadIdx.x] = src;
+  pthread_barrier_wait(&g_bar);
+  const unsigned j = threadIdx.x & 3u;
+  const int r = xch[(threadIdx.x & ~3u) + ((unsigned)(ctrl >> (2 * j)) & 3u)];
+  pthread_barrier_wait(&g_bar);
+  return r;
+}
 inline int __double2hiint(double x) {
   long long b;
   memcpy(&b, &x, 8);

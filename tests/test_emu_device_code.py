@@ -67,8 +67,12 @@ def test_gpu_parity_modules_on_host_build(tmp_path):
     env["DIFFPUMAS_HIP_LIB"] = lib
     mods = ["test_gpu_tetra.py", "test_gpu_dem.py", "test_gpu_direct.py",
             "test_gpu_legacy_samplers.py"]
+    # the lane-quad tracers, on their small cases (the 3000-ray cases take minutes here)
+    trace = [os.path.join(ROOT, "tests", "test_gpu_parity.py") + "::" + t
+             for t in ("test_trace_cube_edge_cases", "test_trace_cube_empty",
+                       "test_trace_edges_edge_cases")]
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "gpu", "-p", "no:cacheprovider",
-                        *[os.path.join(ROOT, "tests", m) for m in mods]],
+                        *[os.path.join(ROOT, "tests", m) for m in mods], *trace],
                        capture_output=True, text=True, timeout=900, env=env, cwd=ROOT)
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]

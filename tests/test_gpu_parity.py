@@ -82,6 +82,28 @@ def test_trace_edges_bit_exact(ctx):
     assert np.array_equal(tr.view(np.uint64), otr.view(np.uint64))
 
 
+def test_trace_edges_edge_cases(ctx):
+    """Origins outside the volume and on edge planes, axis-aligned and
+    vertical directions, surfaces shorter than the start offset, and a ray
+    count that leaves part of a wave empty."""
+    xe = np.array([-30.0, -10.0, 0.0, 15.0, 40.0])
+    ye = np.array([-20.0, 0.0, 5.0, 25.0])
+    ze = np.array([0.0, 10.0, 30.0, 45.0])
+    origins = np.array([[0.0, 0.0, 0.0], [-50.0, 0.0, 1.0], [0.0, 5.0, 10.0], [15.0, -20.0, 0.0],
+                        [1.0, 1.0, 1.0], [2.0, 2.0, 2.0], [0.0, 0.0, 44.0]])
+    dirs = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.6, 0.0, 0.8], [0.0, 0.6, 0.8],
+                     [0.48, 0.6, 0.64], [0.0, 0.0, 1.0], [0.0, 0.0, 1.0]])
+    surface = np.array([60.0, 100.0, 50.0, 80.0, 1e-4, 0.0, 500.0])
+    rp, cells, lens, tr, rem = dp.trace_edge_paths(xe, ye, ze, origins, dirs, surface, ctx=ctx)
+    orp, ocells, olens, otr = O.trace_edges(xe, ye, ze, origins, dirs, surface)
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(cells, ocells)
+    assert np.array_equal(lens.view(np.uint64), olens.view(np.uint64))
+    assert np.array_equal(tr.view(np.uint64), otr.view(np.uint64))
+    assert rp[2] - rp[1] == 0              # starts outside: no segment
+    assert rp[5] - rp[4] == 0 and rp[6] - rp[5] == 0   # surface inside the start offset
+
+
 # ------------------------------------------------------ per-voxel CSDA
 def _problem(physics, ctx, n_det=3, n_dir=24, n_nodes=40, seed=4, nx=12, ny=10, nz=8):
     geo = cube(physics, nx, ny, nz)
