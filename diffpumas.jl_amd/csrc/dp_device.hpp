@@ -257,6 +257,18 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Value of lane Q of the caller's quad of lanes (DPP quad_perm broadcast);
+// every lane of the quad must be active.
+template <int Q>
+__device__ __forceinline__ int quad_bcast(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, Q * 0x55, 0xf, 0xf, false);
+}
+template <int Q>
+__device__ __forceinline__ double quad_bcast(double v) {
+  const int hi = quad_bcast<Q>(__double2hiint(v)), lo = quad_bcast<Q>(__double2loint(v));
+  return __hiloint2double(hi, lo);
+}
+
 }  // namespace dpk
 
 namespace dpk {

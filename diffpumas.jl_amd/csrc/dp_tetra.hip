@@ -11,7 +11,8 @@
 //                      tolerance and the POINT_EPS nudge, holds the origin.
 //                      A tet outside it cannot contain the nudged point, so
 //                      the first hit is the full scan's.
-//   tet_trace_kernel   one ray per lane: nearest exit face (Moller-Trumbore,
+//   tet_trace_kernel   one ray per quad of lanes, one face per lane:
+//                      nearest exit face (Moller-Trumbore,
 //                      src/TriangleIntersect.jl:86-127, with the tolerant
 //                      plane hit of :484-501 as fallback), step to it, nudge
 //                      by POINT_EPS and continue in the neighbour across that
@@ -42,6 +43,8 @@ struct TetDev {
 };
 
 constexpr double TET_POINT_EPS = 1e-4;   // examples/lvd_tomography.jl:147
+constexpr int TET_RAYS_PER_WAVE = 16;
+using dpk::quad_bcast;
 
 __device__ __forceinline__ double tdot(const double* a, const double* b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
@@ -269,57 +272,98 @@ __global__ __launch_bounds__(64) void tet_locate_kernel(TetDev m, int n,
   }
 }
 
+// One ray per quad of lanes, 16 rays per wave: lane f tests face f of the
+// current tet (Moller-Trumbore, then the tolerant plane hit) and loads the
+// neighbour across it; DPP quad broadcasts give every lane of the quad the
+// four results, and the quad takes the nearest face in face order with strict
+// comparisons, as nearest_tetra_face_intersection does.  The hit point is
+// p + dist d, the expression both face tests use.  Every loop runs over the
+// whole wave (a finished ray's quad idles), so each broadcast reads active
+// lanes.
 __global__ __launch_bounds__(64) void tet_trace_kernel(
     TetDev m, int n, const double* __restrict__ origin, const double* __restrict__ dir,
     const double* __restrict__ surface, const int32_t* __restrict__ start,
     int32_t* __restrict__ counts, const int32_t* __restrict__ row_ptr,
     int32_t* __restrict__ cells, double* __restrict__ lens, double* __restrict__ traveled_out,
     int* __restrict__ flag) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+  const int lane = (int)threadIdx.x, f = lane & 3;
+  const int r = blockIdx.x * TET_RAYS_PER_WAVE + (lane >> 2);
+  const bool live = r < n;
   const bool write = cells != nullptr;
-  const int out = write ? row_ptr[r] : 0;
-  const int cap = write ? row_ptr[r + 1] - out : 0;
-  const double d[3] = {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
-  double cur[3] = {origin[3 * r] + d[0] * TET_POINT_EPS, origin[3 * r + 1] + d[1] * TET_POINT_EPS,
-                   origin[3 * r + 2] + d[2] * TET_POINT_EPS};
-  const double sd = surface[r];
-  int tet = start[r];
+  const int out = write && live ? row_ptr[r] : 0;
+  const int cap = write && live ? row_ptr[r + 1] - out : 0;
+  double d[3] = {0.0, 0.0, 0.0}, cur[3] = {0.0, 0.0, 0.0};
+  if (live) {
+    for (int i = 0; i < 3; ++i) d[i] = dir[3 * r + i];
+    for (int i = 0; i < 3; ++i) cur[i] = origin[3 * r + i] + d[i] * TET_POINT_EPS;
+  }
+  const double sd = live ? surface[r] : 0.0;
+  int tet = live ? start[r] : -1;
   double traveled = tet < 0 ? 0.0 : TET_POINT_EPS;
   int ns = 0;
   const long max_steps = 4L * m.nt + 16;
   long it = 0;
-  while (tet >= 0 && traveled < sd - TET_POINT_EPS) {
-    if (++it > max_steps) {
-      atomicOr(flag, 1);
-      break;
+  // face f: vertices (1,2,3), (1,2,4), (1,3,4), (2,3,4) of the tet, 1-based
+  const int k1 = f < 3 ? 0 : 1, k2 = f < 2 ? 1 : 2, k3 = f == 0 ? 2 : 3;
+  bool done = !(tet >= 0 && traveled < sd - TET_POINT_EPS);
+  while (__ballot(!done) != 0) {
+    bool ok = false;
+    double dist = 0.0;
+    int nb = -1;
+    if (!done && ++it > max_steps) {
+      if (f == 0) atomicOr(flag, 1);
+      done = true;
     }
-    double dist, ip[3];
-    const int f = nearest_face(m, tet, cur, d, dist, ip);
-    if (f < 0 || !isfinite(dist)) break;
+    if (!done) {
+      const int32_t* v = m.tets + 4 * (size_t)tet;
+      const double* p1 = m.pts + 3 * (size_t)v[k1];
+      const double* p2 = m.pts + 3 * (size_t)v[k2];
+      const double* p3 = m.pts + 3 * (size_t)v[k3];
+      nb = m.nbr[4 * (size_t)tet + f];
+      double ip[3];
+      ok = moller_trumbore(cur, d, p1, p2, p3, dist, ip);
+      if (!ok) ok = tolerant_hit(cur, d, p1, p2, p3, dist, ip);
+      ok = ok && dist > 1e-8;
+    }
+    const int o0 = quad_bcast<0>((int)ok), o1 = quad_bcast<1>((int)ok),
+              o2 = quad_bcast<2>((int)ok), o3 = quad_bcast<3>((int)ok);
+    const double t0 = quad_bcast<0>(dist), t1 = quad_bcast<1>(dist), t2 = quad_bcast<2>(dist),
+                 t3 = quad_bcast<3>(dist);
+    const int n0 = quad_bcast<0>(nb), n1 = quad_bcast<1>(nb), n2 = quad_bcast<2>(nb),
+              n3 = quad_bcast<3>(nb);
+    if (done) continue;
+    int bf = -1;
+    double best = __builtin_huge_val();
+    if (o0 && t0 < best) { bf = 0; best = t0; }
+    if (o1 && t1 < best) { bf = 1; best = t1; }
+    if (o2 && t2 < best) { bf = 2; best = t2; }
+    if (o3 && t3 < best) { bf = 3; best = t3; }
+    if (bf < 0 || !isfinite(best)) { done = true; continue; }
     const double remaining = sd - traveled;
-    const double step = fmin(dist, remaining);
+    const double step = fmin(best, remaining);
     if (step > 1e-8) {
-      if (write && ns < cap) {
+      if (write && ns < cap && f == 0) {
         cells[out + ns] = tet;
         lens[out + ns] = step;
       }
       ++ns;
       traveled += step;
     }
-    if (dist >= remaining - 1e-6) break;
-    const int nt = m.nbr[4 * (size_t)tet + f];
+    if (best >= remaining - 1e-6) { done = true; continue; }
+    const int nt = bf == 0 ? n0 : bf == 1 ? n1 : bf == 2 ? n2 : n3;
     if (nt < 0) {
       tet = -1;
+      done = true;
     } else {
-      cur[0] = ip[0] + d[0] * TET_POINT_EPS;
-      cur[1] = ip[1] + d[1] * TET_POINT_EPS;
-      cur[2] = ip[2] + d[2] * TET_POINT_EPS;
+      for (int i = 0; i < 3; ++i) cur[i] = (cur[i] + best * d[i]) + d[i] * TET_POINT_EPS;
       tet = nt;
+      if (!(traveled < sd - TET_POINT_EPS)) done = true;
     }
   }
-  if (!write) counts[r] = ns;
-  if (traveled_out) traveled_out[r] = traveled;
+  if (live && f == 0) {
+    if (!write) counts[r] = ns;
+    if (traveled_out) traveled_out[r] = traveled;
+  }
 }
 
 int check_mesh(const char* fn, const dp_tet_mesh* m) {
@@ -493,7 +537,7 @@ int dp_trace_tetra(dp_ctx c, const dp_tet_mesh* mesh, int32_t n, const double* o
   hipLaunchKernelGGL(tet_locate_kernel, dim3(loc_grid), dim3(64), 0, st, m, n, d_or, d_dir,
                      n_groups > 0 ? (const int32_t*)(b + o_grp) : nullptr,
                      (const int32_t*)(b + o_gptr), (const int32_t*)(b + o_cand), d_st);
-  hipLaunchKernelGGL(tet_trace_kernel, dim3((n + 63) / 64), dim3(64), 0, st, m, n, d_or, d_dir,
+  hipLaunchKernelGGL(tet_trace_kernel, dim3((n + TET_RAYS_PER_WAVE - 1) / TET_RAYS_PER_WAVE), dim3(64), 0, st, m, n, d_or, d_dir,
                      (const double*)(b + o_sd), (const int32_t*)d_st,
                      cells ? nullptr : (int32_t*)(b + o_cnt), (const int32_t*)(b + o_rp),
                      cells ? (int32_t*)(b + o_cell) : nullptr, (double*)(b + o_len),

@@ -25,16 +25,7 @@ namespace {
 
 constexpr int TRACE_RAYS_PER_WAVE = 16;
 
-// value of lane Q of the caller's quad of lanes (DPP quad_perm broadcast)
-template <int Q>
-__device__ __forceinline__ int quad_bcast(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, Q * 0x55, 0xf, 0xf, false);
-}
-template <int Q>
-__device__ __forceinline__ double quad_bcast(double v) {
-  const int hi = quad_bcast<Q>(__double2hiint(v)), lo = quad_bcast<Q>(__double2loint(v));
-  return __hiloint2double(hi, lo);
-}
+using dpk::quad_bcast;
 
 struct CubeDev {
   double x_min, x_max, y_min, y_max, z_min, z_max;
