@@ -61,10 +61,10 @@ MEASURED_BYTES_PER_UNIT = {
     "wcsda_kernel_corrected": 4312.8,         # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 253284.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
-    "trace_cube_kernel": 13.0,                # per ray-voxel (fill launch)
+    "trace_cube_kernel": 13.3,                # per ray-voxel (fill launch)
     "ne_normal_kernel": 11.0,                 # per entry of the dense normal matrix
-    "ray_exit_kernel": 112.0,                 # per ray
-    "tet_trace_kernel": 62.0,                 # per ray-tet (fill launch)
+    "ray_exit_kernel": 137.8,                 # per ray
+    "tet_trace_kernel": 63.2,                 # per ray-tet (fill launch)
 }
 FP64_PEAK_TFLOPS = 78.6
 # VALU issue ceiling of an MI355X: 256 CUs x 4 SIMDs, 16 lanes each, so one
@@ -83,9 +83,9 @@ MEASURED_VALU_PER_UNIT = {
     "wcsda_kernel_corrected": 7010.1,         # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 13906.2,                # per event
     "csda_direct_kernel": 36.02,              # per sample
-    "trace_cube_kernel": 5.19,                # per ray-voxel (fill launch)
-    "ray_exit_kernel": 951.5,                 # per ray
-    "tet_trace_kernel": 24.22,                # per ray-tet (fill launch)
+    "trace_cube_kernel": 11.97,               # per ray-voxel (fill launch)
+    "ray_exit_kernel": 318.4,                 # per ray
+    "tet_trace_kernel": 30.35,                # per ray-tet (fill launch)
 }
 
 
