@@ -901,8 +901,10 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
     else DP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_flat_stochastic_kernel<false>, MC_THREADS, 0));
     if (occ < 1) occ = 1;
     // Blocks take 64-event rounds grid-stride; the tuned knob k0 is the
-    // number of blocks per resident block slot (default 16, the fastest
-    // measured at 1e7 events: 1063 ms against 1092 ms at 4 and 1121 ms at 1).
+    // number of blocks per resident block slot, capped at one block per round
+    // (default 64, which at 1e7 events is that cap: every block takes one
+    // round and the hardware hands out blocks as slots free up; measured at
+    // 1e7 events 858 ms, against 860 ms at 32, 866 ms at 16 and 878 ms at 4).
     const int64_t want = (n + MC_THREADS - 1) / MC_THREADS;
     auto launch = [&](const dph::TuneCfg& cfg) -> int {
       int64_t grid = (int64_t)c->n_cu * occ * std::max(1, cfg.k0);
@@ -912,7 +914,7 @@ int mc_run(dp_ctx c, dp_tables t, const dp_flat_geometry* geo, const dp_mc_confi
       DP_HIP(hipGetLastError());
       return DP_OK;
     };
-    static const std::vector<dph::TuneCfg> cands = {{16, 0, 0}, {4, 0, 0}, {1, 0, 0}, {2, 0, 0}, {8, 0, 0}};
+    static const std::vector<dph::TuneCfg> cands = {{64, 0, 0}, {32, 0, 0}, {16, 0, 0}, {8, 0, 0}, {4, 0, 0}};
     const std::string key = std::string("mc_flat_stochastic fast=") + (fast ? "1" : "0") +
                             " events=" + std::to_string(n);
     dph::TuneCfg cfg;
