@@ -354,6 +354,7 @@ __device__ bool air_segment(const WTab& T, double z0, double cos_t, double dista
 struct WArgs {
   const double* tab;
   int K, pow2, shared;
+  int air_shared;  // the air table is on the rock's energy grid
   int rock, water, air, porous;
   WMat mat;
   WCorr corr;
@@ -379,9 +380,14 @@ struct WArgs {
   int* err;
 };
 
-// Doubles of LDS table staging: (E, log E, S_csda) for rock, water, porous,
-// air, plus (S_mixed, Omega) for rock, water, porous with the corrections.
-__host__ __device__ constexpr int w_tab_cols(bool corr) { return corr ? 18 : 12; }
+// Columns (of K doubles) of LDS table staging: (E, log E, S_csda) for rock,
+// water, porous and air, plus (S_mixed, Omega) for rock, water and porous with
+// the corrections.  Materials on the rock's energy grid (sh: water and porous;
+// ash: air) use the rock's E and log E columns, so a block stages fewer
+// columns and more blocks fit in a CU's LDS.
+__host__ __device__ constexpr int w_tab_cols(bool corr, bool sh, bool ash) {
+  return 12 - (sh ? 4 : 0) - (ash ? 2 : 0) + (corr ? 6 : 0);
+}
 // Tape doubles per segment and lane.
 __host__ __device__ constexpr int w_tape(bool corr) { return corr ? 8 : 4; }
 
@@ -412,41 +418,59 @@ __global__ __launch_bounds__(256, (JAC && CORR) ? 2 : 1) void wcsda_kernel(WArgs
   const int wrow = PACK ? grp : wave;         // row this lane's wave (or half) adds to
   const int wl = PACK ? gt : lane;            // lane index within that wave (or half)
   constexpr int NT = w_tape(CORR);
-  double* wsum = lds + w_tab_cols(CORR) * K;              // [nwv][max_seg]
+  const bool sh = a.shared != 0, ash = a.air_shared != 0;
+  double* wsum = lds + w_tab_cols(CORR, sh, ash) * K;     // [nwv][max_seg]
   double* wflux = wsum + (size_t)nwv * a.max_seg;         // [nwv]
   int* scell = (int*)(wflux + nwv) + grp * a.max_seg;     // [slots][max_seg]
   __shared__ int s_ray[2];
   __shared__ double s_G[2], s_M[2], s_q[2], s_mass[2];    // per-bin multipliers (CORR)
+  // column of each (material, quantity); a material on the rock's grid points
+  // its E and log E at the rock's columns
+  int ce[4], cle[4], cs[4], cm[3], co[3];
+  {
+    int nc = 0;
+    for (int j = 0; j < 4; ++j) {
+      const bool alias = j > 0 && (j < 3 ? sh : ash);
+      ce[j] = alias ? ce[0] : nc++;
+      cle[j] = alias ? cle[0] : nc++;
+      cs[j] = nc++;
+    }
+    for (int j = 0; j < 3; ++j) {
+      cm[j] = CORR ? nc++ : 0;
+      co[j] = CORR ? nc++ : 0;
+    }
+  }
   {
     const int mats[4] = {a.rock, a.water, a.porous >= 0 ? a.porous : a.rock, a.air};
     for (int j = 0; j < 4; ++j) {
       const double* t = a.tab + (size_t)mats[j] * 19 * K;
-      double* d = lds + 3 * j * K;
+      const bool own = j == 0 || ce[j] != ce[0];
       for (int i = tid; i < K; i += bd) {
-        d[i] = t[i];                 // energies
-        d[K + i] = t[K + i];         // log energies
-        d[2 * K + i] = t[2 * K + i]; // CSDA stopping power
+        if (own) {
+          lds[(size_t)ce[j] * K + i] = t[i];           // energies
+          lds[(size_t)cle[j] * K + i] = t[K + i];      // log energies
+        }
+        lds[(size_t)cs[j] * K + i] = t[2 * K + i];     // CSDA stopping power
       }
     }
     if (CORR) {
       for (int j = 0; j < 3; ++j) {
         const double* t = a.tab + (size_t)mats[j] * 19 * K;
-        double* d = lds + (12 + 2 * j) * K;
         for (int i = tid; i < K; i += bd) {
-          d[i] = t[5 * K + i];       // MIXED stopping power
-          d[K + i] = t[7 * K + i];   // straggling
+          lds[(size_t)cm[j] * K + i] = t[5 * K + i];   // MIXED stopping power
+          lds[(size_t)co[j] * K + i] = t[7 * K + i];   // straggling
         }
       }
     }
   }
   WTab T;
-  T.rE = lds; T.rLE = lds + K; T.rS = lds + 2 * K;
-  T.wE = lds + 3 * K; T.wLE = lds + 4 * K; T.wS = lds + 5 * K;
-  T.pE = lds + 6 * K; T.pLE = lds + 7 * K; T.pS = lds + 8 * K;
-  T.aE = lds + 9 * K; T.aLE = lds + 10 * K; T.aS = lds + 11 * K;
-  T.rM = lds + 12 * K; T.rO = lds + 13 * K;
-  T.wM = lds + 14 * K; T.wO = lds + 15 * K;
-  T.pM = lds + 16 * K; T.pO = lds + 17 * K;
+  T.rE = lds + ce[0] * K; T.rLE = lds + cle[0] * K; T.rS = lds + cs[0] * K;
+  T.wE = lds + ce[1] * K; T.wLE = lds + cle[1] * K; T.wS = lds + cs[1] * K;
+  T.pE = lds + ce[2] * K; T.pLE = lds + cle[2] * K; T.pS = lds + cs[2] * K;
+  T.aE = lds + ce[3] * K; T.aLE = lds + cle[3] * K; T.aS = lds + cs[3] * K;
+  T.rM = lds + cm[0] * K; T.rO = lds + co[0] * K;
+  T.wM = lds + cm[1] * K; T.wO = lds + co[1] * K;
+  T.pM = lds + cm[2] * K; T.pO = lds + co[2] * K;
   T.K = K; T.pow2 = a.pow2; T.shared = a.shared; T.mass = a.mass; T.ctau = a.ctau;
   const WMat M = a.mat;
   const WCorr C = a.corr;
@@ -731,8 +755,8 @@ __global__ void reset_counters_w(int* queue, int* err, int reset_err) {
   }
 }
 
-size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr, int slots) {
-  return ((size_t)w_tab_cols(corr) * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
+size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr, bool sh, bool ash, int slots) {
+  return ((size_t)w_tab_cols(corr, sh, ash) * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
          (size_t)slots * max_seg * sizeof(int);
 }
 
@@ -775,7 +799,11 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
   const int max_seg = p->max_segments > 0 ? p->max_segments : 1;
   const bool J = jac != nullptr;
   const bool CR = c->corr.enabled || c->mcorr.enabled;
-  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg, CR, PK ? 2 : 1);
+  const int porous = m->porous_material;
+  const bool sh = same_grid(t, m->rock_material, m->water_material) &&
+                  (porous < 0 || same_grid(t, m->rock_material, porous));
+  const bool ash = same_grid(t, m->rock_material, m->air_material);
+  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg, CR, sh, ash, PK ? 2 : 1);
   if (lds > 160 * 1024)
     return dph::fail(DP_E_CAPACITY, "dp_csda_w_flux_jacobian: max_segments " +
                                         std::to_string(max_seg) + " exceeds the LDS budget");
@@ -804,11 +832,11 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
   a.corr.ks = k.kappa_strag; a.corr.kh = k.kappa_hard;
   a.corr.ra = k.resid_a; a.corr.rb = k.resid_b; a.corr.rc = k.resid_c; a.corr.rd = k.resid_d;
   a.corr.lin = c->mcorr.linear; a.corr.quad = c->mcorr.quadratic;
-  const int porous = m->porous_material;
   a.tab = t->data; a.K = t->n_knots; a.pow2 = t->pow2;
   a.rock = m->rock_material; a.water = m->water_material; a.air = m->air_material;
   a.porous = porous;
-  a.shared = same_grid(t, a.rock, a.water) && (porous < 0 || same_grid(t, a.rock, porous));
+  a.shared = sh;
+  a.air_shared = ash;
   a.mat.rho_r = m->rock_density; a.mat.rho_w = m->water_density; a.mat.rho_p = m->porous_density;
   a.mat.porous_on = porous >= 0 && m->porous_top_thickness > 0.0;
   a.det_elev = site->detector_elevation; a.palt = site->primary_altitude;
