@@ -383,10 +383,11 @@ struct WArgs {
 // Columns (of K doubles) of LDS table staging: (E, log E, S_csda) for rock,
 // water, porous and air, plus (S_mixed, Omega) for rock, water and porous with
 // the corrections.  Materials on the rock's energy grid (sh: water and porous;
-// ash: air) use the rock's E and log E columns, so a block stages fewer
-// columns and more blocks fit in a CU's LDS.
-__host__ __device__ constexpr int w_tab_cols(bool corr, bool sh, bool ash) {
-  return 12 - (sh ? 4 : 0) - (ash ? 2 : 0) + (corr ? 6 : 0);
+// ash: air) use the rock's E and log E columns, and without a porous material
+// (pa) the porous columns are the rock's, so a block stages fewer columns and
+// more blocks fit in a CU's LDS.
+__host__ __device__ constexpr int w_tab_cols(bool corr, bool sh, bool ash, bool pa) {
+  return 3 + (sh ? 1 : 3) + (pa ? 0 : (sh ? 1 : 3)) + (ash ? 1 : 3) + (corr ? (pa ? 4 : 6) : 0);
 }
 // Tape doubles per segment and lane.
 __host__ __device__ constexpr int w_tape(bool corr) { return corr ? 8 : 4; }
@@ -418,26 +419,29 @@ __global__ __launch_bounds__(256, (JAC && CORR) ? 2 : 1) void wcsda_kernel(WArgs
   const int wrow = PACK ? grp : wave;         // row this lane's wave (or half) adds to
   const int wl = PACK ? gt : lane;            // lane index within that wave (or half)
   constexpr int NT = w_tape(CORR);
-  const bool sh = a.shared != 0, ash = a.air_shared != 0;
-  double* wsum = lds + w_tab_cols(CORR, sh, ash) * K;     // [nwv][max_seg]
+  const bool sh = a.shared != 0, ash = a.air_shared != 0, pa = a.porous < 0;
+  double* wsum = lds + w_tab_cols(CORR, sh, ash, pa) * K; // [nwv][max_seg]
   double* wflux = wsum + (size_t)nwv * a.max_seg;         // [nwv]
   int* scell = (int*)(wflux + nwv) + grp * a.max_seg;     // [slots][max_seg]
   __shared__ int s_ray[2];
   __shared__ double s_G[2], s_M[2], s_q[2], s_mass[2];    // per-bin multipliers (CORR)
   // column of each (material, quantity); a material on the rock's grid points
-  // its E and log E at the rock's columns
+  // its E and log E at the rock's columns, an absent porous material (staged
+  // as rock) every column
   int ce[4], cle[4], cs[4], cm[3], co[3];
   {
     int nc = 0;
     for (int j = 0; j < 4; ++j) {
-      const bool alias = j > 0 && (j < 3 ? sh : ash);
+      const bool same = j == 2 && pa;
+      const bool alias = same || (j > 0 && (j < 3 ? sh : ash));
       ce[j] = alias ? ce[0] : nc++;
       cle[j] = alias ? cle[0] : nc++;
-      cs[j] = nc++;
+      cs[j] = same ? cs[0] : nc++;
     }
     for (int j = 0; j < 3; ++j) {
-      cm[j] = CORR ? nc++ : 0;
-      co[j] = CORR ? nc++ : 0;
+      const bool same = j == 2 && pa;
+      cm[j] = !CORR ? 0 : same ? cm[0] : nc++;
+      co[j] = !CORR ? 0 : same ? co[0] : nc++;
     }
   }
   {
@@ -445,16 +449,17 @@ __global__ __launch_bounds__(256, (JAC && CORR) ? 2 : 1) void wcsda_kernel(WArgs
     for (int j = 0; j < 4; ++j) {
       const double* t = a.tab + (size_t)mats[j] * 19 * K;
       const bool own = j == 0 || ce[j] != ce[0];
+      const bool own_s = j == 0 || cs[j] != cs[0];
       for (int i = tid; i < K; i += bd) {
         if (own) {
           lds[(size_t)ce[j] * K + i] = t[i];           // energies
           lds[(size_t)cle[j] * K + i] = t[K + i];      // log energies
         }
-        lds[(size_t)cs[j] * K + i] = t[2 * K + i];     // CSDA stopping power
+        if (own_s) lds[(size_t)cs[j] * K + i] = t[2 * K + i];  // CSDA stopping power
       }
     }
     if (CORR) {
-      for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < (pa ? 2 : 3); ++j) {
         const double* t = a.tab + (size_t)mats[j] * 19 * K;
         for (int i = tid; i < K; i += bd) {
           lds[(size_t)cm[j] * K + i] = t[5 * K + i];   // MIXED stopping power
@@ -755,8 +760,9 @@ __global__ void reset_counters_w(int* queue, int* err, int reset_err) {
   }
 }
 
-size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr, bool sh, bool ash, int slots) {
-  return ((size_t)w_tab_cols(corr, sh, ash) * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
+size_t w_lds_bytes(int K, int nwv, int max_seg, bool corr, bool sh, bool ash, bool pa,
+                   int slots) {
+  return ((size_t)w_tab_cols(corr, sh, ash, pa) * K + (size_t)nwv * max_seg + nwv) * sizeof(double) +
          (size_t)slots * max_seg * sizeof(int);
 }
 
@@ -803,7 +809,7 @@ int launch_w(dp_ctx c, dp_tables t, const dp_material_config* m, const dp_site_c
   const bool sh = same_grid(t, m->rock_material, m->water_material) &&
                   (porous < 0 || same_grid(t, m->rock_material, porous));
   const bool ash = same_grid(t, m->rock_material, m->air_material);
-  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg, CR, sh, ash, PK ? 2 : 1);
+  const size_t lds = w_lds_bytes(t->n_knots, nwv, max_seg, CR, sh, ash, porous < 0, PK ? 2 : 1);
   if (lds > 160 * 1024)
     return dph::fail(DP_E_CAPACITY, "dp_csda_w_flux_jacobian: max_segments " +
                                         std::to_string(max_seg) + " exceeds the LDS budget");
