@@ -54,11 +54,11 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy), from the counter
 # runs summarised in profiles/; `traffic` = this x the units of one launch.
 MEASURED_BYTES_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 269087.0,    # per event
+    "mc_flat_stochastic_kernel": 270402.0,    # per event
     "csda_cube_kernel": 8847.8,               # per ray-voxel (all 256 nodes)
     "csda_cube_kernel_fp32": 4701.2,          # per ray-voxel (FP32 stepping)
-    "wcsda_kernel": 2125.4,                   # per ray-voxel (all 32 nodes)
-    "wcsda_kernel_corrected": 4312.8,         # per ray-voxel (all 32 nodes)
+    "wcsda_kernel": 2121.7,                   # per ray-voxel (all 32 nodes)
+    "wcsda_kernel_corrected": 4299.2,         # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 253284.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
     "trace_cube_kernel": 13.3,                # per ray-voxel (fill launch)
@@ -76,11 +76,11 @@ VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 # in profiles/).  The kernels are FP64 VALU bound rather than HBM bound; the
 # `valu` object prices the live kernel time against VALU_PEAK_WAVE_INSTS.
 MEASURED_VALU_PER_UNIT = {
-    "mc_flat_stochastic_kernel": 26531.5,     # per event
+    "mc_flat_stochastic_kernel": 26415.9,     # per event
     "csda_cube_kernel": 938.5,                # per ray-voxel (all 256 nodes)
     "csda_cube_kernel_fp32": 1050.9,          # per ray-voxel (FP32 stepping)
-    "wcsda_kernel": 6701.4,                   # per ray-voxel (all 32 nodes)
-    "wcsda_kernel_corrected": 7010.1,         # per ray-voxel (all 32 nodes)
+    "wcsda_kernel": 6814.6,                   # per ray-voxel (all 32 nodes)
+    "wcsda_kernel_corrected": 7065.9,         # per ray-voxel (all 32 nodes)
     "path_mc_kernel": 13906.2,                # per event
     "csda_direct_kernel": 36.02,              # per sample
     "trace_cube_kernel": 11.97,               # per ray-voxel (fill launch)
