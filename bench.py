@@ -59,7 +59,7 @@ MEASURED_BYTES_PER_UNIT = {
     "csda_cube_kernel_fp32": 4701.2,          # per ray-voxel (FP32 stepping)
     "wcsda_kernel": 2121.7,                   # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 4299.2,         # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 253284.0,               # per event
+    "path_mc_kernel": 255027.0,               # per event
     "csda_direct_kernel": 56.0,               # per sample
     "trace_cube_kernel": 13.3,                # per ray-voxel (fill launch)
     "ne_normal_kernel": 11.0,                 # per entry of the dense normal matrix
@@ -81,7 +81,7 @@ MEASURED_VALU_PER_UNIT = {
     "csda_cube_kernel_fp32": 1050.9,          # per ray-voxel (FP32 stepping)
     "wcsda_kernel": 6814.6,                   # per ray-voxel (all 32 nodes)
     "wcsda_kernel_corrected": 7065.9,         # per ray-voxel (all 32 nodes)
-    "path_mc_kernel": 13906.2,                # per event
+    "path_mc_kernel": 14097.0,                # per event
     "csda_direct_kernel": 36.02,              # per sample
     "trace_cube_kernel": 11.97,               # per ray-voxel (fill launch)
     "ray_exit_kernel": 318.4,                 # per ray
